@@ -1,0 +1,144 @@
+/*
+ * slate_amd — C ABI of the MI355X-native factor-and-solve library.
+ *
+ * All matrix arguments are DEVICE pointers to column-major storage with a
+ * leading dimension, i.e. the memory that slate::Matrix<T>::fromLAPACK /
+ * fromDevices describes: tile (i, j) of an nb-blocked matrix is the nb x nb
+ * view at A + i*nb + j*nb*lda.  Pivots are HOST arrays of int64 in LAPACK
+ * convention (1-based global row indices), the form lapack_api/lapack_gesv.cc
+ * :90-100 converts slate::Pivots into.
+ *
+ * Return value of every solver entry point: >= 0 is SLATE's `info`
+ * (0 = success, i > 0 = U(i,i) exactly zero / leading minor i not positive
+ * definite); < 0 is an error code, with a message in slate_amd_last_error().
+ * No C++ exception crosses this boundary.  Every entry point is synchronous:
+ * it waits for the device work it issued before returning.
+ */
+#ifndef SLATE_AMD_H
+#define SLATE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slate_amd_context_t* slate_amd_context;
+
+/* Factorization precision of the mixed-precision drivers. */
+enum {
+    SLATE_AMD_PREC_FP32 = 0,   /* float factor (SLATE's scalar_lo = float)       */
+    SLATE_AMD_PREC_BF16 = 1,   /* float storage, bf16 MFMA updates, fp32 accum   */
+    SLATE_AMD_PREC_FP64 = 3    /* no refinement: plain FP64 factor and solve     */
+};
+
+enum { SLATE_AMD_LOWER = 0, SLATE_AMD_UPPER = 1 };   /* slate::Uplo                */
+enum { SLATE_AMD_NOTRANS = 0, SLATE_AMD_TRANS = 1 }; /* slate::Op                  */
+
+/* Mirrors the slate::Option values used on this path (include/slate/enums.hh
+ * :453-490, include/slate/types.hh:241-265).  A field left at its
+ * slate_amd_options_init() value takes SLATE's default. */
+typedef struct {
+    int64_t block_size;        /* nb; 0 = choose automatically                   */
+    int64_t inner_blocking;    /* Option::InnerBlocking: panel leaf width, 0=auto */
+    int64_t lookahead;         /* Option::Lookahead (default 1)                  */
+    double  pivot_threshold;   /* Option::PivotThreshold (default 1.0)           */
+    int64_t max_iterations;    /* Option::MaxIterations (default 30)             */
+    double  tolerance;         /* Option::Tolerance; <= 0 -> eps * sqrt(n)       */
+    int32_t use_fallback;      /* Option::UseFallbackSolver (default 1)          */
+    int32_t factor_precision;  /* SLATE_AMD_PREC_* for the mixed drivers         */
+} slate_amd_options;
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates the per-device state (two HIP streams, workspaces).  No counterpart
+ * in the reference: SLATE keeps this state in MatrixStorage (queues,
+ * include/slate/internal/MatrixStorage.hh:574-590) and the Memory pool
+ * (src/core/Memory.cc:17-140). */
+int  slate_amd_init(slate_amd_context* ctx, int device);
+int  slate_amd_finalize(slate_amd_context ctx);
+const char* slate_amd_last_error(void);
+void slate_amd_options_init(slate_amd_options* opts);
+/* Device milliseconds of the last solver call (for benchmarking). */
+double slate_amd_last_time_ms(slate_amd_context ctx);
+
+/* ---- LU ----------------------------------------------------------------- */
+/* slate::getrf(Matrix<T>& A, Pivots&, Options) — include/slate/slate.hh:574-577,
+ * src/getrf.cc:319-356.  Square n x n only. */
+int64_t slate_amd_dgetrf(slate_amd_context ctx, int64_t n, double* dA, int64_t lda,
+                         int64_t* ipiv, const slate_amd_options* opts);
+int64_t slate_amd_sgetrf(slate_amd_context ctx, int64_t n, float* dA, int64_t lda,
+                         int64_t* ipiv, const slate_amd_options* opts);
+
+/* slate::getrs(A, Pivots, B, Options) — slate.hh:603-607, src/getrs.cc:66-117.
+ * trans = SLATE_AMD_TRANS solves A^T X = B (A.op() == Op::Trans). */
+int64_t slate_amd_dgetrs(slate_amd_context ctx, int trans, int64_t n, int64_t nrhs,
+                         const double* dA, int64_t lda, const int64_t* ipiv,
+                         double* dB, int64_t ldb, const slate_amd_options* opts);
+int64_t slate_amd_sgetrs(slate_amd_context ctx, int trans, int64_t n, int64_t nrhs,
+                         const float* dA, int64_t lda, const int64_t* ipiv,
+                         float* dB, int64_t ldb, const slate_amd_options* opts);
+
+/* slate::gesv(A, Pivots, B, Options) — slate.hh:504-508, src/gesv.cc:85-109. */
+int64_t slate_amd_dgesv(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                        int64_t* ipiv, double* dB, int64_t ldb, const slate_amd_options* opts);
+
+/* ---- Cholesky ----------------------------------------------------------- */
+/* slate::potrf(HermitianMatrix<T>&, Options) — slate.hh:714-717, src/potrf.cc:23-283. */
+int64_t slate_amd_dpotrf(slate_amd_context ctx, int uplo, int64_t n, double* dA, int64_t lda,
+                         const slate_amd_options* opts);
+int64_t slate_amd_spotrf(slate_amd_context ctx, int uplo, int64_t n, float* dA, int64_t lda,
+                         const slate_amd_options* opts);
+/* slate::potrs — slate.hh:741-745, src/potrs.cc:55-77. */
+int64_t slate_amd_dpotrs(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs,
+                         const double* dA, int64_t lda, double* dB, int64_t ldb,
+                         const slate_amd_options* opts);
+/* slate::posv — slate.hh:646-650, src/posv.cc:71-94. */
+int64_t slate_amd_dposv(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs, double* dA,
+                        int64_t lda, double* dB, int64_t ldb, const slate_amd_options* opts);
+
+/* ---- mixed precision + iterative refinement ------------------------------ */
+/* slate::gesv_mixed<double,float> — slate.hh:521-535, src/gesv_mixed.cc:106-290.
+ * A is left unchanged unless the FP64 fallback runs.  *iter as in SLATE:
+ * >= 0 iterations, -3 low-precision factor singular, -(itermax+1) no
+ * convergence. */
+int64_t slate_amd_dsgesv(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                         int64_t* ipiv, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                         int* iter, const slate_amd_options* opts);
+/* slate::gesv_mixed_gmres — slate.hh:539-553, src/gesv_mixed_gmres.cc:111-410 (nrhs = 1). */
+int64_t slate_amd_dsgesv_gmres(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA,
+                               int64_t lda, int64_t* ipiv, const double* dB, int64_t ldb,
+                               double* dX, int64_t ldx, int* iter, const slate_amd_options* opts);
+/* slate::posv_mixed — slate.hh:667-681, src/posv_mixed.cc:105-292. */
+int64_t slate_amd_dsposv(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs, double* dA,
+                         int64_t lda, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                         int* iter, const slate_amd_options* opts);
+/* slate::posv_mixed_gmres — slate.hh:687-701, src/posv_mixed_gmres.cc:110-410 (nrhs = 1). */
+int64_t slate_amd_dsposv_gmres(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs,
+                               double* dA, int64_t lda, const double* dB, int64_t ldb,
+                               double* dX, int64_t ldx, int* iter, const slate_amd_options* opts);
+
+/* ---- utilities on the path ---------------------------------------------- */
+/* slate::gemm(alpha, A, B, beta, C) for column-major device matrices,
+ * C (m x n) = alpha * op(A) * op(B) + beta * C, op = NoTrans only —
+ * include/slate/slate.hh (gemm), src/gemm.cc:11-105.  The MFMA kernel behind
+ * every trailing-matrix update of the factorizations. */
+int slate_amd_dgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, double alpha,
+                    const double* dA, int64_t lda, const double* dB, int64_t ldb, double beta,
+                    double* dC, int64_t ldc);
+int slate_amd_sgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, float alpha,
+                    const float* dA, int64_t lda, const float* dB, int64_t ldb, float beta,
+                    float* dC, int64_t ldc);
+
+/* slate::norm(Norm::Inf, A) — src/norm.cc:24-240; sym != 0: symmetric, lower stored. */
+int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double* dA, int64_t lda,
+                        double* result);
+/* matgen random entries (matgen/random.cc:202-231): dist 1 uniform [0,1),
+ * 2 uniform [-1,1) ("rands"), 3 normal ("randn"). */
+int slate_amd_generate(slate_amd_context ctx, int dist, int64_t seed, int64_t m, int64_t n,
+                       int64_t ioffset, int64_t joffset, double* dA, int64_t lda);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLATE_AMD_H */

@@ -1,0 +1,274 @@
+"""CPU oracle for the factor-and-solve path — TEST INFRASTRUCTURE ONLY.
+
+A ctypes wrapper over ``oracle/liboracle.so`` (built from ``slate_oracle.cc`` by
+``oracle/Makefile``), a plain CPU restatement of SLATE's ``Target::HostTask``
+single-rank algorithms.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker (or as the timed CPU baseline).  Nothing in ``slate_amd/`` uses it.
+
+Parity status of each restated function (how each one is pinned):
+
+================  ==========================================  =====================================
+function          reference                                   pinned by
+================  ==========================================  =====================================
+generate          matgen/random.cc:53-127                     parity unpinned for exact bits (the
+                                                              reference needs BLAS++ headers absent
+                                                              here); distribution moments, tiling
+                                                              independence and the committed
+                                                              fixture guard regressions
+dgetrf/sgetrf     src/getrf.cc:23-245,                        LAPACK dgetrf (scipy/OpenBLAS) pivots
+                  src/internal/Tile_getrf.hh:159-447          (committed n=2000 fixture), brute-force
+                                                              long-double GEPP on tiny n, PA=LU
+                                                              invariant, info on exact-zero columns
+dgetrs            src/getrs.cc:66-117                         scipy lu_solve, residual bound, trans
+dpotrf/spotrf     src/potrf.cc:23-211                         LAPACK dpotrf (elementwise), info =
+                                                              first non-SPD minor
+dpotrs/dposv      src/potrs.cc:55-77, src/posv.cc:71-94       scipy cho_solve, residual bound
+dsgesv/dsposv     src/gesv_mixed.cc:106-290,                  backward error <= bound, iteration
+                  src/posv_mixed.cc:105-292                   codes (-3 singular, -(itermax+1)),
+                                                              fallback result equals plain solve
+dsgesv_gmres      src/gesv_mixed_gmres.cc:111-410             same, plus Arnoldi invariants via the
+                                                              GPU path; exact iteration counts are
+                                                              parity unpinned (rounding-dependent)
+rotg              blas::rotg (reference BLAS drotg)           closed form: c*a+s*b=r, -s*a+c*b=0
+norms/residual    src/norm.cc, src/colNorms.cc                numpy closed forms
+================  ==========================================  =====================================
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_int = ctypes.c_int
+c_p = ctypes.c_void_p
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        sigs = {
+            "oracle_generate": (None, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64, c_p, c_i64]),
+            "oracle_philox": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, c_p]),
+            "oracle_dgetrf": (c_i64, [c_i64, c_i64, c_p, c_i64, c_i64, c_i64, c_i64, c_dbl, c_p]),
+            "oracle_sgetrf": (c_i64, [c_i64, c_i64, c_p, c_i64, c_i64, c_i64, c_i64, c_dbl, c_p]),
+            "oracle_dgetrs": (None, [c_i64, c_i64, c_p, c_i64, c_p, c_p, c_i64, c_i64, c_int]),
+            "oracle_sgetrs": (None, [c_i64, c_i64, c_p, c_i64, c_p, c_p, c_i64, c_i64, c_int]),
+            "oracle_dgesv": (c_i64, [c_i64, c_i64, c_p, c_i64, c_p, c_p, c_i64, c_i64, c_i64, c_i64]),
+            "oracle_dpotrf": (c_i64, [c_i64, c_p, c_i64, c_i64]),
+            "oracle_spotrf": (c_i64, [c_i64, c_p, c_i64, c_i64]),
+            "oracle_dpotrs": (None, [c_i64, c_i64, c_p, c_i64, c_p, c_i64, c_i64]),
+            "oracle_dposv": (c_i64, [c_i64, c_i64, c_p, c_i64, c_p, c_i64, c_i64]),
+            "oracle_dsgesv": (c_i64, [c_i64, c_i64, c_p, c_i64, c_p, c_p, c_i64, c_p, c_i64,
+                                      c_i64, c_i64, c_i64, c_i64, c_dbl, c_int, c_p]),
+            "oracle_dsposv": (c_i64, [c_i64, c_i64, c_p, c_i64, c_p, c_i64, c_p, c_i64,
+                                      c_i64, c_i64, c_dbl, c_int, c_p]),
+            "oracle_dsgesv_gmres": (c_i64, [c_i64, c_p, c_i64, c_p, c_p, c_p, c_i64, c_i64,
+                                            c_i64, c_i64, c_dbl, c_int, c_p]),
+            "oracle_dsposv_gmres": (c_i64, [c_i64, c_p, c_i64, c_p, c_p, c_i64, c_i64, c_dbl,
+                                            c_int, c_p]),
+            "oracle_norm_inf": (c_dbl, [c_i64, c_i64, c_p, c_i64]),
+            "oracle_norm_inf_sy": (c_dbl, [c_i64, c_p, c_i64]),
+            "oracle_residual": (None, [c_i64, c_i64, c_p, c_i64, c_int, c_p, c_i64, c_p, c_i64,
+                                       c_p, c_i64]),
+            "oracle_rotg": (None, [c_p, c_p, c_p, c_p]),
+            "oracle_num_threads": (c_int, []),
+            "oracle_set_num_threads": (None, [c_int]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fmat(a, dtype=np.float64):
+    a = np.asarray(a, dtype=dtype)
+    return np.asfortranarray(a)
+
+
+DIST_UNIFORM, DIST_RANDS, DIST_NORMAL = 1, 2, 3
+
+
+def generate(dist, seed, m, n, ioff=0, joff=0):
+    """matgen entry (i, j) = f(seed, i+ioff, j+joff); returns an m x n Fortran array."""
+    A = np.empty((m, n), dtype=np.float64, order="F")
+    lib().oracle_generate(dist, seed, m, n, ioff, joff, _ptr(A), max(m, 1))
+    return A
+
+
+def rands(seed, m, n):
+    return generate(DIST_RANDS, seed, m, n)
+
+
+def randn(seed, m, n):
+    return generate(DIST_NORMAL, seed, m, n)
+
+
+def philox(s0, s1, seed):
+    out = np.zeros(2, dtype=np.uint64)
+    lib().oracle_philox(s0, s1, seed, _ptr(out))
+    return int(out[0]), int(out[1])
+
+
+def getrf(A, nb=256, ib=16, nthreads=1, threshold=1.0, dtype=np.float64):
+    """Tile LU; returns (LU, ipiv 1-based int64, info)."""
+    A = _fmat(A, dtype).copy(order="F")
+    m, n = A.shape
+    ipiv = np.zeros(min(m, n), dtype=np.int64)
+    f = lib().oracle_dgetrf if dtype == np.float64 else lib().oracle_sgetrf
+    info = f(m, n, _ptr(A), max(m, 1), nb, ib, nthreads, threshold, _ptr(ipiv))
+    return A, ipiv, int(info)
+
+
+def getrs(LU, ipiv, B, nb=256, trans=False):
+    dtype = LU.dtype
+    LU = _fmat(LU, dtype)
+    B = _fmat(B, dtype).copy(order="F")
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n = LU.shape[0]
+    f = lib().oracle_dgetrs if dtype == np.float64 else lib().oracle_sgetrs
+    f(n, B.shape[1], _ptr(LU), max(n, 1), _ptr(np.ascontiguousarray(ipiv, dtype=np.int64)),
+      _ptr(B), max(n, 1), nb, int(bool(trans)))
+    return B
+
+
+def gesv(A, B, nb=256, ib=16, nthreads=1):
+    A = _fmat(A).copy(order="F")
+    B = _fmat(B).copy(order="F")
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n = A.shape[0]
+    ipiv = np.zeros(n, dtype=np.int64)
+    info = lib().oracle_dgesv(n, B.shape[1], _ptr(A), max(n, 1), _ptr(ipiv), _ptr(B),
+                              max(n, 1), nb, ib, nthreads)
+    return A, ipiv, B, int(info)
+
+
+def potrf(A, nb=256, dtype=np.float64):
+    """Lower tile Cholesky; returns (L in the lower triangle, info)."""
+    A = _fmat(A, dtype).copy(order="F")
+    n = A.shape[0]
+    f = lib().oracle_dpotrf if dtype == np.float64 else lib().oracle_spotrf
+    info = f(n, _ptr(A), max(n, 1), nb)
+    return A, int(info)
+
+
+def potrs(L, B, nb=256):
+    L = _fmat(L)
+    B = _fmat(B).copy(order="F")
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n = L.shape[0]
+    lib().oracle_dpotrs(n, B.shape[1], _ptr(L), max(n, 1), _ptr(B), max(n, 1), nb)
+    return B
+
+
+def posv(A, B, nb=256):
+    A = _fmat(A).copy(order="F")
+    B = _fmat(B).copy(order="F")
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n = A.shape[0]
+    info = lib().oracle_dposv(n, B.shape[1], _ptr(A), max(n, 1), _ptr(B), max(n, 1), nb)
+    return A, B, int(info)
+
+
+def gesv_mixed(A, B, nb=256, ib=16, nthreads=1, itermax=30, tol=0.0, use_fallback=True):
+    """Classic IR with a float factor; returns (A_out, ipiv, X, info, iter)."""
+    A = _fmat(A).copy(order="F")
+    B = _fmat(B)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n, nrhs = B.shape
+    X = np.zeros((n, nrhs), order="F")
+    ipiv = np.zeros(n, dtype=np.int64)
+    it = ctypes.c_int(0)
+    info = lib().oracle_dsgesv(n, nrhs, _ptr(A), n, _ptr(ipiv), _ptr(B), n, _ptr(X), n, nb, ib,
+                               nthreads, itermax, tol, int(use_fallback), ctypes.byref(it))
+    return A, ipiv, X, int(info), it.value
+
+
+def posv_mixed(A, B, nb=256, itermax=30, tol=0.0, use_fallback=True):
+    A = _fmat(A).copy(order="F")
+    B = _fmat(B)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n, nrhs = B.shape
+    X = np.zeros((n, nrhs), order="F")
+    it = ctypes.c_int(0)
+    info = lib().oracle_dsposv(n, nrhs, _ptr(A), n, _ptr(B), n, _ptr(X), n, nb, itermax, tol,
+                               int(use_fallback), ctypes.byref(it))
+    return A, X, int(info), it.value
+
+
+def gesv_mixed_gmres(A, b, nb=256, ib=16, nthreads=1, itermax=30, tol=0.0, use_fallback=True):
+    A = _fmat(A).copy(order="F")
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.float64).reshape(-1))
+    n = A.shape[0]
+    x = np.zeros(n)
+    ipiv = np.zeros(n, dtype=np.int64)
+    it = ctypes.c_int(0)
+    info = lib().oracle_dsgesv_gmres(n, _ptr(A), n, _ptr(ipiv), _ptr(b), _ptr(x), nb, ib,
+                                     nthreads, itermax, tol, int(use_fallback), ctypes.byref(it))
+    return A, ipiv, x, int(info), it.value
+
+
+def posv_mixed_gmres(A, b, nb=256, itermax=30, tol=0.0, use_fallback=True):
+    A = _fmat(A).copy(order="F")
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.float64).reshape(-1))
+    n = A.shape[0]
+    x = np.zeros(n)
+    it = ctypes.c_int(0)
+    info = lib().oracle_dsposv_gmres(n, _ptr(A), n, _ptr(b), _ptr(x), nb, itermax, tol,
+                                     int(use_fallback), ctypes.byref(it))
+    return A, x, int(info), it.value
+
+
+def norm_inf(A):
+    A = _fmat(A)
+    return lib().oracle_norm_inf(A.shape[0], A.shape[1], _ptr(A), max(A.shape[0], 1))
+
+
+def rotg(a, b):
+    va, vb, vc, vs = (ctypes.c_double(a), ctypes.c_double(b), ctypes.c_double(0), ctypes.c_double(0))
+    lib().oracle_rotg(ctypes.byref(va), ctypes.byref(vb), ctypes.byref(vc), ctypes.byref(vs))
+    return va.value, vb.value, vc.value, vs.value
+
+
+def scaled_residual(A, X, B):
+    """Tester check of test/test_gesv.cc:330-377: ||B-AX||_1 / (n ||A||_1 ||X||_1), in units of eps."""
+    A = np.asarray(A, dtype=np.float64)
+    X = np.asarray(X, dtype=np.float64).reshape(A.shape[0], -1)
+    B = np.asarray(B, dtype=np.float64).reshape(A.shape[0], -1)
+    n = A.shape[0]
+    R = B - A @ X
+    r1 = np.abs(R).sum(axis=0).max()
+    a1 = np.abs(A).sum(axis=0).max()
+    x1 = np.abs(X).sum(axis=0).max()
+    return float(r1 / (n * a1 * x1) / np.finfo(np.float64).eps)
+
+
+def ipiv_from_lapack(ipiv_lapack):
+    """LAPACK 1-based int32 ipiv -> int64 array (same convention as the oracle)."""
+    return np.asarray(ipiv_lapack, dtype=np.int64) + 0

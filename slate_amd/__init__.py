@@ -1,0 +1,308 @@
+"""slate_amd — MI355X-native single-GPU factor-and-solve (getrf/getrs/gesv,
+potrf/potrs/posv, gesv_mixed, gesv_mixed_gmres, posv_mixed, posv_mixed_gmres).
+
+Python binding over the C ABI declared in ``include/slate_amd.h`` and
+implemented by ``slate_amd/libslate_amd.so`` (hand-written HIP kernels for
+gfx950).  Device memory comes from torch tensors; matrices are column-major
+(``A.stride(0) == 1``), exactly the layout ``slate::Matrix::fromLAPACK`` and
+``fromDevices`` describe.  Pivots are LAPACK-style 1-based int64 arrays.
+
+The entry points follow SLATE's driver signatures (include/slate/slate.hh:
+503-745): they factor in place, return ``info`` (0, or the 1-based index of a
+zero pivot / non-SPD minor), and for the mixed drivers also ``iter``.  Errors
+raise ``SlateError``; there is no CPU fallback — if the library is missing or
+no GPU is present, calls fail loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libslate_amd.so")
+
+PREC_FP32, PREC_BF16, PREC_FP64 = 0, 1, 3
+LOWER, UPPER = 0, 1
+NOTRANS, TRANS = 0, 1
+DIST_UNIFORM, DIST_RANDS, DIST_RANDN = 1, 2, 3
+
+# Public C symbols (the C-ABI test checks that the library exports each one).
+C_SYMBOLS = [
+    "slate_amd_init", "slate_amd_finalize", "slate_amd_last_error", "slate_amd_options_init",
+    "slate_amd_last_time_ms",
+    "slate_amd_dgetrf", "slate_amd_sgetrf", "slate_amd_dgetrs", "slate_amd_sgetrs",
+    "slate_amd_dgesv", "slate_amd_dpotrf", "slate_amd_spotrf", "slate_amd_dpotrs",
+    "slate_amd_dposv", "slate_amd_dsgesv", "slate_amd_dsgesv_gmres", "slate_amd_dsposv",
+    "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate",
+    "slate_amd_dgemm", "slate_amd_sgemm",
+]
+
+
+class SlateError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    """Mirror of ``slate_amd_options`` (the slate::Option values of the path)."""
+    _fields_ = [
+        ("block_size", ctypes.c_int64),
+        ("inner_blocking", ctypes.c_int64),
+        ("lookahead", ctypes.c_int64),
+        ("pivot_threshold", ctypes.c_double),
+        ("max_iterations", ctypes.c_int64),
+        ("tolerance", ctypes.c_double),
+        ("use_fallback", ctypes.c_int32),
+        ("factor_precision", ctypes.c_int32),
+    ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().slate_amd_options_init(ctypes.byref(self))
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise TypeError("unknown option %r" % k)
+            setattr(self, k, v)
+
+
+_LIB = None
+_CTX = {}
+
+
+def lib():
+    """Load libslate_amd.so (built by slate_amd/build.py); raise if absent."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise SlateError("libslate_amd.so not built (run `python -m slate_amd.build`)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        pi64 = ctypes.POINTER(ctypes.c_int64)
+        pint = ctypes.POINTER(ctypes.c_int)
+        po = ctypes.POINTER(Options)
+        sig = {
+            "slate_amd_init": (i32, [ctypes.POINTER(vp), i32]),
+            "slate_amd_finalize": (i32, [vp]),
+            "slate_amd_last_error": (ctypes.c_char_p, []),
+            "slate_amd_options_init": (None, [po]),
+            "slate_amd_last_time_ms": (dbl, [vp]),
+            "slate_amd_dgetrf": (i64, [vp, i64, vp, i64, pi64, po]),
+            "slate_amd_sgetrf": (i64, [vp, i64, vp, i64, pi64, po]),
+            "slate_amd_dgetrs": (i64, [vp, i32, i64, i64, vp, i64, pi64, vp, i64, po]),
+            "slate_amd_sgetrs": (i64, [vp, i32, i64, i64, vp, i64, pi64, vp, i64, po]),
+            "slate_amd_dgesv": (i64, [vp, i64, i64, vp, i64, pi64, vp, i64, po]),
+            "slate_amd_dpotrf": (i64, [vp, i32, i64, vp, i64, po]),
+            "slate_amd_spotrf": (i64, [vp, i32, i64, vp, i64, po]),
+            "slate_amd_dpotrs": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, po]),
+            "slate_amd_dposv": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, po]),
+            "slate_amd_dsgesv": (i64, [vp, i64, i64, vp, i64, pi64, vp, i64, vp, i64, pint, po]),
+            "slate_amd_dsgesv_gmres": (i64, [vp, i64, i64, vp, i64, pi64, vp, i64, vp, i64, pint, po]),
+            "slate_amd_dsposv": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, vp, i64, pint, po]),
+            "slate_amd_dsposv_gmres": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, vp, i64, pint, po]),
+            "slate_amd_dnorm_inf": (i32, [vp, i32, i64, vp, i64, ctypes.POINTER(dbl)]),
+            "slate_amd_generate": (i32, [vp, i32, i64, i64, i64, i64, i64, vp, i64]),
+            "slate_amd_dgemm": (i32, [vp, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
+            "slate_amd_sgemm": (i32, [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
+                                      ctypes.c_float, vp, i64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def context(device=0):
+    """Per-device library context (streams + workspaces), created on first use."""
+    if device not in _CTX:
+        h = ctypes.c_void_p()
+        rc = lib().slate_amd_init(ctypes.byref(h), device)
+        if rc != 0:
+            raise SlateError("slate_amd_init: " + last_error())
+        _CTX[device] = h
+    return _CTX[device]
+
+
+def last_error():
+    return lib().slate_amd_last_error().decode()
+
+
+def last_time_ms(device=0):
+    return lib().slate_amd_last_time_ms(context(device))
+
+
+def _check(rc):
+    if rc < 0:
+        raise SlateError("slate_amd error %d: %s" % (rc, last_error()))
+    return int(rc)
+
+
+def _mat(A, dtypes=("float64",)):
+    """Return (ptr, lda, device) for a column-major CUDA matrix tensor."""
+    import torch
+    if not isinstance(A, torch.Tensor) or not A.is_cuda:
+        raise SlateError("expected a CUDA tensor")
+    if str(A.dtype).replace("torch.", "") not in dtypes:
+        raise SlateError("unsupported dtype %s" % A.dtype)
+    if A.dim() == 1:
+        return A.data_ptr(), max(1, A.shape[0]), A.device.index or 0
+    if A.stride(0) != 1 and A.shape[0] > 1:
+        raise SlateError("matrix must be column-major (stride(0) == 1)")
+    lda = A.stride(1) if A.shape[1] > 1 else max(1, A.shape[0])
+    return A.data_ptr(), max(lda, 1), A.device.index or 0
+
+
+def _ncols(B):
+    return 1 if B.dim() == 1 else B.shape[1]
+
+
+def colmajor(t):
+    """Column-major copy of a 2-D tensor (same device, same dtype)."""
+    return t.t().contiguous().t()
+
+
+def empty_colmajor(m, n, dtype, device="cuda", ld=None):
+    import torch
+    ld = ld or m
+    return torch.empty((n, ld), dtype=dtype, device=device).t()[:m]
+
+
+def _opts(opts):
+    return ctypes.byref(opts if opts is not None else Options())
+
+
+def _ipiv_out(n):
+    return np.zeros(max(n, 1), dtype=np.int64)
+
+
+def _p64(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+#---------------------------------------------------------------- LU
+def getrf(A, opts=None):
+    """slate::getrf: A = P L U in place; returns (info, ipiv 1-based)."""
+    ptr, lda, dev = _mat(A, ("float64", "float32"))
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise SlateError("getrf: square matrices only")
+    ipiv = _ipiv_out(n)
+    f = lib().slate_amd_dgetrf if A.dtype.itemsize == 8 else lib().slate_amd_sgetrf
+    info = _check(f(context(dev), n, ptr, lda, _p64(ipiv), _opts(opts)))
+    return info, ipiv[:n]
+
+
+def getrs(LU, ipiv, B, trans=False, opts=None):
+    """slate::getrs: B := op(A)^{-1} B using the factors of getrf."""
+    pa, lda, dev = _mat(LU, ("float64", "float32"))
+    pb, ldb, _ = _mat(B, ("float64", "float32"))
+    n = LU.shape[0]
+    ip = np.ascontiguousarray(ipiv, dtype=np.int64)
+    f = lib().slate_amd_dgetrs if LU.dtype.itemsize == 8 else lib().slate_amd_sgetrs
+    _check(f(context(dev), TRANS if trans else NOTRANS, n, _ncols(B), pa, lda, _p64(ip), pb, ldb,
+             _opts(opts)))
+
+
+def gesv(A, B, opts=None):
+    """slate::gesv: factor A and overwrite B with the solution; returns (info, ipiv)."""
+    pa, lda, dev = _mat(A)
+    pb, ldb, _ = _mat(B)
+    n = A.shape[0]
+    ipiv = _ipiv_out(n)
+    info = _check(lib().slate_amd_dgesv(context(dev), n, _ncols(B), pa, lda, _p64(ipiv), pb, ldb,
+                                        _opts(opts)))
+    return info, ipiv[:n]
+
+
+#---------------------------------------------------------------- Cholesky
+def potrf(A, uplo=LOWER, opts=None):
+    """slate::potrf on a HermitianMatrix stored in the `uplo` triangle; returns info."""
+    pa, lda, dev = _mat(A, ("float64", "float32"))
+    f = lib().slate_amd_dpotrf if A.dtype.itemsize == 8 else lib().slate_amd_spotrf
+    return _check(f(context(dev), uplo, A.shape[0], pa, lda, _opts(opts)))
+
+
+def potrs(L, B, uplo=LOWER, opts=None):
+    pa, lda, dev = _mat(L)
+    pb, ldb, _ = _mat(B)
+    _check(lib().slate_amd_dpotrs(context(dev), uplo, L.shape[0], _ncols(B), pa, lda, pb, ldb,
+                                  _opts(opts)))
+
+
+def posv(A, B, uplo=LOWER, opts=None):
+    pa, lda, dev = _mat(A)
+    pb, ldb, _ = _mat(B)
+    return _check(lib().slate_amd_dposv(context(dev), uplo, A.shape[0], _ncols(B), pa, lda, pb, ldb,
+                                        _opts(opts)))
+
+
+#---------------------------------------------------------------- mixed precision
+def gesv_mixed(A, B, X, opts=None):
+    """slate::gesv_mixed: returns (info, iter, ipiv); X receives the solution."""
+    pa, lda, dev = _mat(A)
+    pb, ldb, _ = _mat(B)
+    px, ldx, _ = _mat(X)
+    n = A.shape[0]
+    ipiv = _ipiv_out(n)
+    it = ctypes.c_int(0)
+    info = _check(lib().slate_amd_dsgesv(context(dev), n, _ncols(B), pa, lda, _p64(ipiv), pb, ldb,
+                                         px, ldx, ctypes.byref(it), _opts(opts)))
+    return info, it.value, ipiv[:n]
+
+
+def gesv_mixed_gmres(A, B, X, opts=None):
+    pa, lda, dev = _mat(A)
+    pb, ldb, _ = _mat(B)
+    px, ldx, _ = _mat(X)
+    n = A.shape[0]
+    ipiv = _ipiv_out(n)
+    it = ctypes.c_int(0)
+    info = _check(lib().slate_amd_dsgesv_gmres(context(dev), n, _ncols(B), pa, lda, _p64(ipiv), pb,
+                                               ldb, px, ldx, ctypes.byref(it), _opts(opts)))
+    return info, it.value, ipiv[:n]
+
+
+def posv_mixed(A, B, X, uplo=LOWER, opts=None):
+    pa, lda, dev = _mat(A)
+    pb, ldb, _ = _mat(B)
+    px, ldx, _ = _mat(X)
+    it = ctypes.c_int(0)
+    info = _check(lib().slate_amd_dsposv(context(dev), uplo, A.shape[0], _ncols(B), pa, lda, pb, ldb,
+                                         px, ldx, ctypes.byref(it), _opts(opts)))
+    return info, it.value
+
+
+def posv_mixed_gmres(A, B, X, uplo=LOWER, opts=None):
+    pa, lda, dev = _mat(A)
+    pb, ldb, _ = _mat(B)
+    px, ldx, _ = _mat(X)
+    it = ctypes.c_int(0)
+    info = _check(lib().slate_amd_dsposv_gmres(context(dev), uplo, A.shape[0], _ncols(B), pa, lda,
+                                               pb, ldb, px, ldx, ctypes.byref(it), _opts(opts)))
+    return info, it.value
+
+
+#---------------------------------------------------------------- utilities
+def gemm(alpha, A, B, beta, C):
+    """slate::gemm: C = alpha A B + beta C (column-major CUDA matrices)."""
+    pa, lda, dev = _mat(A, ("float64", "float32"))
+    pb, ldb, _ = _mat(B, ("float64", "float32"))
+    pc, ldc, _ = _mat(C, ("float64", "float32"))
+    f = lib().slate_amd_dgemm if C.dtype.itemsize == 8 else lib().slate_amd_sgemm
+    _check(f(context(dev), C.shape[0], _ncols(C), A.shape[1], alpha, pa, lda, pb, ldb, beta, pc, ldc))
+
+
+def norm_inf(A, sym=False):
+    pa, lda, dev = _mat(A)
+    out = ctypes.c_double(0)
+    _check(lib().slate_amd_dnorm_inf(context(dev), int(sym), A.shape[0], pa, lda, ctypes.byref(out)))
+    return out.value
+
+
+def generate(A, dist, seed, ioffset=0, joffset=0):
+    """Fill a column-major float64 CUDA matrix with matgen Philox entries."""
+    pa, lda, dev = _mat(A)
+    m = A.shape[0]
+    n = 1 if A.dim() == 1 else A.shape[1]
+    _check(lib().slate_amd_generate(context(dev), dist, seed, m, n, ioffset, joffset, pa, lda))
+    return A

@@ -1,0 +1,1155 @@
+// Host drivers of the factor-and-solve path and the C ABI (include/slate_amd.h).
+//
+// The factorizations run on a row-major view of the matrix (element (i, j) at
+// A[i*ld + j]), which keeps the pivot row interchanges coalesced; the caller's
+// column-major matrix is flipped in place on entry and flipped back on exit,
+// as SLATE does tile by tile for its GPU LU (src/getrf.cc:54-55, :238).
+// For a symmetric matrix the row-major view of the stored lower triangle is
+// an upper triangle, so potrf computes A = U^T U with U = L^T in place.
+#include "common.hh"
+#include "kernels.hh"
+#include "../../include/slate_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace sa {
+
+//------------------------------------------------------------------------------
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        if (p) SA_HIP(hipFree(p));
+        p = nullptr; bytes = 0;
+        SA_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    template <class T> T* as() { return static_cast<T*>(p); }
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t s0 = nullptr;          // trailing updates, solves
+    hipStream_t s1 = nullptr;          // panel + lookahead column
+    hipEvent_t t_start = nullptr, t_stop = nullptr;
+    std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
+    Buf counters, slots, err, dinfo, plan0, plan1, ipiv, work, lo, vec, basis, rhs;
+    double last_ms = 0;
+    int64_t counter_next = 0, counter_cap = 0;
+};
+
+struct Opts {
+    int64_t nb = 0, ib = 0, lookahead = 1, itermax = 30;
+    double threshold = 1.0, tol = 0;
+    bool fallback = true;
+    int prec = SLATE_AMD_PREC_FP32;
+};
+
+static Opts read_opts(const slate_amd_options* o)
+{
+    Opts r;
+    if (!o) return r;
+    r.nb = o->block_size;
+    r.ib = o->inner_blocking;
+    r.lookahead = o->lookahead;
+    r.threshold = o->pivot_threshold;
+    r.itermax = o->max_iterations;
+    r.tol = o->tolerance;
+    r.fallback = o->use_fallback != 0;
+    r.prec = o->factor_precision;
+    return r;
+}
+
+static int64_t choose_nb(int64_t n, int64_t requested)
+{
+    if (requested > 0) return requested;
+    if (n <= 2048) return 128;
+    if (n <= 16384) return 256;
+    return 512;
+}
+
+static void prepare_panel_work(Context& c, int64_t n, int64_t W)
+{
+    int64_t leaves = ceildiv(n, W) * 2 + 64;
+    if (c.counter_cap < leaves) {
+        c.counters.ensure(size_t(leaves) * sizeof(unsigned));
+        c.counter_cap = leaves;
+    }
+    zero_u32(c.counters.as<unsigned>(), leaves, c.s0);
+    c.counter_next = 0;
+    c.slots.ensure(size_t(1) << 22);
+    c.err.ensure(sizeof(int) * 4);
+    SA_HIP(hipMemsetAsync(c.err.p, 0, sizeof(int) * 4, c.s0));
+}
+
+static PanelWork panel_work(Context& c)
+{
+    return PanelWork{c.counters.as<unsigned>(), c.slots.p, int64_t(c.slots.bytes), c.err.as<int>()};
+}
+
+static void check_kernel_error(Context& c)
+{
+    int e = 0;
+    SA_HIP(hipMemcpy(&e, c.err.p, sizeof(int), hipMemcpyDeviceToHost));
+    SA_REQUIRE(e == 0, -101, "panel kernel: cross-workgroup barrier timed out");
+}
+
+//------------------------------------------------------------------------------
+// Recursive triangular solve with an nb x nb triangle and an nb x N row-major
+// right-hand side; off-diagonal blocks go through the MFMA GEMM.
+// tri(i,k) = M[i*ld + k] (cm = 0) or M[i + k*ld] (cm = 1).
+template <class T>
+static void trsm_rec(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
+                     T* B, int64_t ldb, int64_t N, hipStream_t s)
+{
+    if (nb <= 0 || N <= 0) return;
+    constexpr int64_t L = TRSM_LEAF;
+    if (nb <= L) { trsm_leaf<T>(M, ld, cm, lower, unit, nb, B, ldb, N, s); return; }
+    int64_t n1 = ((nb / 2 + L - 1) / L) * L;
+    if (n1 >= nb) n1 = nb - L;
+    int64_t n2 = nb - n1;
+    const T* M11 = M;
+    const T* M22 = M + n1 * ld + n1;
+    T* B1 = B;
+    T* B2 = B + n1 * ldb;
+    GemmArgs g{};
+    g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
+    if (lower) {
+        trsm_rec<T>(M11, ld, cm, 1, unit, n1, B1, ldb, N, s);
+        // B2 -= tri(n1:, 0:n1) * B1
+        g.M = n2; g.N = N; g.K = n1;
+        g.A = cm ? static_cast<const void*>(M + n1) : static_cast<const void*>(M + n1 * ld);
+        g.lda = ld; g.B = B1; g.ldb = ldb; g.C = B2; g.ldc = ldb;
+        gemm<T>(g, cm ? A_MC : A_KC, B_NC, s);
+        trsm_rec<T>(M22, ld, cm, 1, unit, n2, B2, ldb, N, s);
+    }
+    else {
+        trsm_rec<T>(M22, ld, cm, 0, unit, n2, B2, ldb, N, s);
+        // B1 -= tri(0:n1, n1:) * B2
+        g.M = n1; g.N = N; g.K = n2;
+        g.A = cm ? static_cast<const void*>(M + n1 * ld) : static_cast<const void*>(M + n1);
+        g.lda = ld; g.B = B2; g.ldb = ldb; g.C = B1; g.ldc = ldb;
+        gemm<T>(g, cm ? A_MC : A_KC, B_NC, s);
+        trsm_rec<T>(M11, ld, cm, 0, unit, n1, B1, ldb, N, s);
+    }
+}
+
+//------------------------------------------------------------------------------
+// Recursive panel LU of rows [c0, n) x columns [c0, c0+w) of a row-major n x n
+// matrix (the panel's top row is the diagonal).  Leaves of width <= W use
+// the cross-workgroup kernel; the splits apply the left half's interchanges
+// to the right half, solve with its unit-lower block, update with the GEMM,
+// recurse, and bring the right half's interchanges back to the left half.
+template <class T>
+static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64_t w,
+                      int64_t* ipiv, int64_t* info, double thr, int64_t leaf_w,
+                      int64_t* plan, hipStream_t s)
+{
+    if (w <= leaf_w) {
+        panel_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, thr, panel_work(c), c.counter_next++, s);
+        SA_REQUIRE(c.counter_next <= c.counter_cap, -2, "panel: barrier counters exhausted");
+        return;
+    }
+    int64_t w1 = ((w / 2 + leaf_w - 1) / leaf_w) * leaf_w;
+    if (w1 >= w) w1 = w - leaf_w;
+    int64_t w2 = w - w1;
+    panel_rec<T>(c, A, ld, n, c0, w1, ipiv, info, thr, leaf_w, plan, s);
+    laswp<T>(A, ld, n, c0 + w1, c0 + w, c0, c0 + w1, ipiv, plan, s);
+    trsm_rec<T>(A + c0 * ld + c0, ld, 0, 1, 1, w1, A + c0 * ld + c0 + w1, ld, w2, s);
+    GemmArgs g{};
+    g.M = n - c0 - w1; g.N = w2; g.K = w1;
+    g.A = A + (c0 + w1) * ld + c0; g.lda = ld;
+    g.B = A + c0 * ld + c0 + w1; g.ldb = ld;
+    g.C = A + (c0 + w1) * ld + c0 + w1; g.ldc = ld;
+    g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
+    gemm<T>(g, A_KC, B_NC, s);
+    panel_rec<T>(c, A, ld, n, c0 + w1, w2, ipiv, info, thr, leaf_w, plan, s);
+    laswp<T>(A, ld, n, c0, c0 + w1, c0 + w1, c0 + w, ipiv, plan, s);
+}
+
+// Right-looking blocked LU of a row-major n x n matrix with lookahead 1 on a
+// second stream: the panel of step k+1 is factored on s1 while s0 applies
+// step k to the columns beyond k+1 (src/getrf.cc:85-235, with the OpenMP
+// task DAG turned into two streams and events).
+// ipiv: device, global 0-based rows; info: device int64 (zeroed by caller).
+template <class T>
+static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int64_t* ipiv,
+                     int64_t* info)
+{
+    const int64_t nb = choose_nb(n, o.nb);
+    int64_t leaf_w = panel_leaf_width(int(sizeof(T)));
+    if (o.ib > 0 && o.ib < leaf_w) leaf_w = o.ib;
+    prepare_panel_work(c, n, leaf_w);
+    c.plan0.ensure(sizeof(int64_t) * (4 * 2048 + 8));
+    c.plan1.ensure(sizeof(int64_t) * (4 * 2048 + 8));
+    int64_t* plan0 = c.plan0.as<int64_t>();
+    int64_t* plan1 = c.plan1.as<int64_t>();
+    const int64_t nsteps = ceildiv(n, nb);
+    const bool la = o.lookahead > 0 && nsteps > 2;
+    if (int64_t(c.evs.size()) < 2 * nsteps + 4) {
+        for (int64_t i = int64_t(c.evs.size()); i < 2 * nsteps + 4; ++i) {
+            hipEvent_t e;
+            SA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            c.evs.push_back(e);
+        }
+    }
+    hipStream_t s0 = c.s0, s1 = la ? c.s1 : c.s0;
+    // s1 must see the counter reset done on s0
+    if (la) {
+        SA_HIP(hipEventRecord(c.evs[0], s0));
+        SA_HIP(hipStreamWaitEvent(s1, c.evs[0], 0));
+    }
+    // ev_panel[k]: panel k factored and its column block complete (on s1)
+    // ev_trail[k]: step k-1 applied to every column block beyond k (on s0)
+    auto ev_panel = [&](int64_t k) { return c.evs[2 + 2 * k]; };
+    auto ev_trail = [&](int64_t k) { return c.evs[3 + 2 * k]; };
+
+    // panel 0
+    panel_rec<T>(c, A, ld, n, 0, std::min(nb, n), ipiv, info, o.threshold, leaf_w, plan1, s1);
+    SA_HIP(hipEventRecord(ev_panel(0), s1));
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const int64_t k0 = k * nb, kb = std::min(nb, n - k0);
+        const int64_t c1 = k0 + kb;                   // first column after the panel
+        const int64_t r = n - c1;                     // trailing size
+        SA_HIP(hipStreamWaitEvent(s0, ev_panel(k), 0));
+        // pivots of step k applied to the columns on the left (finished L) on s0
+        laswp<T>(A, ld, n, 0, k0, k0, c1, ipiv, plan0, s0);
+        if (r <= 0) break;
+        const int64_t nb1 = std::min(nb, r);          // lookahead column block width
+        if (la && r > nb1) {
+            // s1: step k on column block k+1, then panel k+1
+            SA_HIP(hipStreamWaitEvent(s1, k > 0 ? ev_trail(k) : ev_panel(0), 0));
+            laswp<T>(A, ld, n, c1, c1 + nb1, k0, c1, ipiv, plan1, s1);
+            trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c1, ld, nb1, s1);
+            GemmArgs g{};
+            g.M = r; g.N = nb1; g.K = kb;
+            g.A = A + c1 * ld + k0; g.lda = ld;
+            g.B = A + k0 * ld + c1; g.ldb = ld;
+            g.C = A + c1 * ld + c1; g.ldc = ld;
+            g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
+            gemm<T>(g, A_KC, B_NC, s1);
+            panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
+            SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
+            // s0: step k on the columns beyond k+1
+            const int64_t c2 = c1 + nb1;
+            laswp<T>(A, ld, n, c2, n, k0, c1, ipiv, plan0, s0);
+            trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c2, ld, n - c2, s0);
+            g.N = n - c2;
+            g.B = A + k0 * ld + c2;
+            g.C = A + c1 * ld + c2;
+            gemm<T>(g, A_KC, B_NC, s0);
+            SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
+        }
+        else {
+            // no lookahead: whole trailing matrix on s0, then the next panel
+            laswp<T>(A, ld, n, c1, n, k0, c1, ipiv, plan0, s0);
+            trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c1, ld, r, s0);
+            GemmArgs g{};
+            g.M = r; g.N = r; g.K = kb;
+            g.A = A + c1 * ld + k0; g.lda = ld;
+            g.B = A + k0 * ld + c1; g.ldb = ld;
+            g.C = A + c1 * ld + c1; g.ldc = ld;
+            g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
+            gemm<T>(g, A_KC, B_NC, s0);
+            if (la) {
+                SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
+                SA_HIP(hipStreamWaitEvent(s1, ev_trail(k + 1), 0));
+            }
+            panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
+            SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
+        }
+    }
+    if (la) {
+        SA_HIP(hipEventRecord(c.evs[1], s1));
+        SA_HIP(hipStreamWaitEvent(s0, c.evs[1], 0));
+    }
+}
+
+// Row-major upper Cholesky A = U^T U (the column-major lower factor).
+template <class T>
+static void potrf_diag_rec(Context& c, T* M, int64_t ld, int64_t nb, int64_t col0,
+                           int64_t* info, hipStream_t s)
+{
+    if (nb <= 64) { potrf_leaf<T>(M, ld, nb, col0, info, s); return; }
+    int64_t n1 = ((nb / 2 + 63) / 64) * 64;
+    if (n1 >= nb) n1 = nb - 64;
+    int64_t n2 = nb - n1;
+    potrf_diag_rec<T>(c, M, ld, n1, col0, info, s);
+    trsm_rec<T>(M, ld, 1, 1, 0, n1, M + n1, ld, n2, s);
+    GemmArgs g{};
+    g.M = n2; g.N = n2; g.K = n1;
+    g.A = M + n1; g.lda = ld; g.B = M + n1; g.ldb = ld;
+    g.C = M + n1 * ld + n1; g.ldc = ld;
+    g.alpha = -1.0; g.beta = 1.0; g.uplo = 1;
+    gemm<T>(g, A_MC, B_NC, s);
+    potrf_diag_rec<T>(c, M + n1 * ld + n1, ld, n2, col0 + n1, info, s);
+}
+
+// Right-looking tile Cholesky (src/potrf.cc:84-194): diagonal block, the
+// block row solve (the column-major trsm of A(k+1:, k) by L(k,k)^{-H}), and
+// the herk/gemm update of the trailing triangle as one triangular GEMM.
+template <class T>
+static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int64_t* info)
+{
+    const int64_t nb = choose_nb(n, o.nb);
+    hipStream_t s = c.s0;
+    for (int64_t k0 = 0; k0 < n; k0 += nb) {
+        int64_t kb = std::min(nb, n - k0);
+        T* Akk = A + k0 * ld + k0;
+        potrf_diag_rec<T>(c, Akk, ld, kb, k0, info, s);
+        int64_t r = n - k0 - kb;
+        if (r <= 0) break;
+        trsm_rec<T>(Akk, ld, 1, 1, 0, kb, Akk + kb, ld, r, s);
+        GemmArgs g{};
+        g.M = r; g.N = r; g.K = kb;
+        g.A = Akk + kb; g.lda = ld; g.B = Akk + kb; g.ldb = ld;
+        g.C = A + (k0 + kb) * ld + k0 + kb; g.ldc = ld;
+        g.alpha = -1.0; g.beta = 1.0; g.uplo = 1;
+        gemm<T>(g, A_MC, B_NC, s);
+    }
+}
+
+// Solve with LU factors in row-major (cm = 0) or column-major (cm = 1) layout;
+// B row-major n x nrhs (ldb).  src/getrs.cc:82-115.
+template <class T>
+static void getrs_any(Context& c, const T* LU, int64_t ld, int cm, int64_t n, const int64_t* ipiv,
+                      T* B, int64_t ldb, int64_t nrhs, int trans, hipStream_t s)
+{
+    c.plan0.ensure(sizeof(int64_t) * (4 * 2048 + 8));
+    int64_t* plan = c.plan0.as<int64_t>();
+    if (!trans) {
+        for (int64_t k1 = 0; k1 < n; k1 += 1024)
+            laswp<T>(B, ldb, n, 0, nrhs, k1, std::min(n, k1 + 1024), ipiv, plan, s);
+        trsm_rec<T>(LU, ld, cm, 1, 1, n, B, ldb, nrhs, s);
+        trsm_rec<T>(LU, ld, cm, 0, 0, n, B, ldb, nrhs, s);
+    }
+    else {
+        // A^T = U^T L^T P^T: solve U^T then L^T (transposed access), then P.
+        trsm_rec<T>(LU, ld, 1 - cm, 1, 0, n, B, ldb, nrhs, s);
+        trsm_rec<T>(LU, ld, 1 - cm, 0, 1, n, B, ldb, nrhs, s);
+        // interchanges applied backward (src/getrs.cc:106-114): chunks from the
+        // end, each as the inverse of its forward (dst <- src) plan
+        for (int64_t k2 = n; k2 > 0; k2 -= 1024) {
+            int64_t k1 = std::max<int64_t>(0, k2 - 1024);
+            laswp_plan(k1, k2, n, ipiv, plan, s);
+            laswp_invert_plan(plan, k2 - k1, s);
+            laswp<T>(B, ldb, n, 0, nrhs, k1, k2, ipiv, plan, s, true);
+        }
+    }
+}
+
+// Solve with the row-major upper Cholesky factor: U^T y = b, U x = y.
+template <class T>
+static void potrs_rm(const T* U, int64_t ld, int64_t n, T* B, int64_t ldb, int64_t nrhs, hipStream_t s)
+{
+    trsm_rec<T>(U, ld, 1, 1, 0, n, B, ldb, nrhs, s);
+    trsm_rec<T>(U, ld, 0, 0, 0, n, B, ldb, nrhs, s);
+}
+
+//------------------------------------------------------------------------------
+struct Timer {
+    Context& c;
+    explicit Timer(Context& cc) : c(cc) { SA_HIP(hipEventRecord(c.t_start, c.s0)); }
+    void stop() {
+        SA_HIP(hipEventRecord(c.t_stop, c.s0));
+        SA_HIP(hipEventSynchronize(c.t_stop));
+        float ms = 0;
+        SA_HIP(hipEventElapsedTime(&ms, c.t_start, c.t_stop));
+        c.last_ms = ms;
+    }
+};
+
+static void ipiv_to_host(Context& c, const int64_t* d, int64_t n, int64_t* host)
+{
+    if (!host || n <= 0) return;
+    SA_HIP(hipMemcpyAsync(host, d, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+    for (int64_t i = 0; i < n; ++i) host[i] += 1;     // LAPACK 1-based
+}
+
+static int64_t* ipiv_from_host(Context& c, const int64_t* host, int64_t n)
+{
+    c.ipiv.ensure(sizeof(int64_t) * std::max<int64_t>(n, 1));
+    std::vector<int64_t> tmp(host, host + n);
+    for (int64_t i = 0; i < n; ++i) {
+        // getrf output satisfies i+1 <= ipiv[i] <= n (LAPACK convention)
+        SA_REQUIRE(tmp[i] >= i + 1 && tmp[i] <= n, -5, "ipiv entry out of range");
+        tmp[i] -= 1;
+    }
+    SA_HIP(hipMemcpy(c.ipiv.p, tmp.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    return c.ipiv.as<int64_t>();
+}
+
+static int64_t read_info(Context& c)
+{
+    int64_t info = 0;
+    SA_HIP(hipMemcpyAsync(&info, c.dinfo.p, sizeof(int64_t), hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+    return info;
+}
+
+static void reset_info(Context& c)
+{
+    c.dinfo.ensure(sizeof(int64_t) * 8);
+    SA_HIP(hipMemsetAsync(c.dinfo.p, 0, sizeof(int64_t) * 8, c.s0));
+}
+
+// In-place getrf of a column-major n x n matrix.
+template <class T>
+static int64_t getrf_cm(Context& c, T* A, int64_t n, int64_t lda, const Opts& o, int64_t* d_ipiv)
+{
+    reset_info(c);
+    transpose_inplace<T>(A, n, lda, c.s0);
+    getrf_rm<T>(c, A, n, lda, o, d_ipiv, c.dinfo.as<int64_t>());
+    transpose_inplace<T>(A, n, lda, c.s0);
+    SA_HIP(hipStreamSynchronize(c.s0));
+    check_kernel_error(c);
+    return read_info(c);
+}
+
+// Row-major copy (n x nrhs) of a column-major B into a workspace and back.
+template <class T>
+static T* rhs_rm(Context& c, Buf& buf, const T* B, int64_t ldb, int64_t n, int64_t nrhs)
+{
+    buf.ensure(sizeof(T) * size_t(n) * size_t(nrhs));
+    T* W = buf.as<T>();
+    if (nrhs == 1) copy_convert<T, T>(B, ldb, W, n, n, 1, c.s0);
+    else transpose_convert<T, T>(B, ldb, W, nrhs, n, nrhs, c.s0);
+    return W;
+}
+
+template <class T>
+static void rhs_back(Context& c, const T* W, T* B, int64_t ldb, int64_t n, int64_t nrhs)
+{
+    if (nrhs == 1) copy_convert<T, T>(W, n, B, ldb, n, 1, c.s0);
+    else transpose_convert<T, T>(W, nrhs, B, ldb, nrhs, n, c.s0);
+}
+
+template <class T>
+static void getrs_cm(Context& c, int trans, const T* A, int64_t lda, int64_t n, const int64_t* d_ipiv,
+                     T* B, int64_t ldb, int64_t nrhs)
+{
+    T* W = rhs_rm<T>(c, c.rhs, B, ldb, n, nrhs);
+    getrs_any<T>(c, A, lda, 1, n, d_ipiv, W, nrhs, nrhs, trans, c.s0);
+    rhs_back<T>(c, W, B, ldb, n, nrhs);
+}
+
+template <class T>
+static int64_t potrf_cm(Context& c, int uplo, T* A, int64_t n, int64_t lda, const Opts& o)
+{
+    reset_info(c);
+    // Lower column-major == upper row-major: factor in place.  Upper storage is
+    // flipped to lower first and flipped back (a permutation, so the
+    // unreferenced triangle is restored exactly).
+    if (uplo == SLATE_AMD_UPPER) transpose_inplace<T>(A, n, lda, c.s0);
+    potrf_rm<T>(c, A, n, lda, o, c.dinfo.as<int64_t>());
+    if (uplo == SLATE_AMD_UPPER) transpose_inplace<T>(A, n, lda, c.s0);
+    return read_info(c);
+}
+
+template <class T>
+static void potrs_cm(Context& c, int uplo, T* A, int64_t lda, int64_t n, T* B, int64_t ldb, int64_t nrhs)
+{
+    T* W = rhs_rm<T>(c, c.rhs, B, ldb, n, nrhs);
+    if (uplo == SLATE_AMD_UPPER) {
+        // column-major upper U (A = U^T U) is row-major lower U^T
+        trsm_rec<T>(A, lda, 0, 1, 0, n, W, nrhs, nrhs, c.s0);
+        trsm_rec<T>(A, lda, 1, 0, 0, n, W, nrhs, nrhs, c.s0);
+    }
+    else {
+        potrs_rm<T>(A, lda, n, W, nrhs, nrhs, c.s0);
+    }
+    rhs_back<T>(c, W, B, ldb, n, nrhs);
+}
+
+//------------------------------------------------------------------------------
+// Mixed precision.  The low-precision factor lives in a row-major float copy
+// of A (one fused transpose + cast, src/gesv_mixed.cc:179) and never goes back
+// to column-major; the FP64 residual reads the caller's A.
+struct MixedState {
+    float* Alo;           // row-major n x n
+    int64_t* ipiv;        // device
+    float* xlo;           // row-major n x nrhs
+    double* R;            // column-major n x nrhs
+    double* work;
+    int64_t work_elems;
+    double* norms;        // [0]: ||A||, [1..]: col norms
+};
+
+static bool converged_host(const std::vector<double>& nr, const std::vector<double>& nx, double cte)
+{
+    for (size_t i = 0; i < nr.size(); ++i) if (nr[i] > nx[i] * cte) return false;   // internal_util.hh:121-138
+    return true;
+}
+
+static void colnorms_host(Context& c, const double* X, int64_t n, int64_t ld, int64_t nrhs,
+                          double* dev, std::vector<double>& out)
+{
+    col_norms_max(X, n, nrhs, ld, dev, c.s0);
+    out.resize(nrhs);
+    SA_HIP(hipMemcpyAsync(out.data(), dev, sizeof(double) * nrhs, hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+}
+
+static double norm_host(Context& c, const double* A, int64_t n, int64_t lda, int sym, double* work,
+                        int64_t work_elems, double* dev)
+{
+    norm_inf(A, n, n, lda, sym, work, work_elems, dev, c.s0);
+    double v = 0;
+    SA_HIP(hipMemcpyAsync(&v, dev, sizeof(double), hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+    return v;
+}
+
+static void lo_solve(Context& c, const MixedState& st, int sym, int64_t n, int64_t nrhs)
+{
+    if (sym) potrs_rm<float>(st.Alo, n, n, st.xlo, nrhs, nrhs, c.s0);
+    else getrs_any<float>(c, st.Alo, n, 0, n, st.ipiv, st.xlo, nrhs, nrhs, 0, c.s0);
+}
+
+// x_lo (row-major float) <- V (column-major double)
+static void to_lo(Context& c, const double* V, int64_t ldv, float* xlo, int64_t n, int64_t nrhs)
+{
+    if (nrhs == 1) copy_convert<double, float>(V, ldv, xlo, n, n, 1, c.s0);
+    else transpose_convert<double, float>(V, ldv, xlo, nrhs, n, nrhs, c.s0);
+}
+// V (column-major double) <- x_lo
+static void from_lo(Context& c, const float* xlo, double* V, int64_t ldv, int64_t n, int64_t nrhs)
+{
+    if (nrhs == 1) copy_convert<float, double>(xlo, n, V, ldv, n, 1, c.s0);
+    else transpose_convert<float, double>(xlo, nrhs, V, ldv, nrhs, n, c.s0);
+}
+
+static MixedState mixed_setup(Context& c, int64_t n, int64_t nrhs)
+{
+    MixedState st{};
+    c.lo.ensure(sizeof(float) * size_t(n) * size_t(n) + 256);
+    st.Alo = c.lo.as<float>();
+    int64_t we = std::max<int64_t>(residual_work_elems(n, 4), ceildiv(n, 1024) * n + n + 64);
+    size_t vec_bytes = sizeof(int64_t) * n + sizeof(float) * n * nrhs + sizeof(double) * n * nrhs
+                       + sizeof(double) * we + sizeof(double) * (nrhs + 8) + 1024;
+    c.vec.ensure(vec_bytes);
+    char* p = static_cast<char*>(c.vec.p);
+    auto take = [&](size_t b) { char* q = p; p += (b + 255) & ~size_t(255); return q; };
+    st.ipiv = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * n));
+    st.xlo = reinterpret_cast<float*>(take(sizeof(float) * n * nrhs));
+    st.R = reinterpret_cast<double*>(take(sizeof(double) * n * nrhs));
+    st.work = reinterpret_cast<double*>(take(sizeof(double) * we));
+    st.work_elems = we;
+    st.norms = reinterpret_cast<double*>(take(sizeof(double) * (nrhs + 8)));
+    return st;
+}
+
+static int64_t factor_lo(Context& c, MixedState& st, const double* A, int64_t lda, int64_t n, int sym,
+                         const Opts& o)
+{
+    reset_info(c);
+    // Fused transpose + cast into the row-major float copy.  For the symmetric
+    // case the column-major lower triangle already is the row-major upper
+    // triangle, so the cast is a plain copy.
+    if (sym) copy_convert<double, float>(A, lda, st.Alo, n, n, n, c.s0);
+    else transpose_convert<double, float>(A, lda, st.Alo, n, n, n, c.s0);
+    if (sym) {
+        // transposing lower-stored A gives the upper triangle in row-major,
+        // which is what potrf_rm factors
+        potrf_rm<float>(c, st.Alo, n, n, o, c.dinfo.as<int64_t>());
+    }
+    else {
+        getrf_rm<float>(c, st.Alo, n, n, o, st.ipiv, c.dinfo.as<int64_t>());
+    }
+    SA_HIP(hipStreamSynchronize(c.s0));
+    if (!sym) check_kernel_error(c);
+    return read_info(c);
+}
+
+template <class T> static T* as_mut(const T* p) { return const_cast<T*>(p); }
+
+// gesv_mixed / posv_mixed: src/gesv_mixed.cc:106-290, src/posv_mixed.cc:105-292.
+static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, double* A, int64_t lda,
+                        int64_t* ipiv_host, const double* B, int64_t ldb, double* X, int64_t ldx,
+                        int* iter, const Opts& o)
+{
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
+    const int64_t itermax = o.itermax;
+    *iter = 0;
+    SA_REQUIRE(!(sym && uplo == SLATE_AMD_UPPER), -2, "posv_mixed: Upper storage not supported");
+    MixedState st = mixed_setup(c, n, nrhs);
+    const double Anorm = norm_host(c, A, n, lda, sym, st.work, st.work_elems, st.norms);
+    const double cte = Anorm * tol;
+    std::vector<double> nx, nr;
+    bool converged = false;
+    int64_t info = factor_lo(c, st, A, lda, n, sym, o);
+    if (info != 0) {
+        *iter = -3;
+    }
+    else {
+        to_lo(c, B, ldb, st.xlo, n, nrhs);
+        lo_solve(c, st, sym, n, nrhs);
+        from_lo(c, st.xlo, X, ldx, n, nrhs);
+        residual(A, n, lda, sym, X, ldx, B, ldb, st.R, n, nrhs, st.work, st.work_elems, c.s0);
+        colnorms_host(c, X, n, ldx, nrhs, st.norms + 1, nx);
+        colnorms_host(c, st.R, n, n, nrhs, st.norms + 1, nr);
+        if (converged_host(nr, nx, cte)) { *iter = 0; converged = true; }
+        for (int64_t it = 0; it < itermax && !converged; ++it) {
+            to_lo(c, st.R, n, st.xlo, n, nrhs);
+            lo_solve(c, st, sym, n, nrhs);
+            from_lo(c, st.xlo, st.R, n, n, nrhs);
+            axpy(n, nrhs, 1.0, st.R, n, X, ldx, c.s0);
+            residual(A, n, lda, sym, X, ldx, B, ldb, st.R, n, nrhs, st.work, st.work_elems, c.s0);
+            colnorms_host(c, X, n, ldx, nrhs, st.norms + 1, nx);
+            colnorms_host(c, st.R, n, n, nrhs, st.norms + 1, nr);
+            if (converged_host(nr, nx, cte)) { *iter = int(it + 1); converged = true; }
+        }
+    }
+    if (!converged) {
+        if (info == 0) *iter = int(-itermax - 1);
+        if (o.fallback) {
+            c.ipiv.ensure(sizeof(int64_t) * n);
+            int64_t* dip = c.ipiv.as<int64_t>();
+            if (sym) info = potrf_cm<double>(c, uplo, A, n, lda, o);
+            else info = getrf_cm<double>(c, A, n, lda, o, dip);
+            if (info == 0) {
+                copy_convert<double, double>(B, ldb, X, ldx, n, nrhs, c.s0);
+                if (sym) potrs_cm<double>(c, uplo, A, lda, n, X, ldx, nrhs);
+                else getrs_cm<double>(c, 0, A, lda, n, dip, X, ldx, nrhs);
+            }
+            if (!sym) ipiv_to_host(c, dip, n, ipiv_host);
+        }
+    }
+    else if (!sym) {
+        ipiv_to_host(c, st.ipiv, n, ipiv_host);
+    }
+    SA_HIP(hipStreamSynchronize(c.s0));
+    return info;
+}
+
+// Givens rotation with the reference-BLAS drotg conventions.
+static void rotg(double* a, double* b, double* cc, double* ss)
+{
+    double aa = *a, bb = *b;
+    double roe = std::fabs(aa) > std::fabs(bb) ? aa : bb;
+    double scale = std::fabs(aa) + std::fabs(bb);
+    double r, z;
+    if (scale == 0) { *cc = 1; *ss = 0; r = 0; z = 0; }
+    else {
+        r = scale * std::sqrt((aa / scale) * (aa / scale) + (bb / scale) * (bb / scale));
+        r = std::copysign(1.0, roe) * r;
+        *cc = aa / r; *ss = bb / r;
+        z = 1;
+        if (std::fabs(aa) > std::fabs(bb)) z = *ss;
+        if (std::fabs(bb) >= std::fabs(aa) && *cc != 0) z = 1 / *cc;
+    }
+    *a = r; *b = z;
+}
+
+static inline void rot(double* x, double* y, double cc, double ss)
+{
+    double t = cc * *x + ss * *y;
+    *y = cc * *y - ss * *x;
+    *x = t;
+}
+
+// gesv_mixed_gmres / posv_mixed_gmres: src/gesv_mixed_gmres.cc:111-410.  The
+// Krylov bases V, W are FP64 on the device; the (restart+1)^2 Hessenberg
+// matrix and Givens rotations are on the host like the reference (:168-177).
+static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, int64_t lda,
+                           int64_t* ipiv_host, const double* B, double* X, int* iter, const Opts& o,
+                           int64_t nb_hint)
+{
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
+    const int64_t itermax = o.itermax;
+    const int64_t nb = choose_nb(n, nb_hint);
+    const int64_t restart = std::min<int64_t>(std::min<int64_t>(30, itermax), std::min(nb, n) - 1);
+    SA_REQUIRE(restart >= 1, -2, "gmres: restart length < 1");
+    SA_REQUIRE(!(sym && uplo == SLATE_AMD_UPPER), -2, "posv_mixed_gmres: Upper storage not supported");
+    *iter = 0;
+    MixedState st = mixed_setup(c, n, 1);
+    const int64_t R1 = restart + 1;
+    c.basis.ensure(sizeof(double) * size_t(n) * size_t(2 * R1) + sizeof(double) * 256);
+    double* V = c.basis.as<double>();
+    double* W = V + n * R1;
+    double* dsc = W + n * R1;          // device scalars: [0..R1) h, [64..] z, [128] norm
+    const double Anorm = norm_host(c, A, n, lda, sym, st.work, st.work_elems, st.norms);
+    const double cte = Anorm * tol;
+    std::vector<double> H(R1 * R1, 0.0), S(R1, 0.0), ga(restart), gb(restart), hz(R1), zz(R1);
+    std::vector<double> nx(1), nr(1);
+    bool converged = false;
+    int64_t info = factor_lo(c, st, A, lda, n, sym, o);
+    if (info != 0) {
+        *iter = -3;
+    }
+    else {
+        const double negone = -1.0;
+        SA_HIP(hipMemcpyAsync(dsc + 129, &negone, sizeof(double), hipMemcpyHostToDevice, c.s0));
+        to_lo(c, B, n, st.xlo, n, 1);
+        lo_solve(c, st, sym, n, 1);
+        from_lo(c, st.xlo, X, n, n, 1);
+        int64_t iiter = 0;
+        while (iiter < itermax) {
+            residual(A, n, lda, sym, X, n, B, n, st.R, n, 1, st.work, st.work_elems, c.s0);
+            colnorms_host(c, X, n, n, 1, st.norms + 1, nx);
+            colnorms_host(c, st.R, n, n, 1, st.norms + 1, nr);
+            if (converged_host(nr, nx, cte)) { *iter = int(iiter); converged = true; break; }
+            // v0 = r / ||r||
+            copy_convert<double, double>(st.R, n, V, n, n, 1, c.s0);
+            nrm2(V, n, st.work, dsc + 128, c.s0);
+            double beta = 0;
+            SA_HIP(hipMemcpyAsync(&beta, dsc + 128, sizeof(double), hipMemcpyDeviceToHost, c.s0));
+            SA_HIP(hipStreamSynchronize(c.s0));
+            std::vector<double> arn = {beta};
+            if (beta == 0) { *iter = int(iiter); converged = false; break; }
+            scale(V, n, dsc + 128, c.s0);
+            S.assign(R1, 0.0);
+            S[0] = beta;
+            int64_t j = 0;
+            for (; j < restart && iiter < itermax && !converged_host(arn, nx, cte); ++j, ++iiter) {
+                double* Vj = V + j * n;
+                double* Vj1 = V + (j + 1) * n;
+                double* Wj1 = W + (j + 1) * n;
+                // W(:, j+1) = M^{-1} V(:, j) in low precision (:281-286)
+                to_lo(c, Vj, n, st.xlo, n, 1);
+                lo_solve(c, st, sym, n, 1);
+                from_lo(c, st.xlo, Wj1, n, n, 1);
+                // V(:, j+1) = A W(:, j+1) in FP64 (:288-293): R = 0 - A w, then negate
+                zero_f64(st.R, n, c.s0);
+                residual(A, n, lda, sym, Wj1, n, st.R, n, Vj1, n, 1, st.work, st.work_elems, c.s0);
+                scale(Vj1, n, dsc + 129, c.s0);                 // dsc[129] = -1
+                // CGS2 (:296-327): h = V^T v; v -= V h; z = V^T v; v -= V z; h += z
+                multi_dot(V, n, n, j + 1, Vj1, dsc, st.work, c.s0);
+                multi_axpy(V, n, n, j + 1, dsc, Vj1, c.s0);
+                multi_dot(V, n, n, j + 1, Vj1, dsc + 64, st.work, c.s0);
+                multi_axpy(V, n, n, j + 1, dsc + 64, Vj1, c.s0);
+                nrm2(Vj1, n, st.work, dsc + 128, c.s0);
+                scale(Vj1, n, dsc + 128, c.s0);
+                double nrm = 0;
+                SA_HIP(hipMemcpyAsync(hz.data(), dsc, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c.s0));
+                SA_HIP(hipMemcpyAsync(zz.data(), dsc + 64, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c.s0));
+                SA_HIP(hipMemcpyAsync(&nrm, dsc + 128, sizeof(double), hipMemcpyDeviceToHost, c.s0));
+                SA_HIP(hipStreamSynchronize(c.s0));
+                double* Hj = H.data() + j * R1;
+                for (int64_t p = 0; p <= j; ++p) Hj[p] = hz[p] + zz[p];
+                Hj[j + 1] = nrm;
+                for (int64_t i = 0; i < j; ++i) rot(&Hj[i], &Hj[i + 1], ga[i], gb[i]);
+                double hjj = Hj[j], hj1 = Hj[j + 1];
+                rotg(&hjj, &hj1, &ga[j], &gb[j]);
+                rot(&Hj[j], &Hj[j + 1], ga[j], gb[j]);
+                rot(&S[j], &S[j + 1], ga[j], gb[j]);
+                arn[0] = std::fabs(S[j + 1]);
+            }
+            // X += W(:, 1:j) * (H(0:j, 0:j)^{-1} S(0:j)) (:360-374)
+            std::vector<double> y(S.begin(), S.begin() + j);
+            for (int64_t p = j - 1; p >= 0; --p) {
+                y[p] /= H[p + p * R1];
+                for (int64_t q = 0; q < p; ++q) y[q] -= H[q + p * R1] * y[p];
+            }
+            if (j > 0) {
+                for (int64_t p = 0; p < j; ++p) y[p] = -y[p];       // multi_axpy subtracts
+                SA_HIP(hipMemcpyAsync(dsc + 160, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c.s0));
+                multi_axpy(W + n, n, n, j, dsc + 160, X, c.s0);
+            }
+        }
+    }
+    if (!converged) {
+        if (info == 0) *iter = int(-itermax - 1);
+        if (o.fallback) {
+            c.ipiv.ensure(sizeof(int64_t) * n);
+            int64_t* dip = c.ipiv.as<int64_t>();
+            if (sym) info = potrf_cm<double>(c, uplo, A, n, lda, o);
+            else info = getrf_cm<double>(c, A, n, lda, o, dip);
+            if (info == 0) {
+                copy_convert<double, double>(B, n, X, n, n, 1, c.s0);
+                if (sym) potrs_cm<double>(c, uplo, A, lda, n, X, n, 1);
+                else getrs_cm<double>(c, 0, A, lda, n, dip, X, n, 1);
+            }
+            if (!sym) ipiv_to_host(c, dip, n, ipiv_host);
+        }
+    }
+    else if (!sym) {
+        ipiv_to_host(c, st.ipiv, n, ipiv_host);
+    }
+    SA_HIP(hipStreamSynchronize(c.s0));
+    return info;
+}
+
+} // namespace sa
+
+//==============================================================================
+// C ABI
+//==============================================================================
+struct slate_amd_context_t { sa::Context c; };
+
+static thread_local std::string g_last_error;
+
+// Inputs may have been written on another stream (e.g. torch's): settle them.
+#define SA_ENTRY_SYNC() SA_HIP(hipDeviceSynchronize())
+
+template <class F>
+static int64_t guarded(slate_amd_context ctx, F&& f)
+{
+    try {
+        if (!ctx) throw sa::Error(-1, "null context");
+        SA_HIP(hipSetDevice(ctx->c.device));
+        return f(ctx->c);
+    }
+    catch (const sa::Error& e) { g_last_error = e.what(); return e.code < 0 ? e.code : -1; }
+    catch (const std::exception& e) { g_last_error = e.what(); return -99; }
+    catch (...) { g_last_error = "unknown error"; return -99; }
+}
+
+template <class T>
+static int gemm_entry(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, double alpha,
+                      const T* A, int64_t lda, const T* B, int64_t ldb, double beta, T* C, int64_t ldc)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(m >= 0 && n >= 0 && k >= 0, -3, "bad m/n/k");
+        SA_REQUIRE(lda >= std::max<int64_t>(1, m) && ldb >= std::max<int64_t>(1, k)
+                   && ldc >= std::max<int64_t>(1, m), -3, "bad leading dimension");
+        if (m == 0 || n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Timer t(c);
+        // column-major C = row-major C^T = alpha * B^T A^T + beta * C^T
+        sa::GemmArgs g{};
+        g.M = n; g.N = m; g.K = k;
+        g.A = B; g.lda = ldb; g.B = A; g.ldb = lda; g.C = C; g.ldc = ldc;
+        g.alpha = alpha; g.beta = beta; g.uplo = 0;
+        if (k == 0) {
+            SA_REQUIRE(beta == 1.0, -2, "gemm: k == 0 requires beta == 1");
+        }
+        else {
+            sa::gemm<T>(g, sa::A_KC, sa::B_NC, c.s0);
+        }
+        t.stop();
+        return 0;
+    }));
+}
+
+namespace {
+void check_dims(int64_t n, int64_t lda) { SA_REQUIRE(n >= 0 && lda >= std::max<int64_t>(1, n), -3, "bad n/lda"); }
+void check_rhs(int64_t n, int64_t nrhs, int64_t ldb) { SA_REQUIRE(nrhs >= 0 && ldb >= std::max<int64_t>(1, n), -4, "bad nrhs/ldb"); }
+}
+
+extern "C" {
+
+int slate_amd_init(slate_amd_context* out, int device)
+{
+    try {
+        SA_REQUIRE(out != nullptr, -1, "null output pointer");
+        SA_HIP(hipSetDevice(device));
+        auto* h = new slate_amd_context_t;
+        h->c.device = device;
+        SA_HIP(hipStreamCreateWithFlags(&h->c.s0, hipStreamNonBlocking));
+        SA_HIP(hipStreamCreateWithFlags(&h->c.s1, hipStreamNonBlocking));
+        SA_HIP(hipEventCreate(&h->c.t_start));
+        SA_HIP(hipEventCreate(&h->c.t_stop));
+        *out = h;
+        return 0;
+    }
+    catch (const sa::Error& e) { g_last_error = e.what(); return e.code; }
+    catch (const std::exception& e) { g_last_error = e.what(); return -99; }
+}
+
+int slate_amd_finalize(slate_amd_context ctx)
+{
+    if (!ctx) return 0;
+    sa::Context& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    (void)hipDeviceSynchronize();
+    for (sa::Buf* b : {&c.counters, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
+                       &c.lo, &c.vec, &c.basis, &c.rhs})
+        b->release();
+    for (auto e : c.evs) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(c.t_start);
+    (void)hipEventDestroy(c.t_stop);
+    (void)hipStreamDestroy(c.s0);
+    (void)hipStreamDestroy(c.s1);
+    delete ctx;
+    return 0;
+}
+
+const char* slate_amd_last_error(void) { return g_last_error.c_str(); }
+
+void slate_amd_options_init(slate_amd_options* o)
+{
+    if (!o) return;
+    o->block_size = 0;
+    o->inner_blocking = 0;
+    o->lookahead = 1;
+    o->pivot_threshold = 1.0;
+    o->max_iterations = 30;
+    o->tolerance = 0.0;
+    o->use_fallback = 1;
+    o->factor_precision = SLATE_AMD_PREC_FP32;
+}
+
+double slate_amd_last_time_ms(slate_amd_context ctx) { return ctx ? ctx->c.last_ms : 0.0; }
+
+
+int64_t slate_amd_dgetrf(slate_amd_context ctx, int64_t n, double* dA, int64_t lda, int64_t* ipiv,
+                         const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        if (n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        c.ipiv.ensure(sizeof(int64_t) * n);
+        sa::Timer t(c);
+        int64_t info = sa::getrf_cm<double>(c, dA, n, lda, o, c.ipiv.as<int64_t>());
+        t.stop();
+        sa::ipiv_to_host(c, c.ipiv.as<int64_t>(), n, ipiv);
+        return info;
+    });
+}
+
+int64_t slate_amd_sgetrf(slate_amd_context ctx, int64_t n, float* dA, int64_t lda, int64_t* ipiv,
+                         const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        if (n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        c.ipiv.ensure(sizeof(int64_t) * n);
+        sa::Timer t(c);
+        int64_t info = sa::getrf_cm<float>(c, dA, n, lda, o, c.ipiv.as<int64_t>());
+        t.stop();
+        sa::ipiv_to_host(c, c.ipiv.as<int64_t>(), n, ipiv);
+        return info;
+    });
+}
+
+int64_t slate_amd_dgetrs(slate_amd_context ctx, int trans, int64_t n, int64_t nrhs, const double* dA,
+                         int64_t lda, const int64_t* ipiv, double* dB, int64_t ldb,
+                         const slate_amd_options*)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb);
+        if (n == 0 || nrhs == 0) return 0;
+        SA_REQUIRE(ipiv != nullptr, -5, "null ipiv");
+        SA_ENTRY_SYNC();
+        int64_t* dip = sa::ipiv_from_host(c, ipiv, n);
+        sa::Timer t(c);
+        sa::getrs_cm<double>(c, trans, dA, lda, n, dip, dB, ldb, nrhs);
+        t.stop();
+        return 0;
+    });
+}
+
+int64_t slate_amd_sgetrs(slate_amd_context ctx, int trans, int64_t n, int64_t nrhs, const float* dA,
+                         int64_t lda, const int64_t* ipiv, float* dB, int64_t ldb,
+                         const slate_amd_options*)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb);
+        if (n == 0 || nrhs == 0) return 0;
+        SA_REQUIRE(ipiv != nullptr, -5, "null ipiv");
+        SA_ENTRY_SYNC();
+        int64_t* dip = sa::ipiv_from_host(c, ipiv, n);
+        sa::Timer t(c);
+        sa::getrs_cm<float>(c, trans, dA, lda, n, dip, dB, ldb, nrhs);
+        t.stop();
+        return 0;
+    });
+}
+
+int64_t slate_amd_dgesv(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                        int64_t* ipiv, double* dB, int64_t ldb, const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb);
+        if (n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        c.ipiv.ensure(sizeof(int64_t) * n);
+        sa::Timer t(c);
+        int64_t info = sa::getrf_cm<double>(c, dA, n, lda, o, c.ipiv.as<int64_t>());
+        if (info == 0 && nrhs > 0)
+            sa::getrs_cm<double>(c, 0, dA, lda, n, c.ipiv.as<int64_t>(), dB, ldb, nrhs);
+        t.stop();
+        sa::ipiv_to_host(c, c.ipiv.as<int64_t>(), n, ipiv);
+        return info;
+    });
+}
+
+int64_t slate_amd_dpotrf(slate_amd_context ctx, int uplo, int64_t n, double* dA, int64_t lda,
+                         const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        if (n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        int64_t info = sa::potrf_cm<double>(c, uplo, dA, n, lda, o);
+        t.stop();
+        return info;
+    });
+}
+
+int64_t slate_amd_spotrf(slate_amd_context ctx, int uplo, int64_t n, float* dA, int64_t lda,
+                         const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        if (n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        int64_t info = sa::potrf_cm<float>(c, uplo, dA, n, lda, o);
+        t.stop();
+        return info;
+    });
+}
+
+int64_t slate_amd_dpotrs(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs, const double* dA,
+                         int64_t lda, double* dB, int64_t ldb, const slate_amd_options*)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb);
+        if (n == 0 || nrhs == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Timer t(c);
+        sa::potrs_cm<double>(c, uplo, sa::as_mut(dA), lda, n, dB, ldb, nrhs);
+        t.stop();
+        return 0;
+    });
+}
+
+int64_t slate_amd_dposv(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs, double* dA,
+                        int64_t lda, double* dB, int64_t ldb, const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb);
+        if (n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        int64_t info = sa::potrf_cm<double>(c, uplo, dA, n, lda, o);
+        if (info == 0 && nrhs > 0) sa::potrs_cm<double>(c, uplo, dA, lda, n, dB, ldb, nrhs);
+        t.stop();
+        return info;
+    });
+}
+
+int64_t slate_amd_dsgesv(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                         int64_t* ipiv, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                         int* iter, const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb); check_rhs(n, nrhs, ldx);
+        SA_REQUIRE(iter != nullptr, -6, "null iter");
+        if (n == 0 || nrhs == 0) { *iter = 0; return 0; }
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        int64_t info = sa::mixed_ir(c, 0, SLATE_AMD_LOWER, n, nrhs, dA, lda, ipiv, dB, ldb, dX, ldx, iter, o);
+        t.stop();
+        return info;
+    });
+}
+
+int64_t slate_amd_dsgesv_gmres(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                               int64_t* ipiv, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                               int* iter, const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb); check_rhs(n, nrhs, ldx);
+        SA_REQUIRE(iter != nullptr, -6, "null iter");
+        // src/gesv_mixed_gmres.cc:144-146
+        SA_REQUIRE(nrhs == 1, -7, "block-GMRES for multiple RHS is not yet supported");
+        if (n == 0) { *iter = 0; return 0; }
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        (void)ldb; (void)ldx;
+        int64_t info = sa::mixed_gmres(c, 0, SLATE_AMD_LOWER, n, dA, lda, ipiv, dB, dX, iter, o, o.nb);
+        t.stop();
+        return info;
+    });
+}
+
+int64_t slate_amd_dsposv(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs, double* dA,
+                         int64_t lda, const double* dB, int64_t ldb, double* dX, int64_t ldx, int* iter,
+                         const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb); check_rhs(n, nrhs, ldx);
+        SA_REQUIRE(iter != nullptr, -6, "null iter");
+        if (n == 0 || nrhs == 0) { *iter = 0; return 0; }
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        int64_t info = sa::mixed_ir(c, 1, uplo, n, nrhs, dA, lda, nullptr, dB, ldb, dX, ldx, iter, o);
+        t.stop();
+        return info;
+    });
+}
+
+int64_t slate_amd_dsposv_gmres(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs, double* dA,
+                               int64_t lda, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                               int* iter, const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb); check_rhs(n, nrhs, ldx);
+        SA_REQUIRE(iter != nullptr, -6, "null iter");
+        SA_REQUIRE(nrhs == 1, -7, "block-GMRES for multiple RHS is not yet supported");
+        if (n == 0) { *iter = 0; return 0; }
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        (void)ldb; (void)ldx;
+        int64_t info = sa::mixed_gmres(c, 1, uplo, n, dA, lda, nullptr, dB, dX, iter, o, o.nb);
+        t.stop();
+        return info;
+    });
+}
+
+int slate_amd_dgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, double alpha,
+                    const double* dA, int64_t lda, const double* dB, int64_t ldb, double beta,
+                    double* dC, int64_t ldc)
+{ return gemm_entry<double>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc); }
+
+int slate_amd_sgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, float alpha,
+                    const float* dA, int64_t lda, const float* dB, int64_t ldb, float beta,
+                    float* dC, int64_t ldc)
+{ return gemm_entry<float>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc); }
+
+int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double* dA, int64_t lda,
+                        double* result)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        SA_REQUIRE(result != nullptr, -5, "null result");
+        if (n == 0) { *result = 0; return 0; }
+        SA_ENTRY_SYNC();
+        int64_t we = sa::ceildiv(n, 1024) * n + n + 64;
+        c.work.ensure(sizeof(double) * (we + 8));
+        double* w = c.work.as<double>();
+        *result = sa::norm_host(c, dA, n, lda, sym, w, we, w + we);
+        return 0;
+    }));
+}
+
+int slate_amd_generate(slate_amd_context ctx, int dist, int64_t seed, int64_t m, int64_t n,
+                       int64_t ioffset, int64_t joffset, double* dA, int64_t lda)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(m >= 0 && n >= 0 && lda >= std::max<int64_t>(1, m), -3, "bad m/n/lda");
+        SA_REQUIRE(dist >= 1 && dist <= 3, -2, "unknown distribution");
+        SA_ENTRY_SYNC();
+        sa::generate(dist, seed, m, n, ioffset, joffset, dA, lda, c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return 0;
+    }));
+}
+
+} // extern "C"

@@ -1,0 +1,217 @@
+// MFMA tile GEMM for gfx950: C = alpha * A * B + beta * C, C row-major.
+//
+// This is the kernel behind every level-3 update of the factorizations (the
+// trailing-matrix update of getrf, the herk/gemm update of potrf, the inner
+// updates of the recursive panel and of the blocked triangular solves).  It
+// replaces the vendor blas::batch::gemm / batch::herk calls of
+// src/internal/internal_gemm.cc:498-504 and src/internal/internal_herk.cc:491-516:
+// one launch covers the whole trailing matrix instead of a batch of nb x nb
+// tiles, so there are no host-built pointer arrays and no per-tile launches.
+//
+// Shape: 256 threads = 4 wave64s in a 2x2 arrangement; each wave owns a 64x64
+// block of C held as 4x4 MFMA 16x16 accumulators.  The K loop stages BK-deep
+// slices of A and B through double-buffered LDS; the next slice is fetched into
+// registers while the current one feeds the matrix cores.
+#include "common.hh"
+#include "kernels.hh"
+
+namespace sa {
+
+template <class T> struct Mf;
+
+// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// accumulator register r of lane l holds D[4r + l/16][l%16].
+template <> struct Mf<double> {
+    using acc = v4d;
+    static constexpr int KM = 4;
+    __device__ static acc op(double a, double b, acc c)
+    { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int l, int r) { return 4 * r + (l >> 4); }
+};
+
+// v_mfma_f32_16x16x4f32: same operand layout; register r holds D[4(l/16) + r][l%16].
+template <> struct Mf<float> {
+    using acc = v4f;
+    static constexpr int KM = 4;
+    __device__ static acc op(float a, float b, acc c)
+    { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+// A layout: A_KC = K-contiguous (A[m*lda + k]), A_MC = M-contiguous (A[m + k*lda]).
+// B layout: B_NC = N-contiguous (B[k*ldb + n]), B_KC = K-contiguous (B[k + n*ldb]).
+template <class T, int ALAY, int BLAY, int BM, int BN, int BK>
+__global__ __launch_bounds__(256)
+void gemm_kernel(GemmArgs p)
+{
+    constexpr int NT = 256;
+    constexpr int WM = 2, WN = 2;
+    constexpr int TM = BM / WM, TN = BN / WN;       // per-wave C block
+    constexpr int FM = TM / 16, FN = TN / 16;       // MFMA tiles per wave
+    constexpr int KM = Mf<T>::KM;
+    // LDS tiles mirror the global layout so the copy is a straight store.
+    constexpr int A_CD = (ALAY == A_KC) ? BK : BM;  // contiguous extent
+    constexpr int A_OD = (ALAY == A_KC) ? BM : BK;
+    constexpr int B_CD = (BLAY == B_NC) ? BN : BK;
+    constexpr int B_OD = (BLAY == B_NC) ? BK : BN;
+    constexpr int A_LD = A_CD + (sizeof(T) == 8 ? 1 : 1);
+    constexpr int B_LD = B_CD + (sizeof(T) == 8 ? 2 : 4);
+    constexpr int A_PER = A_CD * A_OD / NT;         // elements per thread
+    constexpr int B_PER = B_CD * B_OD / NT;
+    static_assert(NT % A_CD == 0 && NT % B_CD == 0, "tile shape");
+
+    __shared__ T As[2][A_OD * A_LD];
+    __shared__ T Bs[2][B_OD * B_LD];
+
+    const int64_t M = p.M, N = p.N, K = p.K;
+    const int64_t tiles_m = ceildiv(M, BM), tiles_n = ceildiv(N, BN);
+    const int64_t ntiles = tiles_m * tiles_n;
+    int64_t t = xcd_remap(blockIdx.x, ntiles);
+    // Group GM tile-rows together so the A rows of a group stay in L2.
+    constexpr int64_t GM = 8;
+    int64_t group = t / (GM * tiles_n);
+    int64_t first_m = group * GM;
+    int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
+    int64_t within = t - group * GM * tiles_n;
+    int64_t tm = first_m + within % gm;
+    int64_t tn = within / gm;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    if (p.uplo == 1 && n0 + BN <= m0) return;       // strictly below the diagonal
+
+    const T* __restrict__ A = static_cast<const T*>(p.A);
+    const T* __restrict__ B = static_cast<const T*>(p.B);
+    T* __restrict__ C = static_cast<T*>(p.C);
+    const int64_t lda = p.lda, ldb = p.ldb, ldc = p.ldc;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+
+    T ra[A_PER], rb[B_PER];
+
+    auto load_tiles = [&](int64_t k0) {
+        #pragma unroll
+        for (int i = 0; i < A_PER; ++i) {
+            int e = tid + i * NT;
+            int cd = e % A_CD, od = e / A_CD;
+            int64_t gm_, gk_;
+            if (ALAY == A_KC) { gk_ = k0 + cd; gm_ = m0 + od; }
+            else              { gm_ = m0 + cd; gk_ = k0 + od; }
+            bool ok = gm_ < M && gk_ < K;
+            int64_t off = (ALAY == A_KC) ? gm_ * lda + gk_ : gm_ + gk_ * lda;
+            ra[i] = ok ? A[off] : T(0);
+        }
+        #pragma unroll
+        for (int i = 0; i < B_PER; ++i) {
+            int e = tid + i * NT;
+            int cd = e % B_CD, od = e / B_CD;
+            int64_t gk_, gn_;
+            if (BLAY == B_NC) { gn_ = n0 + cd; gk_ = k0 + od; }
+            else              { gk_ = k0 + cd; gn_ = n0 + od; }
+            bool ok = gn_ < N && gk_ < K;
+            int64_t off = (BLAY == B_NC) ? gk_ * ldb + gn_ : gk_ + gn_ * ldb;
+            rb[i] = ok ? B[off] : T(0);
+        }
+    };
+    auto store_tiles = [&](int buf) {
+        #pragma unroll
+        for (int i = 0; i < A_PER; ++i) {
+            int e = tid + i * NT;
+            As[buf][(e / A_CD) * A_LD + (e % A_CD)] = ra[i];
+        }
+        #pragma unroll
+        for (int i = 0; i < B_PER; ++i) {
+            int e = tid + i * NT;
+            Bs[buf][(e / B_CD) * B_LD + (e % B_CD)] = rb[i];
+        }
+    };
+    auto a_at = [&](int buf, int m, int k) -> T {
+        return (ALAY == A_KC) ? As[buf][m * A_LD + k] : As[buf][k * A_LD + m];
+    };
+    auto b_at = [&](int buf, int k, int n) -> T {
+        return (BLAY == B_NC) ? Bs[buf][k * B_LD + n] : Bs[buf][n * B_LD + k];
+    };
+
+    typename Mf<T>::acc acc[FM][FN];
+    #pragma unroll
+    for (int i = 0; i < FM; ++i)
+        #pragma unroll
+        for (int j = 0; j < FN; ++j)
+            #pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
+
+    const int64_t ktiles = ceildiv(K, BK);
+    load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+
+    const int fr = lane & 15, fk = lane >> 4;
+    for (int64_t kt = 0; kt < ktiles; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < ktiles) load_tiles((kt + 1) * BK);
+        #pragma unroll
+        for (int kk = 0; kk < BK; kk += KM) {
+            T af[FM], bf[FN];
+            #pragma unroll
+            for (int i = 0; i < FM; ++i) af[i] = a_at(cur, wm * TM + i * 16 + fr, kk + fk);
+            #pragma unroll
+            for (int j = 0; j < FN; ++j) bf[j] = b_at(cur, kk + fk, wn * TN + j * 16 + fr);
+            #pragma unroll
+            for (int i = 0; i < FM; ++i)
+                #pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mf<T>::op(af[i], bf[j], acc[i][j]);
+        }
+        if (kt + 1 < ktiles) {
+            store_tiles(cur ^ 1);
+            __syncthreads();
+        }
+    }
+
+    const T alpha = T(p.alpha), beta = T(p.beta);
+    #pragma unroll
+    for (int i = 0; i < FM; ++i)
+        #pragma unroll
+        for (int j = 0; j < FN; ++j)
+            #pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int64_t gm_ = m0 + wm * TM + i * 16 + Mf<T>::row(lane, r);
+                int64_t gn_ = n0 + wn * TN + j * 16 + fr;
+                if (gm_ < M && gn_ < N && (p.uplo == 0 || gn_ >= gm_)) {
+                    T* c = C + gm_ * ldc + gn_;
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * *c;
+                    *c = v;
+                }
+            }
+}
+
+template <class T, int ALAY, int BLAY>
+static void launch(const GemmArgs& a, hipStream_t s)
+{
+    if (a.M <= 0 || a.N <= 0) return;
+    constexpr int BM = 128, BN = 128, BK = (sizeof(T) == 8) ? 16 : 32;
+    int64_t nt = ceildiv(a.M, BM) * ceildiv(a.N, BN);
+    SA_REQUIRE(nt < (int64_t(1) << 31), -2, "gemm: grid too large");
+    hipLaunchKernelGGL((gemm_kernel<T, ALAY, BLAY, BM, BN, BK>), dim3(unsigned(nt)), dim3(256),
+                       0, s, a);
+    SA_CHECK_LAUNCH();
+}
+
+template <class T>
+void gemm(const GemmArgs& a, int alay, int blay, hipStream_t s)
+{
+    if (a.K <= 0) {
+        // C = beta * C only; the factorizations never request this.
+        SA_REQUIRE(a.beta == 1.0, -2, "gemm: K == 0 with beta != 1 unsupported");
+        return;
+    }
+    if (alay == A_KC && blay == B_NC) launch<T, A_KC, B_NC>(a, s);
+    else if (alay == A_MC && blay == B_NC) launch<T, A_MC, B_NC>(a, s);
+    else if (alay == A_KC && blay == B_KC) launch<T, A_KC, B_KC>(a, s);
+    else launch<T, A_MC, B_KC>(a, s);
+}
+
+template void gemm<double>(const GemmArgs&, int, int, hipStream_t);
+template void gemm<float>(const GemmArgs&, int, int, hipStream_t);
+
+} // namespace sa

@@ -1,0 +1,116 @@
+// Host-side launchers of the slate_amd device kernels.  All matrices handed to
+// the factorization kernels are row-major unless stated: element (i, j) at
+// base[i * ld + j].  (A column-major n x n matrix is turned into this layout by
+// an in-place transpose, the same conversion SLATE performs with its RowMajor
+// device tiles, src/getrf.cc:54-55.)
+#pragma once
+#include "common.hh"
+
+namespace sa {
+
+enum { A_KC = 0, A_MC = 1 };     // A operand: K-contiguous / M-contiguous
+enum { B_NC = 0, B_KC = 1 };     // B operand: N-contiguous / K-contiguous
+
+struct GemmArgs {                // C[M x N] = alpha * A * B + beta * C, C row-major
+    int64_t M, N, K;
+    const void* A; int64_t lda;
+    const void* B; int64_t ldb;
+    void* C; int64_t ldc;
+    double alpha, beta;
+    int uplo;                    // 0: all of C; 1: only entries with col >= row
+};
+
+template <class T> void gemm(const GemmArgs& a, int alay, int blay, hipStream_t s);
+
+// Workspace shared by the cross-workgroup panel kernel.
+struct PanelWork {
+    unsigned* counters;          // barrier counters, one per leaf launch (zeroed)
+    void* slots;                 // candidate rows exchanged between workgroups
+    int64_t slots_bytes;
+    int* error;                  // set to 1 if a barrier wait times out
+};
+
+// Partial-pivot LU of a leaf panel: rows [r0, m_total) x columns [c0, c0+w) of
+// a row-major matrix.  Writes ipiv[c0 + j] = global 0-based pivot row and
+// sets *info (1-based global column) at the first exactly-zero pivot.
+template <class T>
+void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
+                int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
+                int64_t counter_index, hipStream_t s);
+int panel_leaf_width(int elem_bytes);   // largest leaf width the kernel supports
+
+// Row interchanges: apply swaps (i, ipiv[i]) for i in [k1, k2), in order, to
+// columns [c0, c1) of a row-major matrix with nrows rows.  Two launches: a
+// one-block plan that turns the sequence into a (dst <- src) row list, then a
+// gather/scatter over column chunks.  plan must hold 4 * (k2 - k1) + 1 int64.
+// Pivots outside [i, nrows) are treated as "no interchange".
+template <class T>
+void laswp(T* A, int64_t ld, int64_t nrows, int64_t c0, int64_t c1, int64_t k1, int64_t k2,
+           const int64_t* ipiv, int64_t* plan, hipStream_t s, bool plan_ready = false);
+void laswp_plan(int64_t k1, int64_t k2, int64_t nrows, const int64_t* ipiv, int64_t* plan,
+                hipStream_t s);
+// Turn a (dst <- src) plan into its inverse permutation (dst/src swapped).
+void laswp_invert_plan(int64_t* plan, int64_t npiv, hipStream_t s);
+
+// Triangular solve leaf: B (nb x N, row-major, ldb) := op(T)^{-1} B for an
+// nb x nb triangle (nb <= TRSM_LEAF).  tri(i,k) = M[i*ld + k] (cm == 0) or
+// M[i + k*ld] (cm == 1).  lower/unit select the triangle and diagonal.
+constexpr int TRSM_LEAF = 32;
+template <class T>
+void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
+               T* B, int64_t ldb, int64_t N, hipStream_t s);
+
+// Cholesky leaf on an nb x nb row-major block (nb <= 64), upper factor U with
+// A = U^T U stored in the upper triangle.  Sets *info = col0 + j + 1 at the
+// first non-positive pivot (if *info == 0).
+template <class T>
+void potrf_leaf(T* A, int64_t ld, int64_t nb, int64_t col0, int64_t* info, hipStream_t s);
+
+// In-place transpose of an n x n matrix (column-major <-> row-major).
+template <class T> void transpose_inplace(T* A, int64_t n, int64_t ld, hipStream_t s);
+// Out-of-place transpose with conversion: dst(j, i) = convert(src(i, j)),
+// src m x n with leading dim lds (element (i,j) at src[i + j*lds]).
+template <class S, class D>
+void transpose_convert(const S* src, int64_t lds, D* dst, int64_t ldd, int64_t m, int64_t n,
+                       hipStream_t s);
+// Elementwise copy/convert of an m x n column-major block.
+template <class S, class D>
+void copy_convert(const S* src, int64_t lds, D* dst, int64_t ldd, int64_t m, int64_t n,
+                  hipStream_t s);
+
+// ||A||_inf of a column-major matrix (general, or symmetric stored lower).
+// Result written to out[0] on the device; work holds ceil(n/256)*m doubles.
+void norm_inf(const double* A, int64_t m, int64_t n, int64_t lda, int sym, double* work,
+              int64_t work_elems, double* out, hipStream_t s);
+// Per-column max |X(:, j)| into out[j].
+void col_norms_max(const double* X, int64_t m, int64_t n, int64_t ldx, double* out,
+                   hipStream_t s);
+// R = B - op(A) X for column-major A (general or symmetric-lower), nrhs <= 4,
+// deterministic two-pass reduction through work.
+void residual(const double* A, int64_t n, int64_t lda, int sym, const double* X, int64_t ldx,
+              const double* B, int64_t ldb, double* R, int64_t ldr, int64_t nrhs,
+              double* work, int64_t work_elems, hipStream_t s);
+int64_t residual_work_elems(int64_t n, int64_t nrhs);
+// Y += alpha * X (column-major m x n).
+void axpy(int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double* Y,
+          int64_t ldy, hipStream_t s);
+
+// Dense dot-products used by GMRES: h[p] = sum_i V[i + p*ldv] * w[i], p < k.
+void multi_dot(const double* V, int64_t ldv, int64_t n, int64_t k, const double* w,
+               double* h, double* work, hipStream_t s);
+// w -= V(:, 0:k) h
+void multi_axpy(const double* V, int64_t ldv, int64_t n, int64_t k, const double* h,
+                double* w, hipStream_t s);
+// out[0] = ||w||_2
+void nrm2(const double* w, int64_t n, double* work, double* out, hipStream_t s);
+void scale(double* w, int64_t n, const double* denom_dev, hipStream_t s);
+
+// Matrix generation (SLATE matgen Philox): A[i + j*lda] = f(seed, i+ioff, j+joff).
+void generate(int dist, int64_t seed, int64_t m, int64_t n, int64_t ioff, int64_t joff,
+              double* A, int64_t lda, hipStream_t s);
+
+// Fill an int/unsigned buffer with zeros (graph-safe replacement for memset).
+void zero_u32(unsigned* p, int64_t n, hipStream_t s);
+void zero_f64(double* p, int64_t n, hipStream_t s);
+
+} // namespace sa

@@ -1,0 +1,421 @@
+// Panel LU leaf and row interchanges (gfx950).
+//
+// The leaf kernel factors a tall, narrow block (up to 65536+ rows x W columns)
+// with partial pivoting.  It replaces SLATE's host panel (internal::getrf_panel
+// <HostTask>, src/internal/internal_getrf.cc:20-121, which pulls the panel to
+// the CPU every step) by a GPU kernel in which G workgroups each keep a slice
+// of rows in registers.  Per column, each workgroup finds its local pivot
+// candidate with wavefront shuffles, publishes it (value, row, and the row's W
+// entries) to a double-buffered exchange area, and meets the others at a
+// grid barrier; every workgroup then reduces the G candidates itself, applies
+// the interchange to the rows it owns, scales and performs the rank-1 update.
+// Pivot rules follow src/internal/Tile_getrf.hh:197-366: largest |a_ij| with
+// the smallest row winning exact ties, the diagonal kept when
+// |a_jj| >= threshold * max, scaling by the reciprocal unless the pivot is
+// below the smallest normal number, and info = first exactly-zero pivot.
+#include "common.hh"
+#include "kernels.hh"
+#include <limits>
+
+namespace sa {
+
+static constexpr int LEAF_THREADS = 256;
+static constexpr int LEAF_GMAX = 128;     // workgroups per leaf launch
+static constexpr int LEAF_RPT_MAX = 4;    // rows per thread
+
+template <class T> struct LeafW;
+template <> struct LeafW<double> { static constexpr int W = 16; };
+template <> struct LeafW<float> { static constexpr int W = 32; };
+
+int panel_leaf_width(int elem_bytes) { return elem_bytes == 8 ? LeafW<double>::W : LeafW<float>::W; }
+
+// Exchange area layout (per parity): cand_abs[G], cand_idx[G], cand_row[G][W], rowj[W].
+template <class T, int W>
+struct Slots {
+    T* abs_; int64_t* idx_; T* row_; T* rowj_;
+    __device__ Slots(void* base, int G, int par) {
+        char* b = static_cast<char*>(base);
+        size_t per = size_t(G) * (sizeof(T) + sizeof(int64_t) + W * sizeof(T)) + W * sizeof(T);
+        per = (per + 255) & ~size_t(255);
+        b += par * per;
+        abs_ = reinterpret_cast<T*>(b);
+        idx_ = reinterpret_cast<int64_t*>(b + size_t(G) * sizeof(T));
+        row_ = reinterpret_cast<T*>(b + size_t(G) * (sizeof(T) + sizeof(int64_t)));
+        rowj_ = row_ + size_t(G) * W;
+    }
+};
+
+template <class T>
+__device__ inline bool better(T a, int64_t ia, T b, int64_t ib)
+{
+    return a > b || (a == b && ia < ib);
+}
+
+template <class T, int W, int RPT>
+__global__ __launch_bounds__(LEAF_THREADS)
+void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0,
+                       int w, int64_t* __restrict__ ipiv, int64_t* __restrict__ info, T threshold,
+                       unsigned* counter, void* slots, int* error)
+{
+    const int G = gridDim.x;
+    const int g = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t m = m_total - r0;
+    constexpr int64_t R = int64_t(LEAF_THREADS) * RPT;
+    const int64_t gbase = int64_t(g) * R;
+
+    __shared__ T s_wabs[4];
+    __shared__ int64_t s_widx[4];
+    __shared__ T s_cabs[LEAF_GMAX];
+    __shared__ int64_t s_cidx[LEAF_GMAX];
+    __shared__ T s_crow[LEAF_GMAX * W];
+    __shared__ T s_rowj[W];
+    __shared__ T s_piv[W];
+    __shared__ int64_t s_p;
+
+    T v[RPT][W];
+    #pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+        #pragma unroll
+        for (int c = 0; c < W; ++c)
+            v[q][c] = (i < m && c < w) ? A[(r0 + i) * ld + c0 + c] : T(0);
+    }
+    const T safe_min = std::numeric_limits<T>::min();
+
+    // The column loop stays rolled; register columns are addressed with
+    // compile-time indices under predicates so v[][] never spills.
+    #pragma unroll 1
+    for (int j = 0; j < w; ++j) {
+        const int par = j & 1;
+        Slots<T, W> sl(slots, G, par);
+
+        // (1) local candidate: first row i >= j of maximal |a_ij|
+        T babs = T(-1);
+        int64_t bidx = INT64_MAX;
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            T x = T(0);
+            #pragma unroll
+            for (int c = 0; c < W; ++c) if (c == j) x = v[q][c];
+            T a = fabs(x);
+            if (i >= j && i < m && a > babs) { babs = a; bidx = i; }
+        }
+        #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            T oa = __shfl_xor(babs, o, 64);
+            int64_t oi = __shfl_xor(bidx, o, 64);
+            if (better(oa, oi, babs, bidx)) { babs = oa; bidx = oi; }
+        }
+        if (lane == 0) { s_wabs[wave] = babs; s_widx[wave] = bidx; }
+        __syncthreads();
+        babs = s_wabs[0]; bidx = s_widx[0];
+        #pragma unroll
+        for (int k = 1; k < 4; ++k)
+            if (better(s_wabs[k], s_widx[k], babs, bidx)) { babs = s_wabs[k]; bidx = s_widx[k]; }
+
+        // (2) publish candidate row and, for the owner, the current row j
+        if (bidx != INT64_MAX && bidx >= gbase && bidx < gbase + R
+            && (bidx - gbase) % LEAF_THREADS == tid) {
+            int qq = int((bidx - gbase) / LEAF_THREADS);
+            #pragma unroll
+            for (int q = 0; q < RPT; ++q)
+                if (q == qq) {
+                    #pragma unroll
+                    for (int c = 0; c < W; ++c) sl.row_[size_t(g) * W + c] = v[q][c];
+                }
+            __threadfence();
+        }
+        if (int64_t(j) >= gbase && int64_t(j) < gbase + R && (j - gbase) % LEAF_THREADS == tid) {
+            int qj = int((j - gbase) / LEAF_THREADS);
+            #pragma unroll
+            for (int q = 0; q < RPT; ++q)
+                if (q == qj) {
+                    #pragma unroll
+                    for (int c = 0; c < W; ++c) sl.rowj_[c] = v[q][c];
+                }
+            __threadfence();
+        }
+        if (tid == 0) {
+            sl.abs_[g] = babs;
+            sl.idx_[g] = bidx;
+            __threadfence();
+        }
+        __syncthreads();
+
+        // (3) grid barrier with a bounded wait
+        if (tid == 0) {
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned target = unsigned(G) * unsigned(j + 1);
+            uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(1);
+                if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+                if (wall_clock64() - t0 > 400000000ull) {       // ~4 s at 100 MHz
+                    __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            __threadfence();
+        }
+        __syncthreads();
+
+        // (4) gather all candidates and row j, pick the pivot
+        for (int e = tid; e < G; e += LEAF_THREADS) {
+            s_cabs[e] = sl.abs_[e];
+            s_cidx[e] = sl.idx_[e];
+        }
+        for (int e = tid; e < G * W; e += LEAF_THREADS) s_crow[e] = sl.row_[e];
+        if (tid < W) s_rowj[tid] = sl.rowj_[tid];
+        __syncthreads();
+        if (wave == 0) {
+            T ba = T(-1);
+            int64_t bi = INT64_MAX;
+            int bg = -1;
+            for (int e = lane; e < G; e += 64)
+                if (better(s_cabs[e], s_cidx[e], ba, bi)) { ba = s_cabs[e]; bi = s_cidx[e]; bg = e; }
+            #pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                T oa = __shfl_xor(ba, o, 64);
+                int64_t oi = __shfl_xor(bi, o, 64);
+                int og = __shfl_xor(bg, o, 64);
+                if (better(oa, oi, ba, bi)) { ba = oa; bi = oi; bg = og; }
+            }
+            // Tile_getrf.hh:283-289: keep the diagonal if it is large enough.
+            bool diag = !(bi != INT64_MAX) || fabs(s_rowj[j]) >= threshold * ba;
+            int64_t p = diag ? int64_t(j) : bi;
+            if (p < j || p >= m) p = j;            // never trust a row outside the leaf
+            if (lane < W) s_piv[lane] = diag ? s_rowj[lane] : s_crow[bg * W + lane];
+            if (lane == 0) s_p = p;
+        }
+        __syncthreads();
+        const int64_t p = s_p;
+
+        // (5) interchange rows j and p among the rows this thread owns
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            if (i == j) {
+                #pragma unroll
+                for (int c = 0; c < W; ++c) v[q][c] = s_piv[c];
+            }
+            else if (i == p) {
+                #pragma unroll
+                for (int c = 0; c < W; ++c) v[q][c] = s_rowj[c];
+            }
+        }
+        const T pv = s_piv[j];
+        if (g == 0 && tid == 0) {
+            ipiv[c0 + j] = r0 + p;
+            if (pv == T(0) && *info == 0) *info = c0 + j + 1;
+        }
+        // (6) scale the column below the pivot and update the leaf to the right
+        const bool tiny = fabs(pv) < safe_min;
+        const T inv = T(1) / pv;
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            if (i > j && i < m) {
+                T l = T(0);
+                #pragma unroll
+                for (int c = 0; c < W; ++c) if (c == j) l = v[q][c];
+                if (pv != T(0)) l = tiny ? l / pv : l * inv;
+                #pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    if (c == j) v[q][c] = l;
+                    else if (c > j) v[q][c] -= l * s_piv[c];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    #pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+        if (i < m) {
+            #pragma unroll
+            for (int c = 0; c < W; ++c)
+                if (c < w) A[(r0 + i) * ld + c0 + c] = v[q][c];
+        }
+    }
+}
+
+template <class T>
+void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
+                int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
+                int64_t counter_index, hipStream_t s)
+{
+    constexpr int W = LeafW<T>::W;
+    const int64_t m = m_total - r0;
+    SA_REQUIRE(w >= 1 && w <= W, -2, "panel_leaf: bad width");
+    SA_REQUIRE(m >= w, -2, "panel_leaf: fewer rows than columns");
+    int rpt = 1;
+    while (rpt < LEAF_RPT_MAX && ceildiv(m, int64_t(LEAF_THREADS) * rpt) > 32) rpt *= 2;
+    int64_t G = ceildiv(m, int64_t(LEAF_THREADS) * rpt);
+    SA_REQUIRE(G <= LEAF_GMAX, -2, "panel_leaf: panel too tall");
+    size_t per = size_t(G) * (sizeof(T) + sizeof(int64_t) + W * sizeof(T)) + W * sizeof(T);
+    per = (per + 255) & ~size_t(255);
+    SA_REQUIRE(int64_t(2 * per) <= pw.slots_bytes, -2, "panel_leaf: slot area too small");
+    unsigned* ctr = pw.counters + counter_index;
+    T thr = T(threshold);
+    switch (rpt) {
+        case 1: hipLaunchKernelGGL((panel_leaf_kernel<T, W, 1>), dim3(G), dim3(LEAF_THREADS), 0, s,
+                                   A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error); break;
+        case 2: hipLaunchKernelGGL((panel_leaf_kernel<T, W, 2>), dim3(G), dim3(LEAF_THREADS), 0, s,
+                                   A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error); break;
+        default: hipLaunchKernelGGL((panel_leaf_kernel<T, W, 4>), dim3(G), dim3(LEAF_THREADS), 0, s,
+                                    A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error); break;
+    }
+    SA_CHECK_LAUNCH();
+}
+
+template void panel_leaf<double>(double*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t*,
+                                 double, const PanelWork&, int64_t, hipStream_t);
+template void panel_leaf<float>(float*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t*,
+                                double, const PanelWork&, int64_t, hipStream_t);
+
+//------------------------------------------------------------------------------
+// Row interchanges.  The sequential LAPACK-style swap list (i, ipiv[i]),
+// i = k1..k2-1, with ipiv[i] >= i (src/internal/internal_swap.cc:198-435 and
+// the parallel-pivot map of :33-77), is turned into an explicit list of
+// (dst row <- src row) moves: every touched row r receives the original row
+// src(r) = t_k1(t_k1+1(...t_k2-1(r))), t_i the i-th transposition.  The apply
+// kernel then gathers the sources of a column chunk into LDS and scatters
+// them, which keeps every row access coalesced in the row-major layout.
+static constexpr int PLAN_MAX = 2048;
+
+__global__ __launch_bounds__(1024)
+void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __restrict__ ipiv,
+                       int64_t* __restrict__ plan)
+{
+    __shared__ int64_t piv[PLAN_MAX];
+    __shared__ int keep[PLAN_MAX];
+    __shared__ int scan[PLAN_MAX];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int np = int(k2 - k1);
+    for (int i = tid; i < np; i += nt) {
+        int64_t p = ipiv[k1 + i];
+        piv[i] = (p >= k1 + i && p < nrows) ? p : k1 + i;   // out-of-range -> no swap
+    }
+    __syncthreads();
+    for (int i = tid; i < np; i += nt) {
+        int64_t p = piv[i];
+        int k = p >= k2 ? 1 : 0;
+        for (int i2 = 0; i2 < i && k; ++i2) if (piv[i2] == p) k = 0;
+        keep[i] = k;
+        scan[i] = k;
+    }
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for (int off = 1; off < np; off <<= 1) {
+        int vals[2];
+        int cnt = 0;
+        for (int i = tid; i < np; i += nt) vals[cnt++] = (i >= off) ? scan[i - off] : 0;
+        __syncthreads();
+        cnt = 0;
+        for (int i = tid; i < np; i += nt) scan[i] += vals[cnt++];
+        __syncthreads();
+    }
+    const int nkeep = np > 0 ? scan[np - 1] : 0;
+    const int count = np + nkeep;
+    auto src_of = [&](int64_t r) {
+        int64_t x = r;
+        for (int i = np - 1; i >= 0; --i) {
+            int64_t a = k1 + i, b = piv[i];
+            if (x == a) x = b;
+            else if (x == b) x = a;
+        }
+        return x;
+    };
+    for (int e = tid; e < np; e += nt) {
+        int64_t r = k1 + e;
+        plan[1 + 2 * e] = r;
+        plan[2 + 2 * e] = src_of(r);
+    }
+    for (int i = tid; i < np; i += nt)
+        if (keep[i]) {
+            int e = np + scan[i] - 1;
+            int64_t r = piv[i];
+            plan[1 + 2 * e] = r;
+            plan[2 + 2 * e] = src_of(r);
+        }
+    if (tid == 0) plan[0] = count;
+}
+
+template <class T>
+__global__ __launch_bounds__(256)
+void laswp_apply_kernel(T* __restrict__ A, int64_t ld, int64_t c0, int64_t c1, int ch,
+                        const int64_t* __restrict__ plan)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* buf = reinterpret_cast<T*>(smem_raw);
+    const int count = int(plan[0]);
+    const int64_t cb = c0 + int64_t(blockIdx.x) * ch;
+    const int tid = threadIdx.x;
+    const int c = tid % ch, e0 = tid / ch, es = 256 / ch;
+    const bool ok = cb + c < c1;
+    for (int e = e0; e < count; e += es) {
+        int64_t src = plan[2 + 2 * e];
+        buf[e * ch + c] = ok ? A[src * ld + cb + c] : T(0);
+    }
+    __syncthreads();
+    for (int e = e0; e < count; e += es) {
+        int64_t dst = plan[1 + 2 * e];
+        if (ok) A[dst * ld + cb + c] = buf[e * ch + c];
+    }
+}
+
+void laswp_plan(int64_t k1, int64_t k2, int64_t nrows, const int64_t* ipiv, int64_t* plan,
+                hipStream_t s)
+{
+    SA_REQUIRE(k2 - k1 <= PLAN_MAX, -2, "laswp: too many pivots in one call");
+    if (k2 <= k1) return;
+    hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(1024), 0, s, k1, k2, nrows, ipiv, plan);
+    SA_CHECK_LAUNCH();
+}
+
+__global__ void laswp_invert_kernel(int64_t* plan, int64_t maxcount)
+{
+    const int64_t count = plan[0];
+    for (int64_t e = threadIdx.x; e < count && e < maxcount; e += blockDim.x) {
+        int64_t d = plan[1 + 2 * e];
+        plan[1 + 2 * e] = plan[2 + 2 * e];
+        plan[2 + 2 * e] = d;
+    }
+}
+
+void laswp_invert_plan(int64_t* plan, int64_t npiv, hipStream_t s)
+{
+    if (npiv <= 0) return;
+    hipLaunchKernelGGL(laswp_invert_kernel, dim3(1), dim3(1024), 0, s, plan, 2 * npiv);
+    SA_CHECK_LAUNCH();
+}
+
+template <class T>
+void laswp(T* A, int64_t ld, int64_t nrows, int64_t c0, int64_t c1, int64_t k1, int64_t k2,
+           const int64_t* ipiv, int64_t* plan, hipStream_t s, bool plan_ready)
+{
+    if (k2 <= k1) return;
+    if (!plan_ready) laswp_plan(k1, k2, nrows, ipiv, plan, s);
+    if (c1 <= c0) return;
+    const int64_t maxcount = 2 * (k2 - k1);
+    // column chunk: power of two, LDS staging <= 64 KiB, 4..256 columns
+    int ch = 256;
+    while (ch > 4 && int64_t(ch) * maxcount * int64_t(sizeof(T)) > 65536) ch /= 2;
+    while (ch > 4 && ch / 2 >= (c1 - c0)) ch /= 2;
+    size_t lds = size_t(ch) * size_t(maxcount) * sizeof(T);
+    SA_REQUIRE(lds <= 65536, -2, "laswp: LDS staging too large");
+    int64_t nblk = ceildiv(c1 - c0, ch);
+    hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3(unsigned(nblk)), dim3(256), lds, s,
+                       A, ld, c0, c1, ch, plan);
+    SA_CHECK_LAUNCH();
+}
+
+template void laswp<double>(double*, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,
+                            const int64_t*, int64_t*, hipStream_t, bool);
+template void laswp<float>(float*, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,
+                           const int64_t*, int64_t*, hipStream_t, bool);
+
+} // namespace sa

@@ -1,0 +1,295 @@
+// Small dense tile kernels: triangular-solve leaf, Cholesky leaf, layout
+// transposes and precision conversions (gfx950).
+//
+// trsm_leaf and potrf_leaf are the diagonal-block work of the blocked
+// triangular solve and Cholesky; the blocks above them are recursive and run
+// their off-diagonal updates through the MFMA GEMM.  They replace the vendor
+// batch::trsm (src/internal/internal_trsm.cc:244-249) and device lapack::potrf
+// (src/internal/internal_potrf.cc:72-78, which also synchronised the host per
+// step to read info; here info stays on the device).
+// transpose_inplace is the square in-place layout flip that SLATE performs per
+// tile with device::transpose (src/cuda/device_transpose.cu:282-316) when it
+// moves tiles to RowMajor for the GPU LU (src/getrf.cc:54-55).
+// copy_convert / transpose_convert are the precision casts of slate::copy
+// (src/internal/internal_gecopy.cc:121 -> device_gecopy.cu:40-115).
+#include "common.hh"
+#include "kernels.hh"
+
+namespace sa {
+
+//------------------------------------------------------------------------------
+// One thread per right-hand-side column; the triangle lives in LDS and is
+// padded to 32 with an identity so the fully unrolled substitution has no
+// data-dependent bounds.
+// FULL (nb == TRSM_LEAF) has no row predicates: predicated stores of the
+// unrolled register array make the compiler spill it.
+template <class T, int LOWER, int UNIT, int CM, int FULL>
+__global__ __launch_bounds__(256)
+void trsm_leaf_kernel(const T* __restrict__ M, int64_t ld, int nb, T* __restrict__ B,
+                      int64_t ldb, int64_t N)
+{
+    constexpr int NB = TRSM_LEAF;
+    __shared__ T L[NB][NB + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += 256) {
+        int i = e / NB, k = e % NB;
+        T val;
+        if (i < nb && k < nb) val = CM ? M[i + int64_t(k) * ld] : M[int64_t(i) * ld + k];
+        else val = (i == k) ? T(1) : T(0);
+        L[i][k] = val;
+    }
+    __syncthreads();
+    const int64_t j = int64_t(blockIdx.x) * 256 + tid;
+    if (j >= N) return;
+    T x[NB];
+    #pragma unroll
+    for (int i = 0; i < NB; ++i) x[i] = (FULL || i < nb) ? B[int64_t(i) * ldb + j] : T(0);
+    // Row-oriented substitution; the scheduling barrier per row keeps the
+    // compiler from hoisting every LDS read of the triangle (which spills).
+    if (LOWER) {
+        #pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            T s = x[i];
+            #pragma unroll
+            for (int k = 0; k < i; ++k) s -= L[i][k] * x[k];
+            x[i] = UNIT ? s : s / L[i][i];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    else {
+        #pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            T s = x[i];
+            #pragma unroll
+            for (int k = i + 1; k < NB; ++k) s -= L[i][k] * x[k];
+            x[i] = UNIT ? s : s / L[i][i];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    #pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (FULL || i < nb) B[int64_t(i) * ldb + j] = x[i];
+}
+
+template <class T>
+void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
+               T* B, int64_t ldb, int64_t N, hipStream_t s)
+{
+    if (N <= 0 || nb <= 0) return;
+    SA_REQUIRE(nb <= TRSM_LEAF, -2, "trsm_leaf: block larger than the leaf");
+    dim3 grid(unsigned(ceildiv(N, 256))), blk(256);
+    int key = (lower ? 4 : 0) | (unit ? 2 : 0) | (cm ? 1 : 0);
+    int inb = int(nb);
+#define SA_TRSM_CASE(K, LO, UN, CMV)                                                              \
+    case K:                                                                                       \
+        if (nb == TRSM_LEAF)                                                                      \
+            hipLaunchKernelGGL((trsm_leaf_kernel<T, LO, UN, CMV, 1>), grid, blk, 0, s, M, ld, inb, B, ldb, N); \
+        else                                                                                      \
+            hipLaunchKernelGGL((trsm_leaf_kernel<T, LO, UN, CMV, 0>), grid, blk, 0, s, M, ld, inb, B, ldb, N); \
+        break;
+    switch (key) {
+        SA_TRSM_CASE(0, 0, 0, 0)
+        SA_TRSM_CASE(1, 0, 0, 1)
+        SA_TRSM_CASE(2, 0, 1, 0)
+        SA_TRSM_CASE(3, 0, 1, 1)
+        SA_TRSM_CASE(4, 1, 0, 0)
+        SA_TRSM_CASE(5, 1, 0, 1)
+        SA_TRSM_CASE(6, 1, 1, 0)
+        default:
+        SA_TRSM_CASE(7, 1, 1, 1)
+    }
+#undef SA_TRSM_CASE
+    SA_CHECK_LAUNCH();
+}
+
+template void trsm_leaf<double>(const double*, int64_t, int, int, int, int64_t, double*, int64_t,
+                                int64_t, hipStream_t);
+template void trsm_leaf<float>(const float*, int64_t, int, int, int, int64_t, float*, int64_t,
+                               int64_t, hipStream_t);
+
+//------------------------------------------------------------------------------
+// Right-looking upper Cholesky (A = U^T U) of one nb x nb block in LDS.
+// Row-major upper == column-major lower, so this is LAPACK's lower dpotrf on
+// the caller's column-major tile.  Stops at the first non-positive pivot.
+template <class T>
+__global__ __launch_bounds__(256)
+void potrf_leaf_kernel(T* __restrict__ A, int64_t ld, int nb, int64_t col0, int64_t* info)
+{
+    constexpr int NB = 64;
+    __shared__ T S[NB][NB + 1];
+    __shared__ int fail;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e / nb, j = e % nb;
+        if (j >= i) S[i][j] = A[int64_t(i) * ld + j];
+    }
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    int nbad = nb;
+    for (int k = 0; k < nb; ++k) {
+        if (tid == 0) {
+            T d = S[k][k];
+            if (!(d > T(0))) {
+                fail = 1;
+                if (*info == 0) *info = col0 + k + 1;
+            }
+            else S[k][k] = sqrt(d);
+        }
+        __syncthreads();
+        if (fail) { nbad = k; break; }
+        const T dk = S[k][k];
+        if (tid > k && tid < nb) S[k][tid] /= dk;
+        __syncthreads();
+        const int r = nb - k - 1;
+        for (int e = tid; e < r * r; e += 256) {
+            int i = k + 1 + e / r, j = k + 1 + e % r;
+            if (j >= i) S[i][j] -= S[k][i] * S[k][j];
+        }
+        __syncthreads();
+    }
+    (void)nbad;
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e / nb, j = e % nb;
+        if (j >= i) A[int64_t(i) * ld + j] = S[i][j];
+    }
+}
+
+template <class T>
+void potrf_leaf(T* A, int64_t ld, int64_t nb, int64_t col0, int64_t* info, hipStream_t s)
+{
+    SA_REQUIRE(nb >= 1 && nb <= 64, -2, "potrf_leaf: bad block size");
+    hipLaunchKernelGGL(potrf_leaf_kernel<T>, dim3(1), dim3(256), 0, s, A, ld, int(nb), col0, info);
+    SA_CHECK_LAUNCH();
+}
+
+template void potrf_leaf<double>(double*, int64_t, int64_t, int64_t, int64_t*, hipStream_t);
+template void potrf_leaf<float>(float*, int64_t, int64_t, int64_t, int64_t*, hipStream_t);
+
+//------------------------------------------------------------------------------
+// In-place square transpose by 32 x 32 tile pairs (ti <= tj), one pair per block.
+template <class T>
+__global__ __launch_bounds__(256)
+void transpose_inplace_kernel(T* __restrict__ A, int64_t n, int64_t ld, int64_t ntile)
+{
+    __shared__ T X[32][33];
+    __shared__ T Y[32][33];
+    // decode blockIdx.x -> (ti, tj), ti <= tj, enumerated column by column
+    int64_t b = blockIdx.x;
+    int64_t tj = int64_t((sqrt(8.0 * double(b) + 1.0) - 1.0) * 0.5);
+    while ((tj + 1) * (tj + 2) / 2 <= b) ++tj;
+    while (tj * (tj + 1) / 2 > b) --tj;
+    int64_t ti = b - tj * (tj + 1) / 2;
+    const int64_t I0 = ti * 32, J0 = tj * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    (void)ntile;
+    // X[a][b] = A(I0+a, J0+b) where A(i,j) = A[i + j*ld]
+    for (int bb = ty; bb < 32; bb += 8) {
+        int64_t i = I0 + tx, j = J0 + bb;
+        X[tx][bb] = (i < n && j < n) ? A[i + j * ld] : T(0);
+    }
+    if (ti != tj) {
+        for (int aa = ty; aa < 32; aa += 8) {
+            int64_t i = J0 + tx, j = I0 + aa;   // Y[b][a] = A(J0+b, I0+a)
+            Y[tx][aa] = (i < n && j < n) ? A[i + j * ld] : T(0);
+        }
+    }
+    __syncthreads();
+    if (ti != tj) {
+        // A(I0+a, J0+b) <- Y[b][a];  A(J0+b, I0+a) <- X[a][b]
+        for (int bb = ty; bb < 32; bb += 8) {
+            int64_t i = I0 + tx, j = J0 + bb;
+            if (i < n && j < n) A[i + j * ld] = Y[bb][tx];
+        }
+        for (int aa = ty; aa < 32; aa += 8) {
+            int64_t i = J0 + tx, j = I0 + aa;
+            if (i < n && j < n) A[i + j * ld] = X[aa][tx];
+        }
+    }
+    else {
+        for (int bb = ty; bb < 32; bb += 8) {
+            int64_t i = I0 + tx, j = J0 + bb;
+            if (i < n && j < n) A[i + j * ld] = X[bb][tx];
+        }
+    }
+}
+
+template <class T>
+void transpose_inplace(T* A, int64_t n, int64_t ld, hipStream_t s)
+{
+    if (n <= 1) return;
+    int64_t nt = ceildiv(n, 32);
+    int64_t nblk = nt * (nt + 1) / 2;
+    SA_REQUIRE(nblk < (int64_t(1) << 31), -2, "transpose: too large");
+    hipLaunchKernelGGL(transpose_inplace_kernel<T>, dim3(unsigned(nblk)), dim3(256), 0, s, A, n, ld, nt);
+    SA_CHECK_LAUNCH();
+}
+
+template void transpose_inplace<double>(double*, int64_t, int64_t, hipStream_t);
+template void transpose_inplace<float>(float*, int64_t, int64_t, hipStream_t);
+
+//------------------------------------------------------------------------------
+template <class D, class S> __device__ inline D cvt(S x) { return D(x); }
+
+// dst[j*ldd + i] = src[i + j*lds]: column-major src -> row-major dst.
+template <class S, class D>
+__global__ __launch_bounds__(256)
+void transpose_convert_kernel(const S* __restrict__ src, int64_t lds, D* __restrict__ dst,
+                              int64_t ldd, int64_t m, int64_t n)
+{
+    __shared__ D X[32][33];
+    const int64_t I0 = int64_t(blockIdx.x) * 32, J0 = int64_t(blockIdx.y) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int bb = ty; bb < 32; bb += 8) {
+        int64_t i = I0 + tx, j = J0 + bb;
+        X[tx][bb] = (i < m && j < n) ? cvt<D>(src[i + j * lds]) : D(0);
+    }
+    __syncthreads();
+    for (int aa = ty; aa < 32; aa += 8) {
+        int64_t i = I0 + aa, j = J0 + tx;
+        if (i < m && j < n) dst[i * ldd + j] = X[aa][tx];
+    }
+}
+
+template <class S, class D>
+void transpose_convert(const S* src, int64_t lds, D* dst, int64_t ldd, int64_t m, int64_t n,
+                       hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    SA_REQUIRE(ceildiv(n, 32) < 65536, -2, "transpose_convert: too many columns");
+    dim3 grid(unsigned(ceildiv(m, 32)), unsigned(ceildiv(n, 32)));
+    hipLaunchKernelGGL((transpose_convert_kernel<S, D>), grid, dim3(256), 0, s, src, lds, dst, ldd, m, n);
+    SA_CHECK_LAUNCH();
+}
+
+template <class S, class D>
+__global__ __launch_bounds__(256)
+void copy_convert_kernel(const S* __restrict__ src, int64_t lds, D* __restrict__ dst, int64_t ldd,
+                         int64_t m, int64_t n)
+{
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y)
+        for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < m; i += int64_t(gridDim.x) * 256)
+            dst[i + j * ldd] = cvt<D>(src[i + j * lds]);
+}
+
+template <class S, class D>
+void copy_convert(const S* src, int64_t lds, D* dst, int64_t ldd, int64_t m, int64_t n,
+                  hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    unsigned gx = unsigned(std::min<int64_t>(ceildiv(m, 256), 1024));
+    unsigned gy = unsigned(std::min<int64_t>(n, 32768));
+    hipLaunchKernelGGL((copy_convert_kernel<S, D>), dim3(gx, gy), dim3(256), 0, s,
+                       src, lds, dst, ldd, m, n);
+    SA_CHECK_LAUNCH();
+}
+
+#define SA_INST_CONV(S, D)                                                                       \
+    template void transpose_convert<S, D>(const S*, int64_t, D*, int64_t, int64_t, int64_t,      \
+                                          hipStream_t);                                         \
+    template void copy_convert<S, D>(const S*, int64_t, D*, int64_t, int64_t, int64_t, hipStream_t);
+SA_INST_CONV(double, double)
+SA_INST_CONV(double, float)
+SA_INST_CONV(float, double)
+SA_INST_CONV(float, float)
+
+} // namespace sa

@@ -1,0 +1,152 @@
+"""GPU parity of the LU path (getrf/getrs/gesv, gemm, matgen) against the oracle.
+
+Every call goes through the C ABI (include/slate_amd.h) via slate_amd.  Integer
+and index results (pivots, info, matgen bits) must be identical; floating-point
+factors agree with the oracle within the stated tolerance and the solutions
+meet the tester's scaled-residual bound (<= 3, stricter than test_gesv.cc's 25).
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import slate_amd
+from helpers import EPS, dev_rands, say, scaled_residual, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_generate_bitwise_equals_oracle(gpu):
+    say("generate rands/uniform 300x200 seed 3")
+    for dist in (slate_amd.DIST_UNIFORM, slate_amd.DIST_RANDS):
+        A = slate_amd.empty_colmajor(300, 200, torch.float64)
+        slate_amd.generate(A, dist, 3, ioffset=5, joffset=7)
+        ref = oracle.generate(dist, 3, 300, 200, ioff=5, joff=7)
+        np.testing.assert_array_equal(to_host(A), ref)
+    A = slate_amd.empty_colmajor(300, 200, torch.float64)
+    slate_amd.generate(A, slate_amd.DIST_RANDN, 3)
+    np.testing.assert_allclose(to_host(A), oracle.randn(3, 300, 200), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (128, 128, 16), (300, 257, 129), (1000, 513, 700)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gemm_matches_reference(gpu, m, n, k, dtype):
+    say("gemm %dx%dx%d %s" % (m, n, k, dtype))
+    rng = np.random.default_rng(m * 7 + n + k)
+    A, B, C = (rng.standard_normal(s) for s in ((m, k), (k, n), (m, n)))
+    dA, dB, dC = to_dev(A, dtype), to_dev(B, dtype), to_dev(C, dtype)
+    slate_amd.gemm(-1.0, dA, dB, 0.5, dC)
+    ref = 0.5 * C - A @ B
+    tol = 1e-13 if dtype == torch.float64 else 2e-5
+    assert np.abs(to_host(dC) - ref).max() <= tol * max(1.0, k) ** 0.5 * 4
+
+
+def test_getrf_n2000_pivots_equal_lapack_and_oracle(gpu):
+    say("dgetrf n=2000 nb=256 rands seed 0")
+    n = 2000
+    A = dev_rands(0, n, n)
+    info, ipiv = slate_amd.getrf(A, slate_amd.Options(block_size=256))
+    assert info == 0
+    ref = np.load(os.path.join(GOLD, "ipiv_lapack_n2000.npy")).astype(np.int64)
+    np.testing.assert_array_equal(ipiv, ref)
+    LU_o, ipiv_o, _ = oracle.getrf(oracle.rands(0, n, n), nb=256, ib=16)
+    np.testing.assert_array_equal(ipiv, ipiv_o)
+    # factors agree within the LU rounding bound, 4 n eps max|LU|
+    assert np.abs(to_host(A) - LU_o).max() <= 4 * n * EPS * np.abs(LU_o).max()
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 100, 257, 513, 1031])
+@pytest.mark.parametrize("nb", [32, 128])
+def test_getrf_sizes_pivots_equal_oracle(gpu, n, nb):
+    say("dgetrf n=%d nb=%d" % (n, nb))
+    A = dev_rands(100 + n, n, n)
+    info, ipiv = slate_amd.getrf(A, slate_amd.Options(block_size=nb))
+    LU_o, ipiv_o, info_o = oracle.getrf(oracle.rands(100 + n, n, n), nb=nb, ib=16)
+    assert info == info_o == 0
+    np.testing.assert_array_equal(ipiv, ipiv_o)
+    assert np.abs(to_host(A) - LU_o).max() <= 4 * max(n, 16) * EPS * np.abs(LU_o).max()
+
+
+def test_getrf_lookahead_is_bitwise_neutral(gpu):
+    say("dgetrf n=1500 nb=128 lookahead 0 vs 1")
+    n = 1500
+    A0 = dev_rands(9, n, n)
+    A1 = dev_rands(9, n, n)
+    i0, p0 = slate_amd.getrf(A0, slate_amd.Options(block_size=128, lookahead=0))
+    i1, p1 = slate_amd.getrf(A1, slate_amd.Options(block_size=128, lookahead=1))
+    np.testing.assert_array_equal(p0, p1)
+    assert torch.equal(A0, A1)
+
+
+def test_getrf_zero_column_info(gpu):
+    say("dgetrf singular n=300 zero column 77")
+    n = 300
+    H = oracle.rands(4, n, n)
+    H[:, 77] = 0.0
+    A = to_dev(H)
+    info, ipiv = slate_amd.getrf(A, slate_amd.Options(block_size=64))
+    _, ipiv_o, info_o = oracle.getrf(H, nb=64, ib=16)
+    assert info == info_o == 78
+    np.testing.assert_array_equal(ipiv, ipiv_o)
+
+
+def test_getrf_threshold_zero_means_no_pivoting(gpu):
+    say("dgetrf threshold 0 n=200")
+    n = 200
+    H = oracle.rands(21, n, n) + n * np.eye(n)
+    A = to_dev(H)
+    info, ipiv = slate_amd.getrf(A, slate_amd.Options(block_size=64, pivot_threshold=0.0))
+    np.testing.assert_array_equal(ipiv, np.arange(1, n + 1))
+    LU = to_host(A)
+    L = np.tril(LU, -1) + np.eye(n)
+    assert np.abs(H - L @ np.triu(LU)).max() < 1e-10
+
+
+@pytest.mark.parametrize("nrhs", [1, 7])
+@pytest.mark.parametrize("trans", [False, True])
+def test_getrs_matches_oracle(gpu, nrhs, trans):
+    say("dgetrs n=700 nrhs=%d trans=%s" % (nrhs, trans))
+    n = 700
+    A = dev_rands(7, n, n)
+    info, ipiv = slate_amd.getrf(A, slate_amd.Options(block_size=128))
+    B = dev_rands(8, n, nrhs)
+    slate_amd.getrs(A, ipiv, B, trans=trans)
+    H = oracle.rands(7, n, n)
+    Bh = oracle.rands(8, n, nrhs)
+    LU_o, ip_o, _ = oracle.getrf(H, nb=128, ib=16)
+    X_o = oracle.getrs(LU_o, ip_o, Bh, nb=128, trans=trans)
+    np.testing.assert_allclose(to_host(B), X_o, rtol=1e-8, atol=1e-10)
+    opA = H.T if trans else H
+    assert scaled_residual(opA, to_host(B), Bh) <= 3.0
+
+
+def test_gesv_n2000_scaled_residual(gpu):
+    say("dgesv n=2000 nb=256 nrhs=1 (config 1 shape)")
+    n = 2000
+    A = dev_rands(0, n, n)
+    B = dev_rands(1, n, 1)
+    info, ipiv = slate_amd.gesv(A, B, slate_amd.Options(block_size=256))
+    assert info == 0
+    H = oracle.rands(0, n, n)
+    b = oracle.rands(1, n, 1)
+    r = scaled_residual(H, to_host(B), b)
+    say("  scaled residual %.3e" % r)
+    assert r <= 3.0
+    _, _, X_o, _ = oracle.gesv(H, b, nb=256)
+    np.testing.assert_allclose(to_host(B), X_o, rtol=1e-8, atol=1e-10)
+
+
+def test_sgetrf_pivots_equal_oracle(gpu):
+    say("sgetrf n=400 nb=64")
+    n = 400
+    A = dev_rands(31, n, n, dtype=torch.float32)
+    info, ipiv = slate_amd.getrf(A, slate_amd.Options(block_size=64))
+    H = oracle.rands(31, n, n).astype(np.float32)
+    LU_o, ip_o, info_o = oracle.getrf(H, nb=64, ib=16, dtype=np.float32)
+    assert info == info_o == 0
+    np.testing.assert_array_equal(ipiv, ip_o)
+    eps32 = float(np.finfo(np.float32).eps)
+    assert np.abs(to_host(A) - LU_o).max() <= 4 * n * eps32 * np.abs(LU_o).max()
