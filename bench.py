@@ -1,0 +1,156 @@
+"""Benchmark: FP64-accurate gesv TFLOP/s, mixed precision + iterative refinement
+vs plain FP64 LU, n = 65536, one MI355X.
+
+One step = one slate_amd.gesv_mixed call (float LU factorization with the
+MFMA trailing updates + FP64 refinement to SLATE's stopping criterion) on a
+matrix already resident in HBM.  TFLOP/s counts FP64-equivalent LU flops
+(2 n^3 / 3), whatever precision executes them.  The same input is also solved
+once with the plain FP64 gesv for the "vs plain FP64 LU" comparison, and the
+oracle's CPU restatement of the HostTask path is timed on a bounded sample.
+
+Prints ONE JSON line (see the keys below).  --gpus > 1 is not supported: this
+path is a single-GPU factor-and-solve.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64-accurate gesv TFLOP/s, mixed+IR vs plain FP64 LU, n=65536, 1 GPU"
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def scaled_residual_dev(A, x, b):
+    """||b - A x||_1 / (n ||A||_1 ||x||_1 eps) on the GPU (checker only)."""
+    import torch
+    n = A.shape[0]
+    r = b - A @ x
+    a1 = A.abs().sum(dim=0).max()
+    return float(r.abs().sum() / (n * a1 * x.abs().sum()) / torch.finfo(torch.float64).eps)
+
+
+def cpu_baseline(n_cpu):
+    """Oracle (CPU restatement of SLATE's HostTask gesv_mixed) on a bounded sample."""
+    import oracle
+    threads = oracle.lib().oracle_num_threads()
+    A = oracle.rands(0, n_cpu, n_cpu)
+    b = oracle.rands(1, n_cpu, 1)
+    t = time.perf_counter()
+    _, _, X, info, it = oracle.gesv_mixed(A, b, nb=256, ib=16, nthreads=max(1, threads // 2))
+    dt = time.perf_counter() - t
+    tf = (2.0 / 3.0) * n_cpu ** 3 / dt / 1e12
+    return {
+        "value": round(tf, 5), "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
+        "sample": "oracle gesv_mixed n=%d nrhs=1 (float tile LU nb=256 ib=16 + FP64 IR, %d iters), "
+                  "matgen rands seed 0/1, %.1f s" % (n_cpu, it, dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--nb", type=int, default=0)
+    ap.add_argument("--cpu-n", type=int, default=12288)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this path runs on one GPU only (single-GPU factor-and-solve); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import slate_amd as S
+
+    n = args.n
+    opts = S.Options(block_size=args.nb)
+    log("bench: generating A (%d x %d, matgen rands seed 0) and b (seed 1) on the GPU" % (n, n))
+    A = S.empty_colmajor(n, n, torch.float64)
+    S.generate(A, S.DIST_RANDS, 0)
+    B = S.empty_colmajor(n, 1, torch.float64)
+    S.generate(B, S.DIST_RANDS, 1)
+    X = S.empty_colmajor(n, 1, torch.float64)
+    torch.cuda.synchronize()
+
+    iters = []
+    for w in range(args.warmup):
+        info, it, _ = S.gesv_mixed(A, B, X, opts)
+        log("bench: warmup %d info %d iter %d  %.1f ms" % (w, info, it, S.last_time_ms()))
+        assert info == 0 and it >= 0, "refinement did not converge (A would be overwritten)"
+
+    S.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for k in range(args.steps):
+        info, it, _ = S.gesv_mixed(A, B, X, opts)
+        dev_ms += S.last_time_ms()
+        iters.append(it)
+        assert info == 0 and it >= 0, "refinement did not converge"
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gemm_ms, gemm_flops, launches = S.profile_read()
+    S.profile_enable(False)
+    ms_per_step = 1e3 * wall / args.steps
+    flops = 2.0 / 3.0 * n ** 3
+    value = flops / (ms_per_step * 1e-3) / 1e12
+    res_mixed = scaled_residual_dev(A, X[:, 0], B[:, 0])
+    log("bench: gesv_mixed %.1f ms/step (device %.1f ms), %.2f TFLOP/s, iters %s, scaled residual %.3f"
+        % (ms_per_step, dev_ms / args.steps, value, iters, res_mixed))
+
+    fp64 = None
+    if not args.no_fp64:
+        A64 = A.clone()
+        B64 = B.clone()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        info64, _ = S.gesv(A64, B64, opts)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t
+        res64 = scaled_residual_dev(A, B64[:, 0], B[:, 0])
+        fp64 = {"ms": round(1e3 * t64, 2), "tflops": round(flops / t64 / 1e12, 3),
+                "scaled_residual": round(res64, 4), "info": info64}
+        log("bench: plain FP64 gesv %.1f ms, %.2f TFLOP/s, scaled residual %.3f" % (1e3 * t64, fp64["tflops"], res64))
+        del A64, B64
+        torch.cuda.empty_cache()
+
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    roofline = {
+        "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK["fp32"], 4), "traffic": None,
+        "kernel": "gemm_kernel<float> trailing update (v_mfma_f32_16x16x4f32)",
+        "launches": int(launches), "avg_launch_ms": round(gemm_ms / max(launches, 1), 4),
+    }
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 factor + f64 refinement", "data": "synthetic",
+        "config": {
+            "workload": "gesv_mixed (float LU + FP64 classic IR) n=%d nrhs=1, A = matgen rands seed 0, "
+                        "b = rands seed 1, already in HBM" % n,
+            "n": n, "nrhs": 1, "nb": args.nb or "auto", "factor_precision": "fp32",
+            "refinement": "classic IR", "iters": iters, "scaled_residual": round(res_mixed, 4),
+            "plain_fp64_gesv": fp64,
+            "speedup_vs_plain_fp64": round(fp64["ms"] / ms_per_step, 3) if fp64 else None,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -117,6 +117,16 @@ int64_t slate_amd_dsposv_gmres(slate_amd_context ctx, int uplo, int64_t n, int64
                                double* dA, int64_t lda, const double* dB, int64_t ldb,
                                double* dX, int64_t ldx, int* iter, const slate_amd_options* opts);
 
+/* ---- instrumentation ----------------------------------------------------- */
+/* Times every trailing-update GEMM launch of the factorizations with HIP
+ * events on the stream it runs on; read returns the summed device time, the
+ * algorithmic flops (2*m*n*k per launch) and the launch count since enable.
+ * Closest reference counterpart: the slate::timers map (include/slate/slate.hh
+ * :35-38) filled by the drivers' Timer blocks. */
+int slate_amd_profile_enable(slate_amd_context ctx, int on);
+int slate_amd_profile_read(slate_amd_context ctx, double* gemm_ms, double* gemm_flops,
+                           int64_t* launches);
+
 /* ---- utilities on the path ---------------------------------------------- */
 /* slate::gemm(alpha, A, B, beta, C) for column-major device matrices,
  * C (m x n) = alpha * op(A) * op(B) + beta * C, op = NoTrans only —

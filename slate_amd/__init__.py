@@ -34,7 +34,7 @@ C_SYMBOLS = [
     "slate_amd_dgesv", "slate_amd_dpotrf", "slate_amd_spotrf", "slate_amd_dpotrs",
     "slate_amd_dposv", "slate_amd_dsgesv", "slate_amd_dsgesv_gmres", "slate_amd_dsposv",
     "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate",
-    "slate_amd_dgemm", "slate_amd_sgemm",
+    "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_profile_enable", "slate_amd_profile_read",
 ]
 
 
@@ -100,6 +100,8 @@ def lib():
             "slate_amd_dsposv_gmres": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, vp, i64, pint, po]),
             "slate_amd_dnorm_inf": (i32, [vp, i32, i64, vp, i64, ctypes.POINTER(dbl)]),
             "slate_amd_generate": (i32, [vp, i32, i64, i64, i64, i64, i64, vp, i64]),
+            "slate_amd_profile_enable": (i32, [vp, i32]),
+            "slate_amd_profile_read": (i32, [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl), pi64]),
             "slate_amd_dgemm": (i32, [vp, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
             "slate_amd_sgemm": (i32, [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
                                       ctypes.c_float, vp, i64]),
@@ -290,6 +292,19 @@ def gemm(alpha, A, B, beta, C):
     pc, ldc, _ = _mat(C, ("float64", "float32"))
     f = lib().slate_amd_dgemm if C.dtype.itemsize == 8 else lib().slate_amd_sgemm
     _check(f(context(dev), C.shape[0], _ncols(C), A.shape[1], alpha, pa, lda, pb, ldb, beta, pc, ldc))
+
+
+def profile_enable(on=True, device=0):
+    """Start (or stop) timing the trailing-update GEMMs; resets the totals."""
+    _check(lib().slate_amd_profile_enable(context(device), int(on)))
+
+
+def profile_read(device=0):
+    """(gemm_ms, gemm_flops, launches) accumulated since profile_enable."""
+    ms, fl, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int64(0)
+    _check(lib().slate_amd_profile_read(context(device), ctypes.byref(ms), ctypes.byref(fl),
+                                        ctypes.byref(n)))
+    return ms.value, fl.value, n.value
 
 
 def norm_inf(A, sym=False):

@@ -391,6 +391,53 @@ void generate(int dist, int64_t seed, int64_t m, int64_t n, int64_t ioff, int64_
     }
 }
 
+//------------------------------------------------------------------------------
+// Row permutation of a vector fused with a precision cast:
+// gather: dst[i] = src[perm[i]]  (applies P: the interchanges of getrf, forward)
+// scatter: dst[perm[i]] = src[i] (applies P^T)
+template <class S, class D, int SCATTER>
+__global__ void permute_convert_kernel(const S* __restrict__ src, D* __restrict__ dst,
+                                       const int64_t* __restrict__ perm, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        if (SCATTER) dst[perm[i]] = D(src[i]);
+        else dst[i] = D(src[perm[i]]);
+    }
+}
+
+template <class S, class D>
+void permute_convert(const S* src, D* dst, const int64_t* perm, int64_t n, int scatter, hipStream_t s)
+{
+    if (n <= 0) return;
+    unsigned gx = unsigned(std::min<int64_t>(ceildiv(n, 256), 2048));
+    if (scatter) hipLaunchKernelGGL((permute_convert_kernel<S, D, 1>), dim3(gx), dim3(256), 0, s, src, dst, perm, n);
+    else hipLaunchKernelGGL((permute_convert_kernel<S, D, 0>), dim3(gx), dim3(256), 0, s, src, dst, perm, n);
+    SA_CHECK_LAUNCH();
+}
+template void permute_convert<double, float>(const double*, float*, const int64_t*, int64_t, int, hipStream_t);
+template void permute_convert<double, double>(const double*, double*, const int64_t*, int64_t, int, hipStream_t);
+template void permute_convert<float, float>(const float*, float*, const int64_t*, int64_t, int, hipStream_t);
+
+// Y += convert(X) for vectors (the "copy X_lo to R; add R to X" step of
+// src/gesv_mixed.cc:228-232 in one pass).
+template <class S>
+__global__ void add_convert_kernel(const S* __restrict__ x, double* __restrict__ y, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        y[i] += double(x[i]);
+}
+
+template <class S>
+void add_convert(const S* x, double* y, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    unsigned gx = unsigned(std::min<int64_t>(ceildiv(n, 256), 2048));
+    hipLaunchKernelGGL(add_convert_kernel<S>, dim3(gx), dim3(256), 0, s, x, y, n);
+    SA_CHECK_LAUNCH();
+}
+template void add_convert<float>(const float*, double*, int64_t, hipStream_t);
+template void add_convert<double>(const double*, double*, int64_t, hipStream_t);
+
 __global__ void zero_u32_kernel(unsigned* p, int64_t n)
 {
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) p[i] = 0;

@@ -40,10 +40,50 @@ struct Context {
     hipStream_t s1 = nullptr;          // panel + lookahead column
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
-    Buf counters, slots, err, dinfo, plan0, plan1, ipiv, work, lo, vec, basis, rhs;
+    Buf counters, slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags;
     double last_ms = 0;
     int64_t counter_next = 0, counter_cap = 0;
+    // trailing-update GEMM timing (enabled by slate_amd_profile_enable)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<double> prof_flops_pending;
+    size_t prof_used = 0;
+    double prof_ms = 0, prof_flops = 0;
+    int64_t prof_launches = 0;
 };
+
+// Event pair around one trailing-update GEMM on stream s (profiling only).
+static void prof_begin(Context& c, hipStream_t s)
+{
+    if (!c.prof) return;
+    while (c.prof_ev.size() < c.prof_used + 2) {
+        hipEvent_t e;
+        SA_HIP(hipEventCreate(&e));
+        c.prof_ev.push_back(e);
+    }
+    SA_HIP(hipEventRecord(c.prof_ev[c.prof_used], s));
+}
+static void prof_end(Context& c, hipStream_t s, double flops)
+{
+    if (!c.prof) return;
+    SA_HIP(hipEventRecord(c.prof_ev[c.prof_used + 1], s));
+    c.prof_flops_pending.push_back(flops);
+    c.prof_used += 2;
+}
+// Fold the recorded pairs into the totals (after the streams are synchronised).
+static void prof_collect(Context& c)
+{
+    if (!c.prof) return;
+    for (size_t i = 0; i + 1 < c.prof_used; i += 2) {
+        float ms = 0;
+        SA_HIP(hipEventElapsedTime(&ms, c.prof_ev[i], c.prof_ev[i + 1]));
+        c.prof_ms += ms;
+        c.prof_flops += c.prof_flops_pending[i / 2];
+        c.prof_launches += 1;
+    }
+    c.prof_used = 0;
+    c.prof_flops_pending.clear();
+}
 
 struct Opts {
     int64_t nb = 0, ib = 0, lookahead = 1, itermax = 30;
@@ -98,7 +138,9 @@ static void check_kernel_error(Context& c)
 {
     int e = 0;
     SA_HIP(hipMemcpy(&e, c.err.p, sizeof(int), hipMemcpyDeviceToHost));
-    SA_REQUIRE(e == 0, -101, "panel kernel: cross-workgroup barrier timed out");
+    if (e != 0) SA_HIP(hipMemset(c.err.p, 0, sizeof(int) * 4));
+    SA_REQUIRE(e != 1, -101, "panel kernel: cross-workgroup barrier timed out");
+    SA_REQUIRE(e != 2, -102, "triangular solve: completion chain timed out");
 }
 
 //------------------------------------------------------------------------------
@@ -211,8 +253,14 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     auto ev_panel = [&](int64_t k) { return c.evs[2 + 2 * k]; };
     auto ev_trail = [&](int64_t k) { return c.evs[3 + 2 * k]; };
 
+    // One swap plan per step, built on s1 right after the step's panel and
+    // read by both streams (plan slots are never reused within a call).
+    const int64_t plan_stride = 4 * nb + 8;
+    c.plans.ensure(sizeof(int64_t) * size_t(plan_stride) * size_t(nsteps + 1));
+    auto step_plan = [&](int64_t k) { return c.plans.as<int64_t>() + k * plan_stride; };
     // panel 0
     panel_rec<T>(c, A, ld, n, 0, std::min(nb, n), ipiv, info, o.threshold, leaf_w, plan1, s1);
+    laswp_plan(0, std::min(nb, n), n, ipiv, step_plan(0), s1);
     SA_HIP(hipEventRecord(ev_panel(0), s1));
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t k0 = k * nb, kb = std::min(nb, n - k0);
@@ -220,13 +268,14 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         const int64_t r = n - c1;                     // trailing size
         SA_HIP(hipStreamWaitEvent(s0, ev_panel(k), 0));
         // pivots of step k applied to the columns on the left (finished L) on s0
-        laswp<T>(A, ld, n, 0, k0, k0, c1, ipiv, plan0, s0);
+        int64_t* pk = step_plan(k);
+        laswp<T>(A, ld, n, 0, k0, k0, c1, ipiv, pk, s0, true);
         if (r <= 0) break;
         const int64_t nb1 = std::min(nb, r);          // lookahead column block width
         if (la && r > nb1) {
             // s1: step k on column block k+1, then panel k+1
             SA_HIP(hipStreamWaitEvent(s1, k > 0 ? ev_trail(k) : ev_panel(0), 0));
-            laswp<T>(A, ld, n, c1, c1 + nb1, k0, c1, ipiv, plan1, s1);
+            laswp<T>(A, ld, n, c1, c1 + nb1, k0, c1, ipiv, pk, s1, true);
             trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c1, ld, nb1, s1);
             GemmArgs g{};
             g.M = r; g.N = nb1; g.K = kb;
@@ -236,20 +285,23 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
             gemm<T>(g, A_KC, B_NC, s1);
             panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
+            laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
             // s0: step k on the columns beyond k+1
             const int64_t c2 = c1 + nb1;
-            laswp<T>(A, ld, n, c2, n, k0, c1, ipiv, plan0, s0);
+            laswp<T>(A, ld, n, c2, n, k0, c1, ipiv, pk, s0, true);
             trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c2, ld, n - c2, s0);
             g.N = n - c2;
             g.B = A + k0 * ld + c2;
             g.C = A + c1 * ld + c2;
+            prof_begin(c, s0);
             gemm<T>(g, A_KC, B_NC, s0);
+            prof_end(c, s0, 2.0 * double(g.M) * double(g.N) * double(g.K));
             SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
         }
         else {
             // no lookahead: whole trailing matrix on s0, then the next panel
-            laswp<T>(A, ld, n, c1, n, k0, c1, ipiv, plan0, s0);
+            laswp<T>(A, ld, n, c1, n, k0, c1, ipiv, pk, s0, true);
             trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c1, ld, r, s0);
             GemmArgs g{};
             g.M = r; g.N = r; g.K = kb;
@@ -257,18 +309,25 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             g.B = A + k0 * ld + c1; g.ldb = ld;
             g.C = A + c1 * ld + c1; g.ldc = ld;
             g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
+            prof_begin(c, s0);
             gemm<T>(g, A_KC, B_NC, s0);
+            prof_end(c, s0, 2.0 * double(g.M) * double(g.N) * double(g.K));
             if (la) {
                 SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
                 SA_HIP(hipStreamWaitEvent(s1, ev_trail(k + 1), 0));
             }
             panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
+            laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
         }
     }
     if (la) {
         SA_HIP(hipEventRecord(c.evs[1], s1));
         SA_HIP(hipStreamWaitEvent(s0, c.evs[1], 0));
+    }
+    if (c.prof) {
+        SA_HIP(hipStreamSynchronize(s0));
+        prof_collect(c);
     }
 }
 
@@ -318,6 +377,62 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
 
 // Solve with LU factors in row-major (cm = 0) or column-major (cm = 1) layout;
 // B row-major n x nrhs (ldb).  src/getrs.cc:82-115.
+// Row permutation equivalent to the interchange sequence: perm[i] = original
+// row that getrf moves to position i (swap perm[i] <-> perm[ipiv[i]] in order).
+static void perm_from_ipiv(const int64_t* ipiv0, int64_t n, std::vector<int64_t>& perm)
+{
+    perm.resize(n);
+    for (int64_t i = 0; i < n; ++i) perm[i] = i;
+    for (int64_t i = 0; i < n; ++i) std::swap(perm[i], perm[ipiv0[i]]);
+}
+
+static int64_t* upload_perm(Context& c, const int64_t* ipiv0_host, int64_t n)
+{
+    std::vector<int64_t> perm;
+    perm_from_ipiv(ipiv0_host, n, perm);
+    c.perm.ensure(sizeof(int64_t) * std::max<int64_t>(n, 1));
+    SA_HIP(hipMemcpyAsync(c.perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+    return c.perm.as<int64_t>();
+}
+
+static unsigned* tv_counter(Context& c)
+{
+    c.flags.ensure(sizeof(unsigned) * 16);
+    return c.flags.as<unsigned>();
+}
+
+static int* err_flag(Context& c)
+{
+    if (c.err.bytes == 0) {
+        c.err.ensure(sizeof(int) * 4);
+        SA_HIP(hipMemsetAsync(c.err.p, 0, sizeof(int) * 4, c.s0));
+    }
+    return c.err.as<int>();
+}
+
+// Single right-hand side: x := op(LU)^{-1} x with x contiguous, using the
+// one-launch look-back triangular solves.  perm is the getrf permutation.
+template <class T>
+static void getrs_vec(Context& c, const T* LU, int64_t ld, int cm, int64_t n, const int64_t* perm,
+                      T* x, T* tmp, int trans, hipStream_t s)
+{
+    unsigned* ctr = tv_counter(c);
+    int* err = err_flag(c);
+    if (!trans) {
+        permute_convert<T, T>(x, tmp, perm, n, 0, s);
+        trsv_lookback<T>(LU, ld, cm, 1, 1, n, tmp, ctr, err, s);
+        trsv_lookback<T>(LU, ld, cm, 0, 0, n, tmp, ctr, err, s);
+        copy_convert<T, T>(tmp, n, x, n, n, 1, s);
+    }
+    else {
+        trsv_lookback<T>(LU, ld, 1 - cm, 1, 0, n, x, ctr, err, s);
+        trsv_lookback<T>(LU, ld, 1 - cm, 0, 1, n, x, ctr, err, s);
+        permute_convert<T, T>(x, tmp, perm, n, 1, s);
+        copy_convert<T, T>(tmp, n, x, n, n, 1, s);
+    }
+}
+
 template <class T>
 static void getrs_any(Context& c, const T* LU, int64_t ld, int cm, int64_t n, const int64_t* ipiv,
                       T* B, int64_t ldb, int64_t nrhs, int trans, hipStream_t s)
@@ -436,6 +551,17 @@ template <class T>
 static void getrs_cm(Context& c, int trans, const T* A, int64_t lda, int64_t n, const int64_t* d_ipiv,
                      T* B, int64_t ldb, int64_t nrhs)
 {
+    if (nrhs == 1) {
+        std::vector<int64_t> ip(n);
+        SA_HIP(hipMemcpyAsync(ip.data(), d_ipiv, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c.s0));
+        SA_HIP(hipStreamSynchronize(c.s0));
+        const int64_t* perm = upload_perm(c, ip.data(), n);
+        c.rhs.ensure(sizeof(T) * size_t(n));
+        getrs_vec<T>(c, A, lda, 1, n, perm, B, c.rhs.as<T>(), trans, c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        check_kernel_error(c);
+        return;
+    }
     T* W = rhs_rm<T>(c, c.rhs, B, ldb, n, nrhs);
     getrs_any<T>(c, A, lda, 1, n, d_ipiv, W, nrhs, nrhs, trans, c.s0);
     rhs_back<T>(c, W, B, ldb, n, nrhs);
@@ -457,6 +583,18 @@ static int64_t potrf_cm(Context& c, int uplo, T* A, int64_t n, int64_t lda, cons
 template <class T>
 static void potrs_cm(Context& c, int uplo, T* A, int64_t lda, int64_t n, T* B, int64_t ldb, int64_t nrhs)
 {
+    if (nrhs == 1) {
+        unsigned* ctr = tv_counter(c);
+        int* err = err_flag(c);
+        // Lower column-major L (A = L L^T): L y = b, L^T x = y.
+        // Upper column-major U (A = U^T U): U^T y = b, U x = y.
+        const int cmL = (uplo == SLATE_AMD_UPPER) ? 0 : 1;
+        trsv_lookback<T>(A, lda, cmL, 1, 0, n, B, ctr, err, c.s0);
+        trsv_lookback<T>(A, lda, 1 - cmL, 0, 0, n, B, ctr, err, c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        check_kernel_error(c);
+        return;
+    }
     T* W = rhs_rm<T>(c, c.rhs, B, ldb, n, nrhs);
     if (uplo == SLATE_AMD_UPPER) {
         // column-major upper U (A = U^T U) is row-major lower U^T
@@ -476,7 +614,9 @@ static void potrs_cm(Context& c, int uplo, T* A, int64_t lda, int64_t n, T* B, i
 struct MixedState {
     float* Alo;           // row-major n x n
     int64_t* ipiv;        // device
+    int64_t* perm;        // device, getrf permutation of the float factor
     float* xlo;           // row-major n x nrhs
+    float* xtmp;          // scratch vector
     double* R;            // column-major n x nrhs
     double* work;
     int64_t work_elems;
@@ -514,6 +654,28 @@ static void lo_solve(Context& c, const MixedState& st, int sym, int64_t n, int64
     else getrs_any<float>(c, st.Alo, n, 0, n, st.ipiv, st.xlo, nrhs, nrhs, 0, c.s0);
 }
 
+// Single vector: dst (+)= A_lo^{-1} src with the float factor, the permutation
+// fused into the cast to float and the cast back fused into the update
+// (copy / getrs / copy / add of src/gesv_mixed.cc:220-232).
+static void lo_solve_vec(Context& c, const MixedState& st, int sym, int64_t n, const double* src,
+                         double* dst, bool accumulate)
+{
+    unsigned* ctr = tv_counter(c);
+    int* err = err_flag(c);
+    if (sym) {
+        copy_convert<double, float>(src, n, st.xlo, n, n, 1, c.s0);
+        trsv_lookback<float>(st.Alo, n, 1, 1, 0, n, st.xlo, ctr, err, c.s0);   // U^T y = b
+        trsv_lookback<float>(st.Alo, n, 0, 0, 0, n, st.xlo, ctr, err, c.s0);   // U x = y
+    }
+    else {
+        permute_convert<double, float>(src, st.xlo, st.perm, n, 0, c.s0);
+        trsv_lookback<float>(st.Alo, n, 0, 1, 1, n, st.xlo, ctr, err, c.s0);   // L y = P b
+        trsv_lookback<float>(st.Alo, n, 0, 0, 0, n, st.xlo, ctr, err, c.s0);   // U x = y
+    }
+    if (accumulate) add_convert<float>(st.xlo, dst, n, c.s0);
+    else copy_convert<float, double>(st.xlo, n, dst, n, n, 1, c.s0);
+}
+
 // x_lo (row-major float) <- V (column-major double)
 static void to_lo(Context& c, const double* V, int64_t ldv, float* xlo, int64_t n, int64_t nrhs)
 {
@@ -533,13 +695,15 @@ static MixedState mixed_setup(Context& c, int64_t n, int64_t nrhs)
     c.lo.ensure(sizeof(float) * size_t(n) * size_t(n) + 256);
     st.Alo = c.lo.as<float>();
     int64_t we = std::max<int64_t>(residual_work_elems(n, 4), ceildiv(n, 1024) * n + n + 64);
-    size_t vec_bytes = sizeof(int64_t) * n + sizeof(float) * n * nrhs + sizeof(double) * n * nrhs
+    size_t vec_bytes = 2 * sizeof(int64_t) * n + 2 * sizeof(float) * n * nrhs + sizeof(double) * n * nrhs
                        + sizeof(double) * we + sizeof(double) * (nrhs + 8) + 1024;
     c.vec.ensure(vec_bytes);
     char* p = static_cast<char*>(c.vec.p);
     auto take = [&](size_t b) { char* q = p; p += (b + 255) & ~size_t(255); return q; };
     st.ipiv = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * n));
+    st.perm = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * n));
     st.xlo = reinterpret_cast<float*>(take(sizeof(float) * n * nrhs));
+    st.xtmp = reinterpret_cast<float*>(take(sizeof(float) * n * nrhs));
     st.R = reinterpret_cast<double*>(take(sizeof(double) * n * nrhs));
     st.work = reinterpret_cast<double*>(take(sizeof(double) * we));
     st.work_elems = we;
@@ -565,7 +729,13 @@ static int64_t factor_lo(Context& c, MixedState& st, const double* A, int64_t ld
         getrf_rm<float>(c, st.Alo, n, n, o, st.ipiv, c.dinfo.as<int64_t>());
     }
     SA_HIP(hipStreamSynchronize(c.s0));
-    if (!sym) check_kernel_error(c);
+    if (!sym) {
+        check_kernel_error(c);
+        std::vector<int64_t> ip(n), perm;
+        SA_HIP(hipMemcpy(ip.data(), st.ipiv, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+        perm_from_ipiv(ip.data(), n, perm);
+        SA_HIP(hipMemcpy(st.perm, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    }
     return read_info(c);
 }
 
@@ -591,18 +761,28 @@ static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, 
         *iter = -3;
     }
     else {
-        to_lo(c, B, ldb, st.xlo, n, nrhs);
-        lo_solve(c, st, sym, n, nrhs);
-        from_lo(c, st.xlo, X, ldx, n, nrhs);
+        if (nrhs == 1) {
+            lo_solve_vec(c, st, sym, n, B, X, false);
+        }
+        else {
+            to_lo(c, B, ldb, st.xlo, n, nrhs);
+            lo_solve(c, st, sym, n, nrhs);
+            from_lo(c, st.xlo, X, ldx, n, nrhs);
+        }
         residual(A, n, lda, sym, X, ldx, B, ldb, st.R, n, nrhs, st.work, st.work_elems, c.s0);
         colnorms_host(c, X, n, ldx, nrhs, st.norms + 1, nx);
         colnorms_host(c, st.R, n, n, nrhs, st.norms + 1, nr);
         if (converged_host(nr, nx, cte)) { *iter = 0; converged = true; }
         for (int64_t it = 0; it < itermax && !converged; ++it) {
-            to_lo(c, st.R, n, st.xlo, n, nrhs);
-            lo_solve(c, st, sym, n, nrhs);
-            from_lo(c, st.xlo, st.R, n, n, nrhs);
-            axpy(n, nrhs, 1.0, st.R, n, X, ldx, c.s0);
+            if (nrhs == 1) {
+                lo_solve_vec(c, st, sym, n, st.R, X, true);
+            }
+            else {
+                to_lo(c, st.R, n, st.xlo, n, nrhs);
+                lo_solve(c, st, sym, n, nrhs);
+                from_lo(c, st.xlo, st.R, n, n, nrhs);
+                axpy(n, nrhs, 1.0, st.R, n, X, ldx, c.s0);
+            }
             residual(A, n, lda, sym, X, ldx, B, ldb, st.R, n, nrhs, st.work, st.work_elems, c.s0);
             colnorms_host(c, X, n, ldx, nrhs, st.norms + 1, nx);
             colnorms_host(c, st.R, n, n, nrhs, st.norms + 1, nr);
@@ -628,6 +808,7 @@ static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, 
         ipiv_to_host(c, st.ipiv, n, ipiv_host);
     }
     SA_HIP(hipStreamSynchronize(c.s0));
+    check_kernel_error(c);
     return info;
 }
 
@@ -690,9 +871,7 @@ static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, 
     else {
         const double negone = -1.0;
         SA_HIP(hipMemcpyAsync(dsc + 129, &negone, sizeof(double), hipMemcpyHostToDevice, c.s0));
-        to_lo(c, B, n, st.xlo, n, 1);
-        lo_solve(c, st, sym, n, 1);
-        from_lo(c, st.xlo, X, n, n, 1);
+        lo_solve_vec(c, st, sym, n, B, X, false);
         int64_t iiter = 0;
         while (iiter < itermax) {
             residual(A, n, lda, sym, X, n, B, n, st.R, n, 1, st.work, st.work_elems, c.s0);
@@ -716,9 +895,7 @@ static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, 
                 double* Vj1 = V + (j + 1) * n;
                 double* Wj1 = W + (j + 1) * n;
                 // W(:, j+1) = M^{-1} V(:, j) in low precision (:281-286)
-                to_lo(c, Vj, n, st.xlo, n, 1);
-                lo_solve(c, st, sym, n, 1);
-                from_lo(c, st.xlo, Wj1, n, n, 1);
+                lo_solve_vec(c, st, sym, n, Vj, Wj1, false);
                 // V(:, j+1) = A W(:, j+1) in FP64 (:288-293): R = 0 - A w, then negate
                 zero_f64(st.R, n, c.s0);
                 residual(A, n, lda, sym, Wj1, n, st.R, n, Vj1, n, 1, st.work, st.work_elems, c.s0);
@@ -777,6 +954,7 @@ static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, 
         ipiv_to_host(c, st.ipiv, n, ipiv_host);
     }
     SA_HIP(hipStreamSynchronize(c.s0));
+    check_kernel_error(c);
     return info;
 }
 
@@ -864,7 +1042,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.counters, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
-                       &c.lo, &c.vec, &c.basis, &c.rhs})
+                       &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);
@@ -1122,6 +1300,26 @@ int slate_amd_sgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, floa
                     const float* dA, int64_t lda, const float* dB, int64_t ldb, float beta,
                     float* dC, int64_t ldc)
 { return gemm_entry<float>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc); }
+
+int slate_amd_profile_enable(slate_amd_context ctx, int on)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        c.prof = on != 0;
+        c.prof_ms = 0; c.prof_flops = 0; c.prof_launches = 0;
+        c.prof_used = 0; c.prof_flops_pending.clear();
+        return 0;
+    }));
+}
+
+int slate_amd_profile_read(slate_amd_context ctx, double* gemm_ms, double* gemm_flops, int64_t* launches)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        if (gemm_ms) *gemm_ms = c.prof_ms;
+        if (gemm_flops) *gemm_flops = c.prof_flops;
+        if (launches) *launches = c.prof_launches;
+        return 0;
+    }));
+}
 
 int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double* dA, int64_t lda,
                         double* result)

@@ -60,6 +60,13 @@ template <class T>
 void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
                T* B, int64_t ldb, int64_t N, hipStream_t s);
 
+// One-launch triangular solve of tri * x = b (one right-hand side, x overwrites
+// b, contiguous) with a completion-counter chain between 128-row blocks.
+// ndone: one zeroed-on-launch counter; error: set to 2 if a wait times out.
+template <class T>
+void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* X,
+                   unsigned* ndone, int* error, hipStream_t s);
+
 // Cholesky leaf on an nb x nb row-major block (nb <= 64), upper factor U with
 // A = U^T U stored in the upper triangle.  Sets *info = col0 + j + 1 at the
 // first non-positive pivot (if *info == 0).
@@ -104,6 +111,13 @@ void multi_axpy(const double* V, int64_t ldv, int64_t n, int64_t k, const double
 // out[0] = ||w||_2
 void nrm2(const double* w, int64_t n, double* work, double* out, hipStream_t s);
 void scale(double* w, int64_t n, const double* denom_dev, hipStream_t s);
+
+// dst[i] = src[perm[i]] (scatter = 0) or dst[perm[i]] = src[i] (scatter = 1),
+// with conversion; perm is the row permutation equivalent to getrf's ipiv.
+template <class S, class D>
+void permute_convert(const S* src, D* dst, const int64_t* perm, int64_t n, int scatter, hipStream_t s);
+// y += convert(x) for vectors.
+template <class S> void add_convert(const S* x, double* y, int64_t n, hipStream_t s);
 
 // Matrix generation (SLATE matgen Philox): A[i + j*lda] = f(seed, i+ioff, j+joff).
 void generate(int dist, int64_t seed, int64_t m, int64_t n, int64_t ioff, int64_t joff,

@@ -302,8 +302,10 @@ void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __r
     __syncthreads();
     for (int i = tid; i < np; i += nt) {
         int64_t p = piv[i];
-        int k = p >= k2 ? 1 : 0;
-        for (int i2 = 0; i2 < i && k; ++i2) if (piv[i2] == p) k = 0;
+        int k = 1;
+        // independent LDS reads, no loop-carried exit: pipelines well
+        for (int i2 = 0; i2 < i; ++i2) k &= (piv[i2] != p);
+        k &= (p >= k2);
         keep[i] = k;
         scan[i] = k;
     }

@@ -23,12 +23,11 @@ namespace sa {
 // data-dependent bounds.
 // FULL (nb == TRSM_LEAF) has no row predicates: predicated stores of the
 // unrolled register array make the compiler spill it.
-template <class T, int LOWER, int UNIT, int CM, int FULL>
+template <class T, int LOWER, int UNIT, int CM, int FULL, int NB>
 __global__ __launch_bounds__(256)
 void trsm_leaf_kernel(const T* __restrict__ M, int64_t ld, int nb, T* __restrict__ B,
                       int64_t ldb, int64_t N)
 {
-    constexpr int NB = TRSM_LEAF;
     __shared__ T L[NB][NB + 1];
     const int tid = threadIdx.x;
     for (int e = tid; e < NB * NB; e += 256) {
@@ -83,9 +82,11 @@ void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
 #define SA_TRSM_CASE(K, LO, UN, CMV)                                                              \
     case K:                                                                                       \
         if (nb == TRSM_LEAF)                                                                      \
-            hipLaunchKernelGGL((trsm_leaf_kernel<T, LO, UN, CMV, 1>), grid, blk, 0, s, M, ld, inb, B, ldb, N); \
+            hipLaunchKernelGGL((trsm_leaf_kernel<T, LO, UN, CMV, 1, TRSM_LEAF>), grid, blk, 0, s, M, ld, inb, B, ldb, N); \
+        else if (nb == TRSM_LEAF / 2)                                                             \
+            hipLaunchKernelGGL((trsm_leaf_kernel<T, LO, UN, CMV, 1, TRSM_LEAF / 2>), grid, blk, 0, s, M, ld, inb, B, ldb, N); \
         else                                                                                      \
-            hipLaunchKernelGGL((trsm_leaf_kernel<T, LO, UN, CMV, 0>), grid, blk, 0, s, M, ld, inb, B, ldb, N); \
+            hipLaunchKernelGGL((trsm_leaf_kernel<T, LO, UN, CMV, 0, TRSM_LEAF>), grid, blk, 0, s, M, ld, inb, B, ldb, N); \
         break;
     switch (key) {
         SA_TRSM_CASE(0, 0, 0, 0)

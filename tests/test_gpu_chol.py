@@ -74,11 +74,12 @@ def test_spotrf_matches_oracle(gpu):
 
 
 @pytest.mark.parametrize("uplo", [slate_amd.LOWER, slate_amd.UPPER])
-def test_posv_residual(gpu, uplo):
-    say("dposv n=900 nrhs=3 uplo=%d" % uplo)
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_posv_residual(gpu, uplo, nrhs):
+    say("dposv n=900 nrhs=%d uplo=%d" % (nrhs, uplo))
     n = 900
     S = spd(4, n)
-    B = oracle.rands(5, n, 3)
+    B = oracle.rands(5, n, nrhs)
     A = to_dev(S)
     dB = to_dev(B)
     info = slate_amd.posv(A, dB, uplo=uplo, opts=slate_amd.Options(block_size=128))
