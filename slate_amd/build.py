@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Wno-unused-result", "-Wno-unused-function",
-    # compile-time guard: packed FP32 instructions stay off
+    # packed FP32 instructions (v_pk_*_f32) stay off in every HIP object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 

@@ -33,6 +33,78 @@ struct Error : std::runtime_error {
 
 __host__ __device__ inline int64_t ceildiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Cross-workgroup publication fences.  Release/acquire atomics already order
+// the publishing thread's own accesses; SA_CUMULATIVE_FENCE additionally
+// issues a full agent-scope fence (L2 writeback + invalidate) where data
+// written by other lanes/waves of the group is published by one thread.
+#ifndef SA_FENCE_MODE
+#define SA_FENCE_MODE 1
+#endif
+__device__ inline void sa_publish_fence()
+{
+#if SA_FENCE_MODE
+    __threadfence();
+#endif
+}
+
+// Tagged-word exchange between workgroups.  Each 64-bit word carries 32 bits
+// of payload and a 32-bit tag naming the step that wrote it; words are written
+// and read with relaxed agent-scope atomics, so a reader that sees the tag it
+// waits for also sees that word's payload (single-location atomicity and
+// coherence), with no fences and no separate ready flag.  A value wider than
+// 32 bits spans several words, each tagged.
+__device__ inline void tw_store(uint64_t* p, uint32_t payload, uint32_t tag)
+{
+    __hip_atomic_store(p, (uint64_t(tag) << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t tw_load(const uint64_t* p)
+{
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Spin (bounded, ~4 s) until the word carries `tag`; returns the payload.
+// On timeout sets *error = code and returns 0.
+__device__ inline uint32_t tw_wait(const uint64_t* p, uint32_t tag, int* error, int code)
+{
+    uint64_t w = tw_load(p);
+    if (uint32_t(w >> 32) == tag) return uint32_t(w);
+    uint64_t t0 = wall_clock64();
+    while (true) {
+        __builtin_amdgcn_s_sleep(1);
+        w = tw_load(p);
+        if (uint32_t(w >> 32) == tag) return uint32_t(w);
+        if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return 0;
+        if (wall_clock64() - t0 > 400000000ull) {
+            __hip_atomic_store(error, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return 0;
+        }
+    }
+}
+template <class T> struct TwWords { static constexpr int n = sizeof(T) / 4; };
+template <class T>
+__device__ inline void tw_store_val(uint64_t* p, T v, uint32_t tag)
+{
+    if constexpr (sizeof(T) == 8) {
+        uint64_t u = __builtin_bit_cast(uint64_t, v);
+        tw_store(p, uint32_t(u), tag);
+        tw_store(p + 1, uint32_t(u >> 32), tag);
+    }
+    else {
+        tw_store(p, __builtin_bit_cast(uint32_t, v), tag);
+    }
+}
+template <class T>
+__device__ inline T tw_wait_val(const uint64_t* p, uint32_t tag, int* error, int code)
+{
+    if constexpr (sizeof(T) == 8) {
+        uint64_t lo = tw_wait(p, tag, error, code);
+        uint64_t hi = tw_wait(p + 1, tag, error, code);
+        return __builtin_bit_cast(T, (hi << 32) | lo);
+    }
+    else {
+        return __builtin_bit_cast(T, tw_wait(p, tag, error, code));
+    }
+}
+
 // Wavefront (64-lane) reductions via cross-lane shuffles.
 template <class T>
 __device__ inline T wave_sum(T v)

@@ -40,7 +40,7 @@ struct Context {
     hipStream_t s1 = nullptr;          // panel + lookahead column
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
-    Buf counters, slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags;
+    Buf counters, slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     int64_t counter_next = 0, counter_cap = 0;
     // trailing-update GEMM timing (enabled by slate_amd_profile_enable)
@@ -183,6 +183,56 @@ static void trsm_rec(const T* M, int64_t ld, int cm, int lower, int unit, int64_
     }
 }
 
+// Wide triangular solve B := tri^{-1} B for an nb x nb triangle and an
+// nb x N row-major B with N >> nb: the triangle is inverted once (the same
+// recursive solve applied to an identity, nb x nb) and the product with B is
+// one full-width MFMA GEMM into a workspace that is copied back.  This is the
+// inverted-diagonal-block form of trsm that GPU BLAS libraries use; it trades
+// nb^2 N extra flops at MFMA rate for 2 nb/32 narrow launches.  Runs on s0
+// only (one workspace).
+template <class T>
+static void trsm_wide(Context& c, const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
+                      T* B, int64_t ldb, int64_t N, hipStream_t s)
+{
+    if (N < 4 * nb || nb <= TRSM_LEAF) { trsm_rec<T>(M, ld, cm, lower, unit, nb, B, ldb, N, s); return; }
+    c.tinv.ensure(sizeof(T) * size_t(nb) * size_t(nb));
+    c.ttmp.ensure(sizeof(T) * size_t(nb) * size_t(N));
+    T* inv = c.tinv.as<T>();
+    T* tmp = c.ttmp.as<T>();
+    set_identity<T>(inv, nb, nb, s);
+    trsm_rec<T>(M, ld, cm, lower, unit, nb, inv, nb, nb, s);
+    GemmArgs g{};
+    g.M = nb; g.N = N; g.K = nb;
+    g.A = inv; g.lda = nb; g.B = B; g.ldb = ldb; g.C = tmp; g.ldc = N;
+    g.alpha = 1.0; g.beta = 0.0; g.uplo = 0;
+    gemm<T>(g, A_KC, B_NC, s);
+    SA_HIP(hipMemcpy2DAsync(B, sizeof(T) * ldb, tmp, sizeof(T) * N, sizeof(T) * N, nb,
+                            hipMemcpyDeviceToDevice, s));
+}
+
+// B := inv * B for an nb x nb (full, row-major, ld nb) inverse and an nb x N
+// row-major B, through the workspace tmp (nb x N).
+template <class T>
+static void apply_inv(const T* inv, int64_t nb, T* B, int64_t ldb, int64_t N, T* tmp, hipStream_t s)
+{
+    if (N <= 0) return;
+    GemmArgs g{};
+    g.M = nb; g.N = N; g.K = nb;
+    g.A = inv; g.lda = nb; g.B = B; g.ldb = ldb; g.C = tmp; g.ldc = N;
+    g.alpha = 1.0; g.beta = 0.0; g.uplo = 0;
+    gemm<T>(g, A_KC, B_NC, s);
+    SA_HIP(hipMemcpy2DAsync(B, sizeof(T) * ldb, tmp, sizeof(T) * N, sizeof(T) * N, nb,
+                            hipMemcpyDeviceToDevice, s));
+}
+
+// inv := L^{-1} for the unit-lower nb x nb diagonal block of a row-major matrix.
+template <class T>
+static void unit_lower_inverse(const T* L, int64_t ld, int64_t nb, T* inv, hipStream_t s)
+{
+    set_identity<T>(inv, nb, nb, s);
+    trsm_rec<T>(L, ld, 0, 1, 1, nb, inv, nb, nb, s);
+}
+
 //------------------------------------------------------------------------------
 // Recursive panel LU of rows [c0, n) x columns [c0, c0+w) of a row-major n x n
 // matrix (the panel's top row is the diagonal).  Leaves of width <= W use
@@ -255,12 +305,23 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
 
     // One swap plan per step, built on s1 right after the step's panel and
     // read by both streams (plan slots are never reused within a call).
+    // L11^{-1} of each step, built on s1 after the panel (two alternating
+    // slots: s0 is done with slot k%2 before s1 starts step k+2), applied by
+    // GEMM to the lookahead block (s1) and to the rest (s0), so the result
+    // is independent of where the columns are split.
+    c.sinv.ensure(sizeof(T) * size_t(2 * nb * nb));
+    c.stmp1.ensure(sizeof(T) * size_t(nb) * size_t(nb));
+    c.ttmp.ensure(sizeof(T) * size_t(nb) * size_t(std::max<int64_t>(n, 1)));
+    auto step_inv = [&](int64_t k) { return c.sinv.as<T>() + (k % 2) * nb * nb; };
+    T* tmp0 = c.ttmp.as<T>();
+    T* tmp1 = c.stmp1.as<T>();
     const int64_t plan_stride = 4 * nb + 8;
     c.plans.ensure(sizeof(int64_t) * size_t(plan_stride) * size_t(nsteps + 1));
     auto step_plan = [&](int64_t k) { return c.plans.as<int64_t>() + k * plan_stride; };
     // panel 0
     panel_rec<T>(c, A, ld, n, 0, std::min(nb, n), ipiv, info, o.threshold, leaf_w, plan1, s1);
     laswp_plan(0, std::min(nb, n), n, ipiv, step_plan(0), s1);
+    unit_lower_inverse<T>(A, ld, std::min(nb, n), step_inv(0), s1);
     SA_HIP(hipEventRecord(ev_panel(0), s1));
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t k0 = k * nb, kb = std::min(nb, n - k0);
@@ -276,7 +337,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             // s1: step k on column block k+1, then panel k+1
             SA_HIP(hipStreamWaitEvent(s1, k > 0 ? ev_trail(k) : ev_panel(0), 0));
             laswp<T>(A, ld, n, c1, c1 + nb1, k0, c1, ipiv, pk, s1, true);
-            trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c1, ld, nb1, s1);
+            apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, nb1, tmp1, s1);
             GemmArgs g{};
             g.M = r; g.N = nb1; g.K = kb;
             g.A = A + c1 * ld + k0; g.lda = ld;
@@ -286,11 +347,12 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             gemm<T>(g, A_KC, B_NC, s1);
             panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
             laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
+            unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
             // s0: step k on the columns beyond k+1
             const int64_t c2 = c1 + nb1;
             laswp<T>(A, ld, n, c2, n, k0, c1, ipiv, pk, s0, true);
-            trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c2, ld, n - c2, s0);
+            apply_inv<T>(step_inv(k), kb, A + k0 * ld + c2, ld, n - c2, tmp0, s0);
             g.N = n - c2;
             g.B = A + k0 * ld + c2;
             g.C = A + c1 * ld + c2;
@@ -302,7 +364,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         else {
             // no lookahead: whole trailing matrix on s0, then the next panel
             laswp<T>(A, ld, n, c1, n, k0, c1, ipiv, pk, s0, true);
-            trsm_rec<T>(A + k0 * ld + k0, ld, 0, 1, 1, kb, A + k0 * ld + c1, ld, r, s0);
+            apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, r, tmp0, s0);
             GemmArgs g{};
             g.M = r; g.N = r; g.K = kb;
             g.A = A + c1 * ld + k0; g.lda = ld;
@@ -318,6 +380,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             }
             panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
             laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
+            unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
         }
     }
@@ -365,7 +428,7 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         potrf_diag_rec<T>(c, Akk, ld, kb, k0, info, s);
         int64_t r = n - k0 - kb;
         if (r <= 0) break;
-        trsm_rec<T>(Akk, ld, 1, 1, 0, kb, Akk + kb, ld, r, s);
+        trsm_wide<T>(c, Akk, ld, 1, 1, 0, kb, Akk + kb, ld, r, s);
         GemmArgs g{};
         g.M = r; g.N = r; g.K = kb;
         g.A = Akk + kb; g.lda = ld; g.B = Akk + kb; g.ldb = ld;
@@ -413,21 +476,40 @@ static int* err_flag(Context& c)
 
 // Single right-hand side: x := op(LU)^{-1} x with x contiguous, using the
 // one-launch look-back triangular solves.  perm is the getrf permutation.
+// Workspace for the two inverted-diagonal-block sets of a triangular pair.
+template <class T>
+static T* dinv_pair(Context& c, int64_t n)
+{
+    const int64_t bs = trsv_block(int(sizeof(T)));
+    const int64_t per = ceildiv(n, bs) * bs * bs;
+    c.dinv.ensure(sizeof(T) * size_t(2 * per));
+    return c.dinv.as<T>();
+}
+
+// Single right-hand side: x := op(LU)^{-1} x with x contiguous, using the
+// one-launch look-back triangular solves.  perm is the getrf permutation.
 template <class T>
 static void getrs_vec(Context& c, const T* LU, int64_t ld, int cm, int64_t n, const int64_t* perm,
                       T* x, T* tmp, int trans, hipStream_t s)
 {
     unsigned* ctr = tv_counter(c);
     int* err = err_flag(c);
+    const int64_t bs = trsv_block(int(sizeof(T)));
+    T* d0 = dinv_pair<T>(c, n);
+    T* d1 = d0 + ceildiv(n, bs) * bs * bs;
     if (!trans) {
+        trsv_diag_inverse<T>(LU, ld, cm, 1, 1, n, d0, s);
+        trsv_diag_inverse<T>(LU, ld, cm, 0, 0, n, d1, s);
         permute_convert<T, T>(x, tmp, perm, n, 0, s);
-        trsv_lookback<T>(LU, ld, cm, 1, 1, n, tmp, ctr, err, s);
-        trsv_lookback<T>(LU, ld, cm, 0, 0, n, tmp, ctr, err, s);
+        trsv_lookback<T>(LU, ld, cm, 1, n, d0, tmp, ctr, err, s);
+        trsv_lookback<T>(LU, ld, cm, 0, n, d1, tmp, ctr, err, s);
         copy_convert<T, T>(tmp, n, x, n, n, 1, s);
     }
     else {
-        trsv_lookback<T>(LU, ld, 1 - cm, 1, 0, n, x, ctr, err, s);
-        trsv_lookback<T>(LU, ld, 1 - cm, 0, 1, n, x, ctr, err, s);
+        trsv_diag_inverse<T>(LU, ld, 1 - cm, 1, 0, n, d0, s);
+        trsv_diag_inverse<T>(LU, ld, 1 - cm, 0, 1, n, d1, s);
+        trsv_lookback<T>(LU, ld, 1 - cm, 1, n, d0, x, ctr, err, s);
+        trsv_lookback<T>(LU, ld, 1 - cm, 0, n, d1, x, ctr, err, s);
         permute_convert<T, T>(x, tmp, perm, n, 1, s);
         copy_convert<T, T>(tmp, n, x, n, n, 1, s);
     }
@@ -589,8 +671,13 @@ static void potrs_cm(Context& c, int uplo, T* A, int64_t lda, int64_t n, T* B, i
         // Lower column-major L (A = L L^T): L y = b, L^T x = y.
         // Upper column-major U (A = U^T U): U^T y = b, U x = y.
         const int cmL = (uplo == SLATE_AMD_UPPER) ? 0 : 1;
-        trsv_lookback<T>(A, lda, cmL, 1, 0, n, B, ctr, err, c.s0);
-        trsv_lookback<T>(A, lda, 1 - cmL, 0, 0, n, B, ctr, err, c.s0);
+        const int64_t bs = trsv_block(int(sizeof(T)));
+        T* d0 = dinv_pair<T>(c, n);
+        T* d1 = d0 + ceildiv(n, bs) * bs * bs;
+        trsv_diag_inverse<T>(A, lda, cmL, 1, 0, n, d0, c.s0);
+        trsv_diag_inverse<T>(A, lda, 1 - cmL, 0, 0, n, d1, c.s0);
+        trsv_lookback<T>(A, lda, cmL, 1, n, d0, B, ctr, err, c.s0);
+        trsv_lookback<T>(A, lda, 1 - cmL, 0, n, d1, B, ctr, err, c.s0);
         SA_HIP(hipStreamSynchronize(c.s0));
         check_kernel_error(c);
         return;
@@ -617,6 +704,8 @@ struct MixedState {
     int64_t* perm;        // device, getrf permutation of the float factor
     float* xlo;           // row-major n x nrhs
     float* xtmp;          // scratch vector
+    float* dinvL;         // inverted diagonal blocks of the first solve
+    float* dinvU;         // ... and of the second
     double* R;            // column-major n x nrhs
     double* work;
     int64_t work_elems;
@@ -664,13 +753,13 @@ static void lo_solve_vec(Context& c, const MixedState& st, int sym, int64_t n, c
     int* err = err_flag(c);
     if (sym) {
         copy_convert<double, float>(src, n, st.xlo, n, n, 1, c.s0);
-        trsv_lookback<float>(st.Alo, n, 1, 1, 0, n, st.xlo, ctr, err, c.s0);   // U^T y = b
-        trsv_lookback<float>(st.Alo, n, 0, 0, 0, n, st.xlo, ctr, err, c.s0);   // U x = y
+        trsv_lookback<float>(st.Alo, n, 1, 1, n, st.dinvL, st.xlo, ctr, err, c.s0);   // U^T y = b
+        trsv_lookback<float>(st.Alo, n, 0, 0, n, st.dinvU, st.xlo, ctr, err, c.s0);   // U x = y
     }
     else {
         permute_convert<double, float>(src, st.xlo, st.perm, n, 0, c.s0);
-        trsv_lookback<float>(st.Alo, n, 0, 1, 1, n, st.xlo, ctr, err, c.s0);   // L y = P b
-        trsv_lookback<float>(st.Alo, n, 0, 0, 0, n, st.xlo, ctr, err, c.s0);   // U x = y
+        trsv_lookback<float>(st.Alo, n, 0, 1, n, st.dinvL, st.xlo, ctr, err, c.s0);   // L y = P b
+        trsv_lookback<float>(st.Alo, n, 0, 0, n, st.dinvU, st.xlo, ctr, err, c.s0);   // U x = y
     }
     if (accumulate) add_convert<float>(st.xlo, dst, n, c.s0);
     else copy_convert<float, double>(st.xlo, n, dst, n, n, 1, c.s0);
@@ -692,8 +781,11 @@ static void from_lo(Context& c, const float* xlo, double* V, int64_t ldv, int64_
 static MixedState mixed_setup(Context& c, int64_t n, int64_t nrhs)
 {
     MixedState st{};
-    c.lo.ensure(sizeof(float) * size_t(n) * size_t(n) + 256);
+    const int64_t dper = ceildiv(n, trsv_block(4)) * trsv_block(4) * trsv_block(4);
+    c.lo.ensure(sizeof(float) * (size_t(n) * size_t(n) + size_t(2 * dper)) + 1024);
     st.Alo = c.lo.as<float>();
+    st.dinvL = st.Alo + size_t(n) * size_t(n);
+    st.dinvU = st.dinvL + dper;
     int64_t we = std::max<int64_t>(residual_work_elems(n, 4), ceildiv(n, 1024) * n + n + 64);
     size_t vec_bytes = 2 * sizeof(int64_t) * n + 2 * sizeof(float) * n * nrhs + sizeof(double) * n * nrhs
                        + sizeof(double) * we + sizeof(double) * (nrhs + 8) + 1024;
@@ -729,6 +821,17 @@ static int64_t factor_lo(Context& c, MixedState& st, const double* A, int64_t ld
         getrf_rm<float>(c, st.Alo, n, n, o, st.ipiv, c.dinfo.as<int64_t>());
     }
     SA_HIP(hipStreamSynchronize(c.s0));
+    if (read_info(c) == 0) {
+        // inverted diagonal blocks of both triangles, reused by every solve
+        if (sym) {
+            trsv_diag_inverse<float>(st.Alo, n, 1, 1, 0, n, st.dinvL, c.s0);
+            trsv_diag_inverse<float>(st.Alo, n, 0, 0, 0, n, st.dinvU, c.s0);
+        }
+        else {
+            trsv_diag_inverse<float>(st.Alo, n, 0, 1, 1, n, st.dinvL, c.s0);
+            trsv_diag_inverse<float>(st.Alo, n, 0, 0, 0, n, st.dinvU, c.s0);
+        }
+    }
     if (!sym) {
         check_kernel_error(c);
         std::vector<int64_t> ip(n), perm;
@@ -1042,7 +1145,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.counters, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
-                       &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags})
+                       &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);

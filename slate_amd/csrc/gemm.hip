@@ -8,10 +8,11 @@
 // one launch covers the whole trailing matrix instead of a batch of nb x nb
 // tiles, so there are no host-built pointer arrays and no per-tile launches.
 //
-// Shape: 256 threads = 4 wave64s in a 2x2 arrangement; each wave owns a 64x64
-// block of C held as 4x4 MFMA 16x16 accumulators.  The K loop stages BK-deep
-// slices of A and B through double-buffered LDS; the next slice is fetched into
-// registers while the current one feeds the matrix cores.
+// Shape: WM x WN wave64s (8 waves = 2 per SIMD for the main update kernels);
+// each wave owns a (BM/WM) x (BN/WN) block of C held as 16x16 MFMA
+// accumulators.  The K loop stages BK-deep slices of A and B through
+// double-buffered LDS; the next slice is fetched into registers while the
+// current one feeds the matrix cores and is written to LDS half-way through.
 #include "common.hh"
 #include "kernels.hh"
 
@@ -40,12 +41,11 @@ template <> struct Mf<float> {
 
 // A layout: A_KC = K-contiguous (A[m*lda + k]), A_MC = M-contiguous (A[m + k*lda]).
 // B layout: B_NC = N-contiguous (B[k*ldb + n]), B_KC = K-contiguous (B[k + n*ldb]).
-template <class T, int ALAY, int BLAY, int BM, int BN, int BK>
-__global__ __launch_bounds__(256)
+template <class T, int ALAY, int BLAY, int BM, int BN, int BK, int WM = 2, int WN = 2, int MINB = 1>
+__global__ __launch_bounds__(WM * WN * 64, MINB)
 void gemm_kernel(GemmArgs p)
 {
-    constexpr int NT = 256;
-    constexpr int WM = 2, WN = 2;
+    constexpr int NT = WM * WN * 64;
     constexpr int TM = BM / WM, TN = BN / WN;       // per-wave C block
     constexpr int FM = TM / 16, FN = TN / 16;       // MFMA tiles per wave
     constexpr int KM = Mf<T>::KM;
@@ -54,12 +54,17 @@ void gemm_kernel(GemmArgs p)
     constexpr int A_OD = (ALAY == A_KC) ? BM : BK;
     constexpr int B_CD = (BLAY == B_NC) ? BN : BK;
     constexpr int B_OD = (BLAY == B_NC) ? BK : BN;
-    constexpr int A_LD = A_CD + (sizeof(T) == 8 ? 1 : 1);
-    constexpr int B_LD = B_CD + (sizeof(T) == 8 ? 2 : 4);
+    // Padding so fragment reads are bank-conflict free: 64-bit reads are served
+    // 16 lanes at a time (one odd 8-byte pad suffices); 32-bit reads 32 lanes at
+    // a time, where lanes 16..31 read the next k, so that k-row must start 2
+    // banks (M/N-fastest index along lanes: 16 banks) further.
+    constexpr int A_LD = (sizeof(T) == 8) ? A_CD + 1 : ((ALAY == A_KC) ? A_CD + 2 : A_CD + 16);
+    constexpr int B_LD = (sizeof(T) == 8) ? B_CD + 1 : ((BLAY == B_NC) ? B_CD + 16 : B_CD + 2);
     constexpr int A_PER = A_CD * A_OD / NT;         // elements per thread
     constexpr int B_PER = B_CD * B_OD / NT;
     static_assert(NT % A_CD == 0 && NT % B_CD == 0, "tile shape");
 
+    static_assert((A_CD * A_OD) % NT == 0 && (B_CD * B_OD) % NT == 0, "tile/threads");
     __shared__ T As[2][A_OD * A_LD];
     __shared__ T Bs[2][B_OD * B_LD];
 
@@ -97,9 +102,12 @@ void gemm_kernel(GemmArgs p)
             int64_t gm_, gk_;
             if (ALAY == A_KC) { gk_ = k0 + cd; gm_ = m0 + od; }
             else              { gm_ = m0 + cd; gk_ = k0 + od; }
+            // branch-free edges: clamp the address, zero the value
             bool ok = gm_ < M && gk_ < K;
-            int64_t off = (ALAY == A_KC) ? gm_ * lda + gk_ : gm_ + gk_ * lda;
-            ra[i] = ok ? A[off] : T(0);
+            int64_t cm_ = gm_ < M ? gm_ : M - 1, ck_ = gk_ < K ? gk_ : K - 1;
+            int64_t off = (ALAY == A_KC) ? cm_ * lda + ck_ : cm_ + ck_ * lda;
+            T v = A[off];
+            ra[i] = ok ? v : T(0);
         }
         #pragma unroll
         for (int i = 0; i < B_PER; ++i) {
@@ -109,8 +117,10 @@ void gemm_kernel(GemmArgs p)
             if (BLAY == B_NC) { gn_ = n0 + cd; gk_ = k0 + od; }
             else              { gk_ = k0 + cd; gn_ = n0 + od; }
             bool ok = gn_ < N && gk_ < K;
-            int64_t off = (BLAY == B_NC) ? gk_ * ldb + gn_ : gk_ + gn_ * ldb;
-            rb[i] = ok ? B[off] : T(0);
+            int64_t cn_ = gn_ < N ? gn_ : N - 1, ck_ = gk_ < K ? gk_ : K - 1;
+            int64_t off = (BLAY == B_NC) ? ck_ * ldb + cn_ : ck_ + cn_ * ldb;
+            T v = B[off];
+            rb[i] = ok ? v : T(0);
         }
     };
     auto store_tiles = [&](int buf) {
@@ -148,9 +158,14 @@ void gemm_kernel(GemmArgs p)
     const int fr = lane & 15, fk = lane >> 4;
     for (int64_t kt = 0; kt < ktiles; ++kt) {
         const int cur = kt & 1;
-        if (kt + 1 < ktiles) load_tiles((kt + 1) * BK);
+        const bool more = kt + 1 < ktiles;
+        if (more) load_tiles((kt + 1) * BK);
         #pragma unroll
         for (int kk = 0; kk < BK; kk += KM) {
+            // Stage the next slice half-way through this one, so the LDS writes
+            // overlap the remaining MFMAs instead of draining the matrix pipe
+            // (the other buffer was last read before the previous barrier).
+            if (kk == BK / 2 && more) store_tiles(cur ^ 1);
             T af[FM], bf[FN];
             #pragma unroll
             for (int i = 0; i < FM; ++i) af[i] = a_at(cur, wm * TM + i * 16 + fr, kk + fk);
@@ -161,10 +176,7 @@ void gemm_kernel(GemmArgs p)
                 #pragma unroll
                 for (int j = 0; j < FN; ++j) acc[i][j] = Mf<T>::op(af[i], bf[j], acc[i][j]);
         }
-        if (kt + 1 < ktiles) {
-            store_tiles(cur ^ 1);
-            __syncthreads();
-        }
+        if (more) __syncthreads();
     }
 
     const T alpha = T(p.alpha), beta = T(p.beta);
@@ -189,10 +201,12 @@ template <class T, int ALAY, int BLAY>
 static void launch(const GemmArgs& a, hipStream_t s)
 {
     if (a.M <= 0 || a.N <= 0) return;
+    // 128 x 128 tiles, 4 waves, two workgroups per CU (2 waves per SIMD) —
+    // the fastest of the tilings measured for both FP64 and FP32.
     constexpr int BM = 128, BN = 128, BK = (sizeof(T) == 8) ? 16 : 32;
     int64_t nt = ceildiv(a.M, BM) * ceildiv(a.N, BN);
     SA_REQUIRE(nt < (int64_t(1) << 31), -2, "gemm: grid too large");
-    hipLaunchKernelGGL((gemm_kernel<T, ALAY, BLAY, BM, BN, BK>), dim3(unsigned(nt)), dim3(256),
+    hipLaunchKernelGGL((gemm_kernel<T, ALAY, BLAY, BM, BN, BK, 2, 2, 2>), dim3(unsigned(nt)), dim3(256),
                        0, s, a);
     SA_CHECK_LAUNCH();
 }

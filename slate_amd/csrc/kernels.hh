@@ -61,10 +61,17 @@ void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
                T* B, int64_t ldb, int64_t N, hipStream_t s);
 
 // One-launch triangular solve of tri * x = b (one right-hand side, x overwrites
-// b, contiguous) with a completion-counter chain between 128-row blocks.
-// ndone: one zeroed-on-launch counter; error: set to 2 if a wait times out.
+// b, contiguous) with a completion-counter chain between blocks of
+// trsv_block() rows.  Dinv holds the inverted diagonal blocks made by
+// trsv_diag_inverse (ceil(n/bs) * bs^2 elements; the unit/non-unit choice is
+// folded into it).  ndone: one counter, zeroed on launch; error: set to 2 if
+// a wait times out.
+int64_t trsv_block(int elem_bytes);
 template <class T>
-void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* X,
+void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* Dinv,
+                       hipStream_t s);
+template <class T>
+void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T* Dinv, T* X,
                    unsigned* ndone, int* error, hipStream_t s);
 
 // Cholesky leaf on an nb x nb row-major block (nb <= 64), upper factor U with
@@ -72,6 +79,9 @@ void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int unit, int64_t 
 // first non-positive pivot (if *info == 0).
 template <class T>
 void potrf_leaf(T* A, int64_t ld, int64_t nb, int64_t col0, int64_t* info, hipStream_t s);
+
+// A := I (n x n, row-major, ld).
+template <class T> void set_identity(T* A, int64_t n, int64_t ld, hipStream_t s);
 
 // In-place transpose of an n x n matrix (column-major <-> row-major).
 template <class T> void transpose_inplace(T* A, int64_t n, int64_t ld, hipStream_t s);

@@ -51,7 +51,7 @@ __device__ inline bool better(T a, int64_t ia, T b, int64_t ib)
     return a > b || (a == b && ia < ib);
 }
 
-template <class T, int W, int RPT>
+template <class T, int W, int RPT, int VEC>
 __global__ __launch_bounds__(LEAF_THREADS)
 void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0,
                        int w, int64_t* __restrict__ ipiv, int64_t* __restrict__ info, T threshold,
@@ -73,13 +73,28 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
     __shared__ T s_piv[W];
     __shared__ int64_t s_p;
 
+    // Each thread's rows are read with 16-byte vector loads when the leaf is
+    // aligned (VEC), so a row segment costs W*sizeof(T)/16 requests.
+    constexpr int VE = 16 / int(sizeof(T));
     T v[RPT][W];
     #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
-        #pragma unroll
-        for (int c = 0; c < W; ++c)
-            v[q][c] = (i < m && c < w) ? A[(r0 + i) * ld + c0 + c] : T(0);
+        const T* rowp = A + (r0 + (i < m ? i : 0)) * ld + c0;
+        if (VEC && w == W) {
+            #pragma unroll
+            for (int c = 0; c < W; c += VE) {
+                typedef T vec_t __attribute__((ext_vector_type(VE)));
+                vec_t x = *reinterpret_cast<const vec_t*>(rowp + c);
+                #pragma unroll
+                for (int e = 0; e < VE; ++e) v[q][c + e] = (i < m) ? x[e] : T(0);
+            }
+        }
+        else {
+            #pragma unroll
+            for (int c = 0; c < W; ++c)
+                v[q][c] = (i < m && c < w) ? rowp[c] : T(0);
+        }
     }
     const T safe_min = std::numeric_limits<T>::min();
 
@@ -125,7 +140,7 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
                     #pragma unroll
                     for (int c = 0; c < W; ++c) sl.row_[size_t(g) * W + c] = v[q][c];
                 }
-            __threadfence();
+            sa_publish_fence();
         }
         if (int64_t(j) >= gbase && int64_t(j) < gbase + R && (j - gbase) % LEAF_THREADS == tid) {
             int qj = int((j - gbase) / LEAF_THREADS);
@@ -135,12 +150,11 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
                     #pragma unroll
                     for (int c = 0; c < W; ++c) sl.rowj_[c] = v[q][c];
                 }
-            __threadfence();
+            sa_publish_fence();
         }
         if (tid == 0) {
             sl.abs_[g] = babs;
             sl.idx_[g] = bidx;
-            __threadfence();
         }
         __syncthreads();
 
@@ -157,17 +171,32 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
                     break;
                 }
             }
-            __threadfence();
+            sa_publish_fence();
         }
         __syncthreads();
 
-        // (4) gather all candidates and row j, pick the pivot
-        for (int e = tid; e < G; e += LEAF_THREADS) {
-            s_cabs[e] = sl.abs_[e];
-            s_cidx[e] = sl.idx_[e];
+        // (4) gather all candidates and row j, pick the pivot.  Fixed trip
+        // counts and clamped addresses keep every load independent (one
+        // memory round trip for the whole gather, not one per iteration).
+        {
+            constexpr int NCR = (LEAF_GMAX * W + LEAF_THREADS - 1) / LEAF_THREADS;
+            T rv[NCR];
+            #pragma unroll
+            for (int u = 0; u < NCR; ++u) {
+                int e = tid + u * LEAF_THREADS;
+                rv[u] = sl.row_[e < G * W ? e : G * W - 1];
+            }
+            T av = sl.abs_[tid < G ? tid : G - 1];
+            int64_t xv = sl.idx_[tid < G ? tid : G - 1];
+            T jv = sl.rowj_[tid < W ? tid : W - 1];
+            #pragma unroll
+            for (int u = 0; u < NCR; ++u) {
+                int e = tid + u * LEAF_THREADS;
+                if (e < G * W) s_crow[e] = rv[u];
+            }
+            if (tid < G) { s_cabs[tid] = av; s_cidx[tid] = xv; }
+            if (tid < W) s_rowj[tid] = jv;
         }
-        for (int e = tid; e < G * W; e += LEAF_THREADS) s_crow[e] = sl.row_[e];
-        if (tid < W) s_rowj[tid] = sl.rowj_[tid];
         __syncthreads();
         if (wave == 0) {
             T ba = T(-1);
@@ -235,9 +264,22 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
     for (int q = 0; q < RPT; ++q) {
         int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
         if (i < m) {
-            #pragma unroll
-            for (int c = 0; c < W; ++c)
-                if (c < w) A[(r0 + i) * ld + c0 + c] = v[q][c];
+            T* rowp = A + (r0 + i) * ld + c0;
+            if (VEC && w == W) {
+                #pragma unroll
+                for (int c = 0; c < W; c += VE) {
+                    typedef T vec_t __attribute__((ext_vector_type(VE)));
+                    vec_t x;
+                    #pragma unroll
+                    for (int e = 0; e < VE; ++e) x[e] = v[q][c + e];
+                    *reinterpret_cast<vec_t*>(rowp + c) = x;
+                }
+            }
+            else {
+                #pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (c < w) rowp[c] = v[q][c];
+            }
         }
     }
 }
@@ -260,14 +302,17 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
     SA_REQUIRE(int64_t(2 * per) <= pw.slots_bytes, -2, "panel_leaf: slot area too small");
     unsigned* ctr = pw.counters + counter_index;
     T thr = T(threshold);
-    switch (rpt) {
-        case 1: hipLaunchKernelGGL((panel_leaf_kernel<T, W, 1>), dim3(G), dim3(LEAF_THREADS), 0, s,
-                                   A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error); break;
-        case 2: hipLaunchKernelGGL((panel_leaf_kernel<T, W, 2>), dim3(G), dim3(LEAF_THREADS), 0, s,
-                                   A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error); break;
-        default: hipLaunchKernelGGL((panel_leaf_kernel<T, W, 4>), dim3(G), dim3(LEAF_THREADS), 0, s,
-                                    A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error); break;
+    constexpr int VE = 16 / int(sizeof(T));
+    const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (ld % VE == 0) && (c0 % VE == 0);
+#define SA_LEAF(R, V) hipLaunchKernelGGL((panel_leaf_kernel<T, W, R, V>), dim3(G), dim3(LEAF_THREADS), 0, s, \
+                                          A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error)
+    if (vec) {
+        switch (rpt) { case 1: SA_LEAF(1, 1); break; case 2: SA_LEAF(2, 1); break; default: SA_LEAF(4, 1); break; }
     }
+    else {
+        switch (rpt) { case 1: SA_LEAF(1, 0); break; case 2: SA_LEAF(2, 0); break; default: SA_LEAF(4, 0); break; }
+    }
+#undef SA_LEAF
     SA_CHECK_LAUNCH();
 }
 
@@ -358,15 +403,92 @@ void laswp_apply_kernel(T* __restrict__ A, int64_t ld, int64_t c0, int64_t c1, i
     const int tid = threadIdx.x;
     const int c = tid % ch, e0 = tid / ch, es = 256 / ch;
     const bool ok = cb + c < c1;
-    for (int e = e0; e < count; e += es) {
-        int64_t src = plan[2 + 2 * e];
-        buf[e * ch + c] = ok ? A[src * ld + cb + c] : T(0);
+    const int64_t cc = ok ? cb + c : c1 - 1;
+    // Batches of 8 independent loads (clamped), so row fetches overlap.
+    for (int e = e0; e < count; e += 8 * es) {
+        T v[8];
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int ee = e + u * es;
+            int64_t src = plan[2 + 2 * (ee < count ? ee : count - 1)];
+            v[u] = A[src * ld + cc];
+        }
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int ee = e + u * es;
+            if (ee < count) buf[ee * ch + c] = ok ? v[u] : T(0);
+        }
     }
     __syncthreads();
     for (int e = e0; e < count; e += es) {
         int64_t dst = plan[1 + 2 * e];
         if (ok) A[dst * ld + cb + c] = buf[e * ch + c];
     }
+}
+
+// Small interchange sequences (<= 64 pivots, the ones inside the recursive
+// panel): every workgroup derives the (dst <- src) list itself in LDS and
+// applies it to its column chunk, so no separate plan launch is needed.
+// Entry e < np is row k1+e; entry np+i is pivot row ipiv[k1+i] when it lies
+// at or beyond k2 and is its first occurrence, otherwise unused.
+template <class T>
+__global__ __launch_bounds__(256)
+void laswp_small_kernel(T* __restrict__ A, int64_t ld, int64_t nrows, int64_t c0, int64_t c1, int ch,
+                        int64_t k1, int np, const int64_t* __restrict__ ipiv)
+{
+    __shared__ int64_t piv[64];
+    __shared__ int64_t dst[128], src[128];
+    extern __shared__ unsigned char smem_raw[];
+    T* buf = reinterpret_cast<T*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int64_t k2 = k1 + np;
+    if (tid < np) {
+        int64_t p = ipiv[k1 + tid];
+        piv[tid] = (p >= k1 + tid && p < nrows) ? p : k1 + tid;
+    }
+    __syncthreads();
+    if (tid < 2 * np) {
+        int64_t r = -1;
+        if (tid < np) r = k1 + tid;
+        else {
+            int i = tid - np;
+            int64_t p = piv[i];
+            bool first = p >= k2;
+            for (int i2 = 0; i2 < i; ++i2) first = first && (piv[i2] != p);
+            if (first) r = p;
+        }
+        int64_t x = r;
+        if (r >= 0)
+            for (int i = np - 1; i >= 0; --i) {
+                int64_t a = k1 + i, b = piv[i];
+                if (x == a) x = b;
+                else if (x == b) x = a;
+            }
+        dst[tid] = r;
+        src[tid] = x;
+    }
+    __syncthreads();
+    const int64_t cb = c0 + int64_t(blockIdx.x) * ch;
+    const int c = tid % ch, e0 = tid / ch, es = 256 / ch;
+    const bool ok = cb + c < c1;
+    const int64_t cc = ok ? cb + c : c1 - 1;
+    for (int e = e0; e < 2 * np; e += 8 * es) {
+        T v[8];
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int ee = e + u * es;
+            int64_t sr = (ee < 2 * np && dst[ee] >= 0) ? src[ee] : k1;
+            v[u] = A[sr * ld + cc];
+        }
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int ee = e + u * es;
+            if (ee < 2 * np && dst[ee] >= 0) buf[ee * ch + c] = ok ? v[u] : T(0);
+        }
+    }
+    __syncthreads();
+    for (int e = e0; e < 2 * np; e += es)
+        if (dst[e] >= 0 && ok) A[dst[e] * ld + cb + c] = buf[e * ch + c];
 }
 
 void laswp_plan(int64_t k1, int64_t k2, int64_t nrows, const int64_t* ipiv, int64_t* plan,
@@ -400,6 +522,19 @@ void laswp(T* A, int64_t ld, int64_t nrows, int64_t c0, int64_t c1, int64_t k1, 
            const int64_t* ipiv, int64_t* plan, hipStream_t s, bool plan_ready)
 {
     if (k2 <= k1) return;
+    if (!plan_ready && k2 - k1 <= 64) {
+        if (c1 <= c0) return;
+        const int np = int(k2 - k1);
+        int ch = 256;
+        while (ch > 4 && int64_t(ch) * 2 * np * int64_t(sizeof(T)) > 65536) ch /= 2;
+        while (ch > 4 && ch / 2 >= (c1 - c0)) ch /= 2;
+        size_t lds = size_t(ch) * size_t(2 * np) * sizeof(T);
+        int64_t nblk = ceildiv(c1 - c0, ch);
+        hipLaunchKernelGGL(laswp_small_kernel<T>, dim3(unsigned(nblk)), dim3(256), lds, s,
+                           A, ld, nrows, c0, c1, ch, k1, np, ipiv);
+        SA_CHECK_LAUNCH();
+        return;
+    }
     if (!plan_ready) laswp_plan(k1, k2, nrows, ipiv, plan, s);
     if (c1 <= c0) return;
     const int64_t maxcount = 2 * (k2 - k1);

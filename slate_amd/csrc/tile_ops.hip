@@ -229,6 +229,27 @@ template void transpose_inplace<double>(double*, int64_t, int64_t, hipStream_t);
 template void transpose_inplace<float>(float*, int64_t, int64_t, hipStream_t);
 
 //------------------------------------------------------------------------------
+template <class T>
+__global__ void set_identity_kernel(T* A, int64_t n, int64_t ld)
+{
+    const int64_t i = blockIdx.y;
+    for (int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x; j < n; j += int64_t(gridDim.x) * 256)
+        A[i * ld + j] = (i == j) ? T(1) : T(0);
+}
+
+template <class T>
+void set_identity(T* A, int64_t n, int64_t ld, hipStream_t s)
+{
+    if (n <= 0) return;
+    SA_REQUIRE(n < 65536, -2, "set_identity: too large");
+    hipLaunchKernelGGL(set_identity_kernel<T>, dim3(unsigned(ceildiv(n, 256)), unsigned(n)), dim3(256), 0, s,
+                       A, n, ld);
+    SA_CHECK_LAUNCH();
+}
+template void set_identity<double>(double*, int64_t, int64_t, hipStream_t);
+template void set_identity<float>(float*, int64_t, int64_t, hipStream_t);
+
+//------------------------------------------------------------------------------
 template <class D, class S> __device__ inline D cvt(S x) { return D(x); }
 
 // dst[j*ldd + i] = src[i + j*lds]: column-major src -> row-major dst.
