@@ -22,6 +22,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64-accurate gesv TFLOP/s, mixed+IR vs plain FP64 LU, n=65536, 1 GPU"
 PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}
+# factor precision -> (C-ABI value, bf16 products per update (0 = FP32 MFMA))
+PRECS = {"fp32": (0, 0), "bf16": (1, 1), "bf16x2": (2, 3), "bf16x3": (4, 6)}
 
 
 def log(msg):
@@ -62,8 +64,14 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--nb", type=int, default=0)
     ap.add_argument("--cpu-n", type=int, default=12288)
+    ap.add_argument("--factor-precision", default="fp32", choices=sorted(PRECS),
+                    help="factor precision of the mixed solve (bf16x3 = float-accurate split-BF16)")
+    ap.add_argument("--gmres", action="store_true", help="GMRES-IR instead of classic IR")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one plain elementwise copy (y = 2 x, float32) of this many MiB "
+                         "each way, the calibration kernel for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU only (single-GPU factor-and-solve); "
@@ -73,8 +81,19 @@ def main():
     import torch
     import slate_amd as S
 
+    if args.copy_probe_mib > 0:
+        cnt = args.copy_probe_mib * (1 << 20) // 4
+        xp = torch.ones(cnt, dtype=torch.float32, device="cuda")
+        yp = torch.empty_like(xp)
+        torch.mul(xp, 2.0, out=yp)
+        torch.cuda.synchronize()
+        log("bench: copy probe %d MiB read + %d MiB written" % (args.copy_probe_mib, args.copy_probe_mib))
+        del xp, yp
+
     n = args.n
-    opts = S.Options(block_size=args.nb)
+    prec_code, products = PRECS[args.factor_precision]
+    opts = S.Options(block_size=args.nb, factor_precision=prec_code)
+    solve = S.gesv_mixed_gmres if args.gmres else S.gesv_mixed
     log("bench: generating A (%d x %d, matgen rands seed 0) and b (seed 1) on the GPU" % (n, n))
     A = S.empty_colmajor(n, n, torch.float64)
     S.generate(A, S.DIST_RANDS, 0)
@@ -85,7 +104,7 @@ def main():
 
     iters = []
     for w in range(args.warmup):
-        info, it, _ = S.gesv_mixed(A, B, X, opts)
+        info, it, _ = solve(A, B, X, opts)
         log("bench: warmup %d info %d iter %d  %.1f ms" % (w, info, it, S.last_time_ms()))
         assert info == 0 and it >= 0, "refinement did not converge (A would be overwritten)"
 
@@ -94,7 +113,7 @@ def main():
     t0 = time.perf_counter()
     dev_ms = 0.0
     for k in range(args.steps):
-        info, it, _ = S.gesv_mixed(A, B, X, opts)
+        info, it, _ = solve(A, B, X, opts)
         dev_ms += S.last_time_ms()
         iters.append(it)
         assert info == 0 and it >= 0, "refinement did not converge"
@@ -125,11 +144,19 @@ def main():
         del A64, B64
         torch.cuda.empty_cache()
 
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    # Dominant kernel: the trailing update.  Its MFMA work is the algorithmic
+    # 2mnk per launch times the bf16 products per update of the split form.
+    mfma_flops = gemm_flops * (products if products else 1)
+    achieved = mfma_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    if products:
+        peak, kname = PEAK["bf16"], ("gemm_bf16s_kernel<%d,1> trailing update (v_mfma_f32_16x16x32_bf16, "
+                                     "%d products per update)" % ({1: 1, 3: 2, 6: 3}[products], products))
+    else:
+        peak, kname = PEAK["fp32"], "gemm_kernel<float,0,0,128,128,32,2,2,2,1> trailing update (v_mfma_f32_16x16x4f32)"
     roofline = {
-        "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK["fp32"], 4), "traffic": None,
-        "kernel": "gemm_kernel<float> trailing update (v_mfma_f32_16x16x4f32)",
+        "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 4), "traffic": None,
+        "kernel": kname,
         "launches": int(launches), "avg_launch_ms": round(gemm_ms / max(launches, 1), 4),
     }
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
@@ -137,12 +164,16 @@ def main():
         "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 factor + f64 refinement", "data": "synthetic",
+        "dtype": ("f32 factor" if not products else "f32 factor, %s updates" % args.factor_precision)
+                 + " + f64 refinement", "data": "synthetic",
         "config": {
-            "workload": "gesv_mixed (float LU + FP64 classic IR) n=%d nrhs=1, A = matgen rands seed 0, "
-                        "b = rands seed 1, already in HBM" % n,
-            "n": n, "nrhs": 1, "nb": args.nb or "auto", "factor_precision": "fp32",
-            "refinement": "classic IR", "iters": iters, "scaled_residual": round(res_mixed, 4),
+            "workload": "%s (float LU, %s updates, + FP64 %s) n=%d nrhs=1, A = matgen rands seed 0, "
+                        "b = rands seed 1, already in HBM"
+                        % ("gesv_mixed_gmres" if args.gmres else "gesv_mixed", args.factor_precision,
+                           "GMRES-IR" if args.gmres else "classic IR", n),
+            "n": n, "nrhs": 1, "nb": args.nb or "auto", "factor_precision": args.factor_precision,
+            "refinement": "GMRES-IR" if args.gmres else "classic IR", "iters": iters,
+            "scaled_residual": round(res_mixed, 4),
             "plain_fp64_gesv": fp64,
             "speedup_vs_plain_fp64": round(fp64["ms"] / ms_per_step, 3) if fp64 else None,
         },

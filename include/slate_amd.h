@@ -25,11 +25,16 @@ extern "C" {
 
 typedef struct slate_amd_context_t* slate_amd_context;
 
-/* Factorization precision of the mixed-precision drivers. */
+/* Factorization precision of the mixed-precision drivers.  The BF16 forms keep
+ * a float factor whose trailing updates run on bf16 MFMA with fp32
+ * accumulation, each float operand split into 1, 2 or 3 bf16 terms (about 8,
+ * 16 or 24 significant bits per product; BF16X3 is float-accurate). */
 enum {
     SLATE_AMD_PREC_FP32 = 0,   /* float factor (SLATE's scalar_lo = float)       */
     SLATE_AMD_PREC_BF16 = 1,   /* float storage, bf16 MFMA updates, fp32 accum   */
-    SLATE_AMD_PREC_FP64 = 3    /* no refinement: plain FP64 factor and solve     */
+    SLATE_AMD_PREC_BF16X2 = 2, /* two-term bf16 split: 3 products per update     */
+    SLATE_AMD_PREC_FP64 = 3,   /* no refinement: plain FP64 factor and solve     */
+    SLATE_AMD_PREC_BF16X3 = 4  /* three-term bf16 split: 6 products per update   */
 };
 
 enum { SLATE_AMD_LOWER = 0, SLATE_AMD_UPPER = 1 };   /* slate::Uplo                */
@@ -138,6 +143,15 @@ int slate_amd_dgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, doub
 int slate_amd_sgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, float alpha,
                     const float* dA, int64_t lda, const float* dB, int64_t ldb, float beta,
                     float* dC, int64_t ldc);
+
+/* C (m x n, column-major) += alpha * A * B for float A (m x k), B (k x n),
+ * computed by the split-BF16 MFMA kernel behind the BF16 factor precisions:
+ * each operand split into `terms` = 1, 2 or 3 bf16 terms, fp32 accumulation.
+ * No counterpart in the reference: SLATE's low-precision update is the
+ * vendor sgemm (src/internal/internal_gemm.cc:498-504); exposed for the
+ * accuracy tests of that kernel. */
+int slate_amd_sgemm_bf16(slate_amd_context ctx, int terms, int64_t m, int64_t n, int64_t k, float alpha,
+                         const float* dA, int64_t lda, const float* dB, int64_t ldb, float* dC, int64_t ldc);
 
 /* slate::norm(Norm::Inf, A) — src/norm.cc:24-240; sym != 0: symmetric, lower stored. */
 int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double* dA, int64_t lda,

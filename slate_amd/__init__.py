@@ -21,7 +21,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libslate_amd.so")
 
-PREC_FP32, PREC_BF16, PREC_FP64 = 0, 1, 3
+PREC_FP32, PREC_BF16, PREC_BF16X2, PREC_FP64, PREC_BF16X3 = 0, 1, 2, 3, 4
 LOWER, UPPER = 0, 1
 NOTRANS, TRANS = 0, 1
 DIST_UNIFORM, DIST_RANDS, DIST_RANDN = 1, 2, 3
@@ -34,7 +34,8 @@ C_SYMBOLS = [
     "slate_amd_dgesv", "slate_amd_dpotrf", "slate_amd_spotrf", "slate_amd_dpotrs",
     "slate_amd_dposv", "slate_amd_dsgesv", "slate_amd_dsgesv_gmres", "slate_amd_dsposv",
     "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate",
-    "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_profile_enable", "slate_amd_profile_read",
+    "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_profile_enable",
+    "slate_amd_profile_read",
 ]
 
 
@@ -105,6 +106,8 @@ def lib():
             "slate_amd_dgemm": (i32, [vp, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
             "slate_amd_sgemm": (i32, [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
                                       ctypes.c_float, vp, i64]),
+            "slate_amd_sgemm_bf16": (i32, [vp, i32, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
+                                           vp, i64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -292,6 +295,16 @@ def gemm(alpha, A, B, beta, C):
     pc, ldc, _ = _mat(C, ("float64", "float32"))
     f = lib().slate_amd_dgemm if C.dtype.itemsize == 8 else lib().slate_amd_sgemm
     _check(f(context(dev), C.shape[0], _ncols(C), A.shape[1], alpha, pa, lda, pb, ldb, beta, pc, ldc))
+
+
+def gemm_bf16(terms, alpha, A, B, C):
+    """C += alpha A B (float32, column-major) on the split-BF16 MFMA kernel
+    with `terms` (1, 2 or 3) bf16 terms per operand."""
+    pa, lda, dev = _mat(A, ("float32",))
+    pb, ldb, _ = _mat(B, ("float32",))
+    pc, ldc, _ = _mat(C, ("float32",))
+    _check(lib().slate_amd_sgemm_bf16(context(dev), int(terms), C.shape[0], _ncols(C), A.shape[1],
+                                      alpha, pa, lda, pb, ldb, pc, ldc))
 
 
 def profile_enable(on=True, device=0):

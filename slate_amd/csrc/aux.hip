@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256)
 void generate_kernel(int dist, uint64_t seed, int64_t m, int64_t n, int64_t ioff, int64_t joff,
                      double* __restrict__ A, int64_t lda)
 {
-    const int64_t j = blockIdx.y;
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y)
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < m; i += int64_t(gridDim.x) * 256) {
         uint64_t b0, b1;
         philox_dev(uint64_t(i + ioff), uint64_t(j + joff), seed, b0, b1);
@@ -382,13 +382,12 @@ void generate(int dist, int64_t seed, int64_t m, int64_t n, int64_t ioff, int64_
               double* A, int64_t lda, hipStream_t s)
 {
     if (m <= 0 || n <= 0) return;
-    for (int64_t j0 = 0; j0 < n; j0 += 65535) {
-        int64_t nn = std::min<int64_t>(65535, n - j0);
-        unsigned gx = unsigned(std::min<int64_t>(ceildiv(m, 256), 256));
-        hipLaunchKernelGGL(generate_kernel, dim3(gx, unsigned(nn)), dim3(256), 0, s, dist,
-                           uint64_t(seed), m, nn, ioff, joff + j0, A + j0 * lda, lda);
-        SA_CHECK_LAUNCH();
-    }
+    // a bounded grid (<= 256 x 2048 workgroups) that strides over the matrix
+    unsigned gx = unsigned(std::min<int64_t>(ceildiv(m, 256), 256));
+    unsigned gy = unsigned(std::min<int64_t>(n, 2048));
+    hipLaunchKernelGGL(generate_kernel, dim3(gx, gy), dim3(256), 0, s, dist,
+                       uint64_t(seed), m, n, ioff, joff, A, lda);
+    SA_CHECK_LAUNCH();
 }
 
 //------------------------------------------------------------------------------

@@ -40,9 +40,10 @@ struct Context {
     hipStream_t s1 = nullptr;          // panel + lookahead column
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
-    Buf counters, slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
+    Buf bfl, bfu0, bfu1, bfg;    // split-BF16 operand planes (LU update, gemm entry)
+    Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
-    int64_t counter_next = 0, counter_cap = 0;
+    uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
     // trailing-update GEMM timing (enabled by slate_amd_profile_enable)
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
@@ -92,6 +93,18 @@ struct Opts {
     int prec = SLATE_AMD_PREC_FP32;
 };
 
+// Number of bf16 terms per operand of the float LU's trailing update
+// (0 = FP32 MFMA).
+static int split_terms(int prec)
+{
+    switch (prec) {
+        case SLATE_AMD_PREC_BF16: return 1;
+        case SLATE_AMD_PREC_BF16X2: return 2;
+        case SLATE_AMD_PREC_BF16X3: return 3;
+        default: return 0;
+    }
+}
+
 static Opts read_opts(const slate_amd_options* o)
 {
     Opts r;
@@ -115,23 +128,26 @@ static int64_t choose_nb(int64_t n, int64_t requested)
     return 512;
 }
 
+// Tags of the panel exchange words advance monotonically over the life of the
+// context; the slot area is cleared when it is first allocated and whenever
+// the 32-bit tag space would wrap, so a stale word never carries a live tag.
 static void prepare_panel_work(Context& c, int64_t n, int64_t W)
 {
-    int64_t leaves = ceildiv(n, W) * 2 + 64;
-    if (c.counter_cap < leaves) {
-        c.counters.ensure(size_t(leaves) * sizeof(unsigned));
-        c.counter_cap = leaves;
+    (void)W;
+    const size_t bytes = size_t(1) << 22;
+    const bool fresh = c.slots.bytes < bytes;
+    c.slots.ensure(bytes);
+    if (fresh || c.leaf_tag > 0xF0000000u - uint32_t(std::min<int64_t>(n, 1 << 26))) {
+        zero_u32(static_cast<unsigned*>(c.slots.p), int64_t(c.slots.bytes / 4), c.s0);
+        c.leaf_tag = 1;
     }
-    zero_u32(c.counters.as<unsigned>(), leaves, c.s0);
-    c.counter_next = 0;
-    c.slots.ensure(size_t(1) << 22);
     c.err.ensure(sizeof(int) * 4);
     SA_HIP(hipMemsetAsync(c.err.p, 0, sizeof(int) * 4, c.s0));
 }
 
 static PanelWork panel_work(Context& c)
 {
-    return PanelWork{c.counters.as<unsigned>(), c.slots.p, int64_t(c.slots.bytes), c.err.as<int>()};
+    return PanelWork{c.slots.p, int64_t(c.slots.bytes), c.err.as<int>()};
 }
 
 static void check_kernel_error(Context& c)
@@ -245,8 +261,8 @@ static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64
                       int64_t* plan, hipStream_t s)
 {
     if (w <= leaf_w) {
-        panel_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, thr, panel_work(c), c.counter_next++, s);
-        SA_REQUIRE(c.counter_next <= c.counter_cap, -2, "panel: barrier counters exhausted");
+        panel_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, thr, panel_work(c), c.leaf_tag, s);
+        c.leaf_tag += uint32_t(w);
         return;
     }
     int64_t w1 = ((w / 2 + leaf_w - 1) / leaf_w) * leaf_w;
@@ -279,9 +295,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     int64_t leaf_w = panel_leaf_width(int(sizeof(T)));
     if (o.ib > 0 && o.ib < leaf_w) leaf_w = o.ib;
     prepare_panel_work(c, n, leaf_w);
-    c.plan0.ensure(sizeof(int64_t) * (4 * 2048 + 8));
     c.plan1.ensure(sizeof(int64_t) * (4 * 2048 + 8));
-    int64_t* plan0 = c.plan0.as<int64_t>();
     int64_t* plan1 = c.plan1.as<int64_t>();
     const int64_t nsteps = ceildiv(n, nb);
     const bool la = o.lookahead > 0 && nsteps > 2;
@@ -315,6 +329,50 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     auto step_inv = [&](int64_t k) { return c.sinv.as<T>() + (k % 2) * nb * nb; };
     T* tmp0 = c.ttmp.as<T>();
     T* tmp1 = c.stmp1.as<T>();
+
+    // Float factor with split-BF16 updates (factor_precision BF16, BF16X2,
+    // BF16X3): the L panel of each step is split once on s1 into one of two
+    // alternating slots (s0 is done with slot k%2 before s1 refills it at
+    // step k+2), and each U row block right before its update.
+    const int ns = std::is_same<T, float>::value ? split_terms(o.prec) : 0;
+    const int64_t Kp = ceildiv(nb, 32) * 32;
+    const int64_t lplane = n * Kp, uplane0 = n * Kp, uplane1 = nb * Kp;
+    if (ns > 0) {
+        c.bfl.ensure(sizeof(__bf16) * size_t(2 * ns) * size_t(lplane));
+        c.bfu0.ensure(sizeof(__bf16) * size_t(ns) * size_t(uplane0));
+        c.bfu1.ensure(sizeof(__bf16) * size_t(ns) * size_t(uplane1));
+    }
+    auto lslot = [&](int64_t k) { return c.bfl.as<__bf16>() + (k % 2) * ns * lplane; };
+    auto split_panel = [&](int64_t k, int64_t k0, int64_t kb, hipStream_t st) {
+        if constexpr (std::is_same<T, float>::value) {
+            if (ns > 0)
+                split_rows(ns, A + (k0 + kb) * ld + k0, ld, n - k0 - kb, kb, lslot(k), Kp, lplane, st);
+        }
+    };
+    // Update of columns [cb, cb + ncols) by step k (rows below the panel):
+    // C -= L21 U12, U12 already multiplied by L11^{-1}.
+    auto update = [&](int64_t k, int64_t k0, int64_t kb, int64_t cb, int64_t ncols, bool lookahead_block,
+                      hipStream_t st, int role) {
+        const int64_t c1 = k0 + kb;
+        if (ncols <= 0 || n - c1 <= 0) return;
+        if constexpr (std::is_same<T, float>::value) {
+            if (ns > 0) {
+                __bf16* uh = lookahead_block ? c.bfu1.as<__bf16>() : c.bfu0.as<__bf16>();
+                const int64_t up = lookahead_block ? uplane1 : uplane0;
+                split_cols_t(ns, A + k0 * ld + cb, ld, kb, ncols, uh, Kp, up, st);
+                gemm_bf16s(ns, n - c1, ncols, Kp, lslot(k), Kp, lplane, uh, Kp, up, A + c1 * ld + cb, ld,
+                           -1.0f, st, role);
+                return;
+            }
+        }
+        GemmArgs g{};
+        g.M = n - c1; g.N = ncols; g.K = kb;
+        g.A = A + c1 * ld + k0; g.lda = ld;
+        g.B = A + k0 * ld + cb; g.ldb = ld;
+        g.C = A + c1 * ld + cb; g.ldc = ld;
+        g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
+        gemm<T>(g, A_KC, B_NC, st, role);
+    };
     const int64_t plan_stride = 4 * nb + 8;
     c.plans.ensure(sizeof(int64_t) * size_t(plan_stride) * size_t(nsteps + 1));
     auto step_plan = [&](int64_t k) { return c.plans.as<int64_t>() + k * plan_stride; };
@@ -322,6 +380,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     panel_rec<T>(c, A, ld, n, 0, std::min(nb, n), ipiv, info, o.threshold, leaf_w, plan1, s1);
     laswp_plan(0, std::min(nb, n), n, ipiv, step_plan(0), s1);
     unit_lower_inverse<T>(A, ld, std::min(nb, n), step_inv(0), s1);
+    split_panel(0, 0, std::min(nb, n), s1);
     SA_HIP(hipEventRecord(ev_panel(0), s1));
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t k0 = k * nb, kb = std::min(nb, n - k0);
@@ -338,42 +397,28 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             SA_HIP(hipStreamWaitEvent(s1, k > 0 ? ev_trail(k) : ev_panel(0), 0));
             laswp<T>(A, ld, n, c1, c1 + nb1, k0, c1, ipiv, pk, s1, true);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, nb1, tmp1, s1);
-            GemmArgs g{};
-            g.M = r; g.N = nb1; g.K = kb;
-            g.A = A + c1 * ld + k0; g.lda = ld;
-            g.B = A + k0 * ld + c1; g.ldb = ld;
-            g.C = A + c1 * ld + c1; g.ldc = ld;
-            g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
-            gemm<T>(g, A_KC, B_NC, s1);
+            update(k, k0, kb, c1, nb1, true, s1, 0);
             panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
             laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
             unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
+            split_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
             // s0: step k on the columns beyond k+1
             const int64_t c2 = c1 + nb1;
             laswp<T>(A, ld, n, c2, n, k0, c1, ipiv, pk, s0, true);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c2, ld, n - c2, tmp0, s0);
-            g.N = n - c2;
-            g.B = A + k0 * ld + c2;
-            g.C = A + c1 * ld + c2;
             prof_begin(c, s0);
-            gemm<T>(g, A_KC, B_NC, s0);
-            prof_end(c, s0, 2.0 * double(g.M) * double(g.N) * double(g.K));
+            update(k, k0, kb, c2, n - c2, false, s0, 1);
+            prof_end(c, s0, 2.0 * double(r) * double(n - c2) * double(kb));
             SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
         }
         else {
             // no lookahead: whole trailing matrix on s0, then the next panel
             laswp<T>(A, ld, n, c1, n, k0, c1, ipiv, pk, s0, true);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, r, tmp0, s0);
-            GemmArgs g{};
-            g.M = r; g.N = r; g.K = kb;
-            g.A = A + c1 * ld + k0; g.lda = ld;
-            g.B = A + k0 * ld + c1; g.ldb = ld;
-            g.C = A + c1 * ld + c1; g.ldc = ld;
-            g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
             prof_begin(c, s0);
-            gemm<T>(g, A_KC, B_NC, s0);
-            prof_end(c, s0, 2.0 * double(g.M) * double(g.N) * double(g.K));
+            update(k, k0, kb, c1, r, false, s0, 1);
+            prof_end(c, s0, 2.0 * double(r) * double(r) * double(kb));
             if (la) {
                 SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
                 SA_HIP(hipStreamWaitEvent(s1, ev_trail(k + 1), 0));
@@ -381,6 +426,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
             laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
             unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
+            split_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
         }
     }
@@ -422,6 +468,12 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
 {
     const int64_t nb = choose_nb(n, o.nb);
     hipStream_t s = c.s0;
+    // Float factor with split-BF16 updates: U12 (kb x r) is split and
+    // transposed once per step into r x Kp planes, which are both operands
+    // of the triangular update C -= U12^T U12.
+    const int ns = std::is_same<T, float>::value ? split_terms(o.prec) : 0;
+    const int64_t Kp = ceildiv(nb, 32) * 32, plane = n * Kp;
+    if (ns > 0) c.bfu0.ensure(sizeof(__bf16) * size_t(ns) * size_t(plane));
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
         int64_t kb = std::min(nb, n - k0);
         T* Akk = A + k0 * ld + k0;
@@ -429,6 +481,15 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         int64_t r = n - k0 - kb;
         if (r <= 0) break;
         trsm_wide<T>(c, Akk, ld, 1, 1, 0, kb, Akk + kb, ld, r, s);
+        if constexpr (std::is_same<T, float>::value) {
+            if (ns > 0) {
+                __bf16* u = c.bfu0.as<__bf16>();
+                split_cols_t(ns, Akk + kb, ld, kb, r, u, Kp, plane, s);
+                gemm_bf16s(ns, r, r, Kp, u, Kp, plane, u, Kp, plane, A + (k0 + kb) * ld + k0 + kb, ld,
+                           -1.0f, s, 0, 1);
+                continue;
+            }
+        }
         GemmArgs g{};
         g.M = r; g.N = r; g.K = kb;
         g.A = Akk + kb; g.lda = ld; g.B = Akk + kb; g.ldb = ld;
@@ -1113,6 +1174,33 @@ static int gemm_entry(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, do
     }));
 }
 
+// C (column-major m x n) += alpha * A B through the split-BF16 kernel:
+// column-major C is row-major C^T = B^T A^T, so B^T is the K-contiguous left
+// operand (split_rows) and A^T the right one (split_cols_t).
+static int gemm_bf16_entry(slate_amd_context ctx, int terms, int64_t m, int64_t n, int64_t k, float alpha,
+                           const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(terms >= 1 && terms <= 3, -2, "terms must be 1, 2 or 3");
+        SA_REQUIRE(m >= 0 && n >= 0 && k >= 0, -3, "bad m/n/k");
+        SA_REQUIRE(lda >= std::max<int64_t>(1, m) && ldb >= std::max<int64_t>(1, k)
+                   && ldc >= std::max<int64_t>(1, m), -3, "bad leading dimension");
+        if (m == 0 || n == 0 || k == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Timer t(c);
+        const int64_t Kp = sa::ceildiv(k, 32) * 32;
+        const int64_t pa = n * Kp, pb = m * Kp;
+        c.bfg.ensure(sizeof(__bf16) * size_t(terms) * size_t(pa + pb));
+        __bf16* a = c.bfg.as<__bf16>();
+        __bf16* b = a + terms * pa;
+        sa::split_rows(terms, B, ldb, n, k, a, Kp, pa, c.s0);
+        sa::split_cols_t(terms, A, lda, k, m, b, Kp, pb, c.s0);
+        sa::gemm_bf16s(terms, n, m, Kp, a, Kp, pa, b, Kp, pb, C, ldc, alpha, c.s0, 0, 0);
+        t.stop();
+        return 0;
+    }));
+}
+
 namespace {
 void check_dims(int64_t n, int64_t lda) { SA_REQUIRE(n >= 0 && lda >= std::max<int64_t>(1, n), -3, "bad n/lda"); }
 void check_rhs(int64_t n, int64_t nrhs, int64_t ldb) { SA_REQUIRE(nrhs >= 0 && ldb >= std::max<int64_t>(1, n), -4, "bad nrhs/ldb"); }
@@ -1144,7 +1232,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     sa::Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
-    for (sa::Buf* b : {&c.counters, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
+    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
@@ -1403,6 +1491,10 @@ int slate_amd_sgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, floa
                     const float* dA, int64_t lda, const float* dB, int64_t ldb, float beta,
                     float* dC, int64_t ldc)
 { return gemm_entry<float>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc); }
+
+int slate_amd_sgemm_bf16(slate_amd_context ctx, int terms, int64_t m, int64_t n, int64_t k, float alpha,
+                         const float* dA, int64_t lda, const float* dB, int64_t ldb, float* dC, int64_t ldc)
+{ return gemm_bf16_entry(ctx, terms, m, n, k, alpha, dA, lda, dB, ldb, dC, ldc); }
 
 int slate_amd_profile_enable(slate_amd_context ctx, int on)
 {

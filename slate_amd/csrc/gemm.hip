@@ -41,7 +41,10 @@ template <> struct Mf<float> {
 
 // A layout: A_KC = K-contiguous (A[m*lda + k]), A_MC = M-contiguous (A[m + k*lda]).
 // B layout: B_NC = N-contiguous (B[k*ldb + n]), B_KC = K-contiguous (B[k + n*ldb]).
-template <class T, int ALAY, int BLAY, int BM, int BN, int BK, int WM = 2, int WN = 2, int MINB = 1>
+// ROLE only names the instantiation: 1 = the trailing-matrix update of the
+// blocked LU (profiled on its own), 0 = every other product.
+template <class T, int ALAY, int BLAY, int BM, int BN, int BK, int WM = 2, int WN = 2, int MINB = 1,
+          int ROLE = 0>
 __global__ __launch_bounds__(WM * WN * 64, MINB)
 void gemm_kernel(GemmArgs p)
 {
@@ -197,7 +200,7 @@ void gemm_kernel(GemmArgs p)
             }
 }
 
-template <class T, int ALAY, int BLAY>
+template <class T, int ALAY, int BLAY, int ROLE = 0>
 static void launch(const GemmArgs& a, hipStream_t s)
 {
     if (a.M <= 0 || a.N <= 0) return;
@@ -206,26 +209,30 @@ static void launch(const GemmArgs& a, hipStream_t s)
     constexpr int BM = 128, BN = 128, BK = (sizeof(T) == 8) ? 16 : 32;
     int64_t nt = ceildiv(a.M, BM) * ceildiv(a.N, BN);
     SA_REQUIRE(nt < (int64_t(1) << 31), -2, "gemm: grid too large");
-    hipLaunchKernelGGL((gemm_kernel<T, ALAY, BLAY, BM, BN, BK, 2, 2, 2>), dim3(unsigned(nt)), dim3(256),
+    hipLaunchKernelGGL((gemm_kernel<T, ALAY, BLAY, BM, BN, BK, 2, 2, 2, ROLE>), dim3(unsigned(nt)), dim3(256),
                        0, s, a);
     SA_CHECK_LAUNCH();
 }
 
 template <class T>
-void gemm(const GemmArgs& a, int alay, int blay, hipStream_t s)
+void gemm(const GemmArgs& a, int alay, int blay, hipStream_t s, int role)
 {
     if (a.K <= 0) {
         // C = beta * C only; the factorizations never request this.
         SA_REQUIRE(a.beta == 1.0, -2, "gemm: K == 0 with beta != 1 unsupported");
         return;
     }
-    if (alay == A_KC && blay == B_NC) launch<T, A_KC, B_NC>(a, s);
+    if (role == 1) {
+        SA_REQUIRE(alay == A_KC && blay == B_NC, -2, "gemm: trailing-update role needs A_KC/B_NC");
+        launch<T, A_KC, B_NC, 1>(a, s);
+    }
+    else if (alay == A_KC && blay == B_NC) launch<T, A_KC, B_NC>(a, s);
     else if (alay == A_MC && blay == B_NC) launch<T, A_MC, B_NC>(a, s);
     else if (alay == A_KC && blay == B_KC) launch<T, A_KC, B_KC>(a, s);
     else launch<T, A_MC, B_KC>(a, s);
 }
 
-template void gemm<double>(const GemmArgs&, int, int, hipStream_t);
-template void gemm<float>(const GemmArgs&, int, int, hipStream_t);
+template void gemm<double>(const GemmArgs&, int, int, hipStream_t, int);
+template void gemm<float>(const GemmArgs&, int, int, hipStream_t, int);
 
 } // namespace sa

@@ -20,23 +20,39 @@ struct GemmArgs {                // C[M x N] = alpha * A * B + beta * C, C row-m
     int uplo;                    // 0: all of C; 1: only entries with col >= row
 };
 
-template <class T> void gemm(const GemmArgs& a, int alay, int blay, hipStream_t s);
+// role 1 marks the LU trailing-matrix update (a separately named kernel).
+template <class T> void gemm(const GemmArgs& a, int alay, int blay, hipStream_t s, int role = 0);
+
+// Split-BF16 update (gemm_bf16.hip): split_rows turns an M x K row-major
+// fp32 block into ns bf16 planes of M x Kp (plane stride `plane`),
+// split_cols_t a K x N row-major block into ns planes of N x Kp (transposed);
+// gemm_bf16s: C (fp32 row-major) += alpha * sum_{i+j<ns} A_i B_j^T (uplo = 1:
+// only entries with column >= row).
+void split_rows(int ns, const float* src, int64_t ld, int64_t M, int64_t K, __bf16* dst, int64_t Kp,
+                int64_t plane, hipStream_t s);
+void split_cols_t(int ns, const float* src, int64_t ld, int64_t K, int64_t N, __bf16* dst, int64_t Kp,
+                  int64_t plane, hipStream_t s);
+void gemm_bf16s(int ns, int64_t M, int64_t N, int64_t Kp, const __bf16* A, int64_t lda, int64_t planeA,
+                const __bf16* B, int64_t ldb, int64_t planeB, float* C, int64_t ldc, float alpha,
+                hipStream_t s, int role = 0, int uplo = 0);
 
 // Workspace shared by the cross-workgroup panel kernel.
 struct PanelWork {
-    unsigned* counters;          // barrier counters, one per leaf launch (zeroed)
-    void* slots;                 // candidate rows exchanged between workgroups
+    void* slots;                 // tagged candidate words exchanged between workgroups
     int64_t slots_bytes;
-    int* error;                  // set to 1 if a barrier wait times out
+    int* error;                  // set to 1 if a wait for another workgroup times out
 };
 
 // Partial-pivot LU of a leaf panel: rows [r0, m_total) x columns [c0, c0+w) of
 // a row-major matrix.  Writes ipiv[c0 + j] = global 0-based pivot row and
 // sets *info (1-based global column) at the first exactly-zero pivot.
+// Column j exchanges words tagged tag0 + j: the caller hands every launch on
+// the same slot area fresh tags (it advances tag0 by w), and slot words start
+// at tag 0, which is never used.
 template <class T>
 void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
                 int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
-                int64_t counter_index, hipStream_t s);
+                uint32_t tag0, hipStream_t s);
 int panel_leaf_width(int elem_bytes);   // largest leaf width the kernel supports
 
 // Row interchanges: apply swaps (i, ipiv[i]) for i in [k1, k2), in order, to

@@ -5,9 +5,10 @@
 // <HostTask>, src/internal/internal_getrf.cc:20-121, which pulls the panel to
 // the CPU every step) by a GPU kernel in which G workgroups each keep a slice
 // of rows in registers.  Per column, each workgroup finds its local pivot
-// candidate with wavefront shuffles, publishes it (value, row, and the row's W
-// entries) to a double-buffered exchange area, and meets the others at a
-// grid barrier; every workgroup then reduces the G candidates itself, applies
+// candidate with wavefront shuffles and publishes it (value, row, and the row's
+// W entries) as tagged words to a double-buffered exchange area; every
+// workgroup reads all G candidates (one memory round trip once they have
+// landed, with no grid barrier and no fences), reduces them itself, applies
 // the interchange to the rows it owns, scales and performs the rank-1 update.
 // Pivot rules follow src/internal/Tile_getrf.hh:197-366: largest |a_ij| with
 // the smallest row winning exact ties, the diagonal kept when
@@ -29,21 +30,26 @@ template <> struct LeafW<float> { static constexpr int W = 32; };
 
 int panel_leaf_width(int elem_bytes) { return elem_bytes == 8 ? LeafW<double>::W : LeafW<float>::W; }
 
-// Exchange area layout (per parity): cand_abs[G], cand_idx[G], cand_row[G][W], rowj[W].
+// Exchange area (64-bit tagged words, see tw_store in common.hh), per column
+// parity: G candidate entries of CW words [|a| (NWV words), row index (1
+// word), the candidate row (W values)], then row j (W values).  NWV = words
+// per value.  Column j of a launch carries tag tag0 + j; alternating parity
+// areas keep a workgroup that runs one column ahead from overwriting words
+// that a slower one has not read yet.
 template <class T, int W>
-struct Slots {
-    T* abs_; int64_t* idx_; T* row_; T* rowj_;
-    __device__ Slots(void* base, int G, int par) {
-        char* b = static_cast<char*>(base);
-        size_t per = size_t(G) * (sizeof(T) + sizeof(int64_t) + W * sizeof(T)) + W * sizeof(T);
-        per = (per + 255) & ~size_t(255);
-        b += par * per;
-        abs_ = reinterpret_cast<T*>(b);
-        idx_ = reinterpret_cast<int64_t*>(b + size_t(G) * sizeof(T));
-        row_ = reinterpret_cast<T*>(b + size_t(G) * (sizeof(T) + sizeof(int64_t)));
-        rowj_ = row_ + size_t(G) * W;
-    }
+struct LeafSlots {
+    static constexpr int NWV = int(sizeof(T)) / 4;
+    static constexpr int CW = NWV * (W + 1) + 1;
+    static constexpr int MAXWORDS = LEAF_GMAX * CW + W * NWV;
+    __host__ __device__ static int64_t per(int G) { return (int64_t(G) * CW + W * NWV + 63) & ~int64_t(63); }
 };
+
+template <class T>
+__device__ inline T lds_val(const uint32_t* w)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, (uint64_t(w[1]) << 32) | w[0]);
+    else return __builtin_bit_cast(T, w[0]);
+}
 
 template <class T>
 __device__ inline bool better(T a, int64_t ia, T b, int64_t ib)
@@ -55,20 +61,22 @@ template <class T, int W, int RPT, int VEC>
 __global__ __launch_bounds__(LEAF_THREADS)
 void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0,
                        int w, int64_t* __restrict__ ipiv, int64_t* __restrict__ info, T threshold,
-                       unsigned* counter, void* slots, int* error)
+                       uint32_t tag0, uint64_t* slots, int* error)
 {
+    using SL = LeafSlots<T, W>;
+    constexpr int NWV = SL::NWV, CW = SL::CW;
     const int G = gridDim.x;
     const int g = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t m = m_total - r0;
     constexpr int64_t R = int64_t(LEAF_THREADS) * RPT;
     const int64_t gbase = int64_t(g) * R;
+    const int total = G * CW + W * NWV;     // words exchanged per column
+    const int64_t per = SL::per(G);
 
     __shared__ T s_wabs[4];
     __shared__ int64_t s_widx[4];
-    __shared__ T s_cabs[LEAF_GMAX];
-    __shared__ int64_t s_cidx[LEAF_GMAX];
-    __shared__ T s_crow[LEAF_GMAX * W];
+    __shared__ uint32_t s_words[SL::MAXWORDS];
     __shared__ T s_rowj[W];
     __shared__ T s_piv[W];
     __shared__ int64_t s_p;
@@ -102,8 +110,8 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
     // compile-time indices under predicates so v[][] never spills.
     #pragma unroll 1
     for (int j = 0; j < w; ++j) {
-        const int par = j & 1;
-        Slots<T, W> sl(slots, G, par);
+        uint64_t* buf = slots + (j & 1) * per;
+        const uint32_t tag = tag0 + uint32_t(j);
 
         // (1) local candidate: first row i >= j of maximal |a_ij|
         T babs = T(-1);
@@ -130,7 +138,9 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
         for (int k = 1; k < 4; ++k)
             if (better(s_wabs[k], s_widx[k], babs, bidx)) { babs = s_wabs[k]; bidx = s_widx[k]; }
 
-        // (2) publish candidate row and, for the owner, the current row j
+        // (2) publish the candidate (|a|, row, row values) and, from the owner
+        // of row j, row j itself, all as tagged words: no fence, no barrier.
+        uint64_t* ent = buf + int64_t(g) * CW;
         if (bidx != INT64_MAX && bidx >= gbase && bidx < gbase + R
             && (bidx - gbase) % LEAF_THREADS == tid) {
             int qq = int((bidx - gbase) / LEAF_THREADS);
@@ -138,9 +148,8 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
             for (int q = 0; q < RPT; ++q)
                 if (q == qq) {
                     #pragma unroll
-                    for (int c = 0; c < W; ++c) sl.row_[size_t(g) * W + c] = v[q][c];
+                    for (int c = 0; c < W; ++c) tw_store_val<T>(ent + NWV + 1 + c * NWV, v[q][c], tag);
                 }
-            sa_publish_fence();
         }
         if (int64_t(j) >= gbase && int64_t(j) < gbase + R && (j - gbase) % LEAF_THREADS == tid) {
             int qj = int((j - gbase) / LEAF_THREADS);
@@ -148,62 +157,48 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
             for (int q = 0; q < RPT; ++q)
                 if (q == qj) {
                     #pragma unroll
-                    for (int c = 0; c < W; ++c) sl.rowj_[c] = v[q][c];
+                    for (int c = 0; c < W; ++c) tw_store_val<T>(buf + int64_t(G) * CW + c * NWV, v[q][c], tag);
                 }
-            sa_publish_fence();
         }
+        if (bidx == INT64_MAX && tid < W)        // no candidate here: tagged zeros
+            tw_store_val<T>(ent + NWV + 1 + tid * NWV, T(0), tag);
         if (tid == 0) {
-            sl.abs_[g] = babs;
-            sl.idx_[g] = bidx;
+            tw_store_val<T>(ent, babs, tag);
+            tw_store(ent + NWV, bidx == INT64_MAX ? 0xFFFFFFFFu : uint32_t(bidx), tag);
         }
-        __syncthreads();
 
-        // (3) grid barrier with a bounded wait
-        if (tid == 0) {
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned target = unsigned(G) * unsigned(j + 1);
-            uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                __builtin_amdgcn_s_sleep(1);
-                if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-                if (wall_clock64() - t0 > 400000000ull) {       // ~4 s at 100 MHz
-                    __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-            sa_publish_fence();
-        }
-        __syncthreads();
-
-        // (4) gather all candidates and row j, pick the pivot.  Fixed trip
-        // counts and clamped addresses keep every load independent (one
-        // memory round trip for the whole gather, not one per iteration).
+        // (3) gather every word of this column: all loads issued at once
+        // (fixed trip count, clamped addresses), then only the words whose
+        // tag is not there yet are polled (bounded wait).
         {
-            constexpr int NCR = (LEAF_GMAX * W + LEAF_THREADS - 1) / LEAF_THREADS;
-            T rv[NCR];
+            constexpr int NCR = (SL::MAXWORDS + LEAF_THREADS - 1) / LEAF_THREADS;
+            uint64_t wv[NCR];
             #pragma unroll
             for (int u = 0; u < NCR; ++u) {
                 int e = tid + u * LEAF_THREADS;
-                rv[u] = sl.row_[e < G * W ? e : G * W - 1];
+                wv[u] = tw_load(buf + (e < total ? e : total - 1));
             }
-            T av = sl.abs_[tid < G ? tid : G - 1];
-            int64_t xv = sl.idx_[tid < G ? tid : G - 1];
-            T jv = sl.rowj_[tid < W ? tid : W - 1];
             #pragma unroll
             for (int u = 0; u < NCR; ++u) {
                 int e = tid + u * LEAF_THREADS;
-                if (e < G * W) s_crow[e] = rv[u];
+                if (e < total) {
+                    uint32_t x = uint32_t(wv[u]);
+                    if (uint32_t(wv[u] >> 32) != tag) x = tw_wait(buf + e, tag, error, 1);
+                    s_words[e] = x;
+                }
             }
-            if (tid < G) { s_cabs[tid] = av; s_cidx[tid] = xv; }
-            if (tid < W) s_rowj[tid] = jv;
         }
         __syncthreads();
         if (wave == 0) {
             T ba = T(-1);
             int64_t bi = INT64_MAX;
             int bg = -1;
-            for (int e = lane; e < G; e += 64)
-                if (better(s_cabs[e], s_cidx[e], ba, bi)) { ba = s_cabs[e]; bi = s_cidx[e]; bg = e; }
+            for (int e = lane; e < G; e += 64) {
+                T a = lds_val<T>(&s_words[e * CW]);
+                uint32_t x = s_words[e * CW + NWV];
+                int64_t xi = (x == 0xFFFFFFFFu) ? INT64_MAX : int64_t(x);
+                if (better(a, xi, ba, bi)) { ba = a; bi = xi; bg = e; }
+            }
             #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 T oa = __shfl_xor(ba, o, 64);
@@ -211,17 +206,22 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
                 int og = __shfl_xor(bg, o, 64);
                 if (better(oa, oi, ba, bi)) { ba = oa; bi = oi; bg = og; }
             }
+            const T rj = lds_val<T>(&s_words[G * CW + (lane < W ? lane : 0) * NWV]);
+            const T ajj = lds_val<T>(&s_words[G * CW + j * NWV]);
             // Tile_getrf.hh:283-289: keep the diagonal if it is large enough.
-            bool diag = !(bi != INT64_MAX) || fabs(s_rowj[j]) >= threshold * ba;
+            bool diag = !(bi != INT64_MAX) || bg < 0 || fabs(ajj) >= threshold * ba;
             int64_t p = diag ? int64_t(j) : bi;
             if (p < j || p >= m) p = j;            // never trust a row outside the leaf
-            if (lane < W) s_piv[lane] = diag ? s_rowj[lane] : s_crow[bg * W + lane];
+            if (lane < W) {
+                s_rowj[lane] = rj;
+                s_piv[lane] = diag ? rj : lds_val<T>(&s_words[(bg < 0 ? 0 : bg) * CW + NWV + 1 + lane * NWV]);
+            }
             if (lane == 0) s_p = p;
         }
         __syncthreads();
         const int64_t p = s_p;
 
-        // (5) interchange rows j and p among the rows this thread owns
+        // (4) interchange rows j and p among the rows this thread owns
         #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
@@ -239,7 +239,7 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
             ipiv[c0 + j] = r0 + p;
             if (pv == T(0) && *info == 0) *info = c0 + j + 1;
         }
-        // (6) scale the column below the pivot and update the leaf to the right
+        // (5) scale the column below the pivot and update the leaf to the right
         const bool tiny = fabs(pv) < safe_min;
         const T inv = T(1) / pv;
         #pragma unroll
@@ -257,7 +257,9 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
                 }
             }
         }
-        __syncthreads();
+        // No barrier here: the next writes of s_wabs, s_words, s_piv and
+        // s_rowj all come after barriers that every wave reaches only once it
+        // has finished this column.
     }
 
     #pragma unroll
@@ -287,25 +289,25 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
 template <class T>
 void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
                 int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
-                int64_t counter_index, hipStream_t s)
+                uint32_t tag0, hipStream_t s)
 {
     constexpr int W = LeafW<T>::W;
     const int64_t m = m_total - r0;
     SA_REQUIRE(w >= 1 && w <= W, -2, "panel_leaf: bad width");
     SA_REQUIRE(m >= w, -2, "panel_leaf: fewer rows than columns");
+    SA_REQUIRE(m < int64_t(0xFFFFFFFFu), -2, "panel_leaf: panel too tall");
     int rpt = 1;
     while (rpt < LEAF_RPT_MAX && ceildiv(m, int64_t(LEAF_THREADS) * rpt) > 32) rpt *= 2;
     int64_t G = ceildiv(m, int64_t(LEAF_THREADS) * rpt);
     SA_REQUIRE(G <= LEAF_GMAX, -2, "panel_leaf: panel too tall");
-    size_t per = size_t(G) * (sizeof(T) + sizeof(int64_t) + W * sizeof(T)) + W * sizeof(T);
-    per = (per + 255) & ~size_t(255);
-    SA_REQUIRE(int64_t(2 * per) <= pw.slots_bytes, -2, "panel_leaf: slot area too small");
-    unsigned* ctr = pw.counters + counter_index;
+    const int64_t slot_bytes = int64_t(2 * sizeof(uint64_t)) * LeafSlots<T, W>::per(int(G));
+    SA_REQUIRE(slot_bytes <= pw.slots_bytes, -2, "panel_leaf: slot area too small");
     T thr = T(threshold);
     constexpr int VE = 16 / int(sizeof(T));
     const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (ld % VE == 0) && (c0 % VE == 0);
+    uint64_t* slots = static_cast<uint64_t*>(pw.slots);
 #define SA_LEAF(R, V) hipLaunchKernelGGL((panel_leaf_kernel<T, W, R, V>), dim3(G), dim3(LEAF_THREADS), 0, s, \
-                                          A, ld, r0, m_total, c0, w, ipiv, info, thr, ctr, pw.slots, pw.error)
+                                          A, ld, r0, m_total, c0, w, ipiv, info, thr, tag0, slots, pw.error)
     if (vec) {
         switch (rpt) { case 1: SA_LEAF(1, 1); break; case 2: SA_LEAF(2, 1); break; default: SA_LEAF(4, 1); break; }
     }
@@ -317,9 +319,9 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
 }
 
 template void panel_leaf<double>(double*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t*,
-                                 double, const PanelWork&, int64_t, hipStream_t);
+                                 double, const PanelWork&, uint32_t, hipStream_t);
 template void panel_leaf<float>(float*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t*,
-                                double, const PanelWork&, int64_t, hipStream_t);
+                                double, const PanelWork&, uint32_t, hipStream_t);
 
 //------------------------------------------------------------------------------
 // Row interchanges.  The sequential LAPACK-style swap list (i, ipiv[i]),
