@@ -161,6 +161,27 @@ int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double*
 int slate_amd_generate(slate_amd_context ctx, int dist, int64_t seed, int64_t m, int64_t n,
                        int64_t ioffset, int64_t joffset, double* dA, int64_t lda);
 
+/* slate::gecondest(Norm, A, Anorm, Options) — src/gecondest.cc:15-118:
+ * reciprocal condition number estimate from the LU factors of getrf
+ * (Hager/Higham estimate of ||A^{-1}||, src/internal/internal_norm1est.cc).
+ * norm = '1' or 'O' (one norm) or 'I' (infinity norm); anorm = ||A|| in that
+ * norm (slate_amd_dnorm_inf for 'I').  Returns 0 or a negative error. */
+int64_t slate_amd_dgecondest(slate_amd_context ctx, int norm, int64_t n, const double* dLU, int64_t lda,
+                             const int64_t* ipiv, double anorm, double* rcond);
+/* slate::pocondest — src/pocondest.cc: the same from the Cholesky factor. */
+int64_t slate_amd_dpocondest(slate_amd_context ctx, int uplo, int64_t n, const double* dL, int64_t lda,
+                             double anorm, double* rcond);
+
+/* Test matrix with a prescribed spectrum: A = U diag(sigma) V^T, U and V
+ * products of n Householder reflectors whose vectors are the columns of
+ * matgen randn matrices (seed + 1 for U, seed for V); sym != 0 gives
+ * A = V diag(sigma) V^T.  sigma_dist 1 arith, 2 geo, 3 cluster0, 4 cluster1
+ * with condition number cond (matgen/generate_sigma.hh:61-111).  The
+ * reference's "svd" kind (matgen/generate_type_svd.hh:34-200) builds U, V
+ * from the QR of a Gaussian matrix instead. */
+int slate_amd_generate_svd(slate_amd_context ctx, int sigma_dist, double cond, int64_t seed, int sym,
+                           int64_t n, double* dA, int64_t lda);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,11 @@ dsgesv_gmres      src/gesv_mixed_gmres.cc:111-410             same, plus Arnoldi
                                                               GPU path; exact iteration counts are
                                                               parity unpinned (rounding-dependent)
 rotg              blas::rotg (reference BLAS drotg)           closed form: c*a+s*b=r, -s*a+c*b=0
+sigma             matgen/generate_sigma.hh:61-111             closed forms (endpoints 1 and 1/cond,
+                                                              geometric ratio, arithmetic step)
+generate_svd      (this library's generator; the reference's  invariants: singular values = sigma,
+                  svd kind, generate_type_svd.hh:90-157,      orthogonality, det = (-1)^n of each
+                  uses QR of a Gaussian instead)              n-reflector product, symmetry for sym
 norms/residual    src/norm.cc, src/colNorms.cc                numpy closed forms
 ================  ==========================================  =====================================
 """
@@ -243,6 +248,45 @@ def posv_mixed_gmres(A, b, nb=256, itermax=30, tol=0.0, use_fallback=True):
     info = lib().oracle_dsposv_gmres(n, _ptr(A), n, _ptr(b), _ptr(x), nb, itermax, tol,
                                      int(use_fallback), ctypes.byref(it))
     return A, x, int(info), it.value
+
+
+def sigma(dist, cond, n):
+    """matgen/generate_sigma.hh:61-111: 1 arith, 2 geo, 3 cluster0, 4 cluster1."""
+    i = np.arange(n, dtype=np.float64)
+    t = i / (n - 1) if n > 1 else np.zeros(n)
+    if dist == 1:
+        return 1.0 - t * (1.0 - 1.0 / cond)
+    if dist == 2:
+        return cond ** (-t)
+    if dist == 3:
+        s = np.full(n, 1.0 / cond)
+        s[0] = 1.0
+        return s
+    if dist == 4:
+        s = np.ones(n)
+        s[-1] = 1.0 / cond
+        return s
+    raise ValueError("unknown sigma distribution %r" % dist)
+
+
+def reflector_product(seed, n):
+    """Q = H_1 H_2 ... H_n, H_i = I - 2 v_i v_i^T / (v_i^T v_i), v_i = column i
+    of matgen randn(seed) (n x n), applied one reflector at a time."""
+    V = randn(seed, n, n)
+    Q = np.eye(n)
+    for i in range(n - 1, -1, -1):          # Q := H_i Q, last reflector first
+        v = V[:, i]
+        Q -= np.outer(v, (2.0 / (v @ v)) * (v @ Q))
+    return Q
+
+
+def generate_svd(sigma_dist, cond, seed, n, sym=False):
+    """A = U diag(sigma) V^T with U = reflector_product(seed + 1, n) and
+    V = reflector_product(seed, n); sym: A = V diag(sigma) V^T."""
+    sg = sigma(sigma_dist, cond, n)
+    V = reflector_product(seed, n)
+    U = V if sym else reflector_product(seed + 1, n)
+    return (U * sg) @ V.T
 
 
 def norm_inf(A):

@@ -25,6 +25,7 @@ PREC_FP32, PREC_BF16, PREC_BF16X2, PREC_FP64, PREC_BF16X3 = 0, 1, 2, 3, 4
 LOWER, UPPER = 0, 1
 NOTRANS, TRANS = 0, 1
 DIST_UNIFORM, DIST_RANDS, DIST_RANDN = 1, 2, 3
+SIGMA_ARITH, SIGMA_GEO, SIGMA_CLUSTER0, SIGMA_CLUSTER1 = 1, 2, 3, 4
 
 # Public C symbols (the C-ABI test checks that the library exports each one).
 C_SYMBOLS = [
@@ -33,7 +34,8 @@ C_SYMBOLS = [
     "slate_amd_dgetrf", "slate_amd_sgetrf", "slate_amd_dgetrs", "slate_amd_sgetrs",
     "slate_amd_dgesv", "slate_amd_dpotrf", "slate_amd_spotrf", "slate_amd_dpotrs",
     "slate_amd_dposv", "slate_amd_dsgesv", "slate_amd_dsgesv_gmres", "slate_amd_dsposv",
-    "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate",
+    "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate", "slate_amd_generate_svd",
+    "slate_amd_dgecondest", "slate_amd_dpocondest",
     "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_profile_enable",
     "slate_amd_profile_read",
 ]
@@ -101,6 +103,9 @@ def lib():
             "slate_amd_dsposv_gmres": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, vp, i64, pint, po]),
             "slate_amd_dnorm_inf": (i32, [vp, i32, i64, vp, i64, ctypes.POINTER(dbl)]),
             "slate_amd_generate": (i32, [vp, i32, i64, i64, i64, i64, i64, vp, i64]),
+            "slate_amd_generate_svd": (i32, [vp, i32, dbl, i64, i32, i64, vp, i64]),
+            "slate_amd_dgecondest": (i64, [vp, i32, i64, vp, i64, pi64, dbl, ctypes.POINTER(dbl)]),
+            "slate_amd_dpocondest": (i64, [vp, i32, i64, vp, i64, dbl, ctypes.POINTER(dbl)]),
             "slate_amd_profile_enable": (i32, [vp, i32]),
             "slate_amd_profile_read": (i32, [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl), pi64]),
             "slate_amd_dgemm": (i32, [vp, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
@@ -325,6 +330,38 @@ def norm_inf(A, sym=False):
     out = ctypes.c_double(0)
     _check(lib().slate_amd_dnorm_inf(context(dev), int(sym), A.shape[0], pa, lda, ctypes.byref(out)))
     return out.value
+
+
+def gecondest(LU, ipiv, anorm, norm="1"):
+    """slate::gecondest: reciprocal condition estimate from getrf's factors."""
+    pa, lda, dev = _mat(LU, ("float64",))
+    ip = np.ascontiguousarray(ipiv, dtype=np.int64)
+    out = ctypes.c_double(0)
+    _check(lib().slate_amd_dgecondest(context(dev), ord(norm), LU.shape[0], pa, lda, _p64(ip), float(anorm),
+                                      ctypes.byref(out)))
+    return out.value
+
+
+def pocondest(L, anorm, uplo=LOWER):
+    """slate::pocondest: reciprocal condition estimate from potrf's factor."""
+    pa, lda, dev = _mat(L, ("float64",))
+    out = ctypes.c_double(0)
+    _check(lib().slate_amd_dpocondest(context(dev), uplo, L.shape[0], pa, lda, float(anorm),
+                                      ctypes.byref(out)))
+    return out.value
+
+
+def generate_svd(A, sigma_dist, cond, seed, sym=False):
+    """A = U diag(sigma) V^T (sym: V diag(sigma) V^T) with U, V products of n
+    Householder reflectors from matgen randn columns; sigma per
+    matgen/generate_sigma.hh with condition number cond."""
+    pa, lda, dev = _mat(A, ("float64",))
+    n = A.shape[0]
+    if _ncols(A) != n:
+        raise SlateError("generate_svd: square matrices only")
+    _check(lib().slate_amd_generate_svd(context(dev), int(sigma_dist), float(cond), int(seed), int(sym), n,
+                                        pa, lda))
+    return A
 
 
 def generate(A, dist, seed, ioffset=0, joffset=0):

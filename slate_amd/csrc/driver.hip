@@ -1201,6 +1201,125 @@ static int gemm_bf16_entry(slate_amd_context ctx, int terms, int64_t m, int64_t 
     }));
 }
 
+namespace sa {
+
+// Estimate of ||A^{-1}||_1 by Hager's method with Higham's refinements, the
+// steps of LAPACK dlacn2 that SLATE's internal::norm1est restates
+// (src/internal/internal_norm1est.cc): x = 1/n, then alternately
+// y = A^{-1} x and z = A^{-T} sign(y), moving x to the unit vector of the
+// largest |z_j| until the sign vector repeats, the estimate stops growing or
+// five iterations pass; finally the alternating vector
+// x_i = (-1)^i (1 + i/(n-1)) guards against unlucky starts.  The vector logic
+// runs on the host (n doubles per step); solve(x, t) overwrites the device
+// vector x with A^{-1} x (t = 0) or A^{-T} x (t = 1).
+template <class Solve>
+static double norm1est_inv(int64_t n, double* xd, Solve solve)
+{
+    std::vector<double> x(size_t(n), 1.0 / double(n));
+    std::vector<int> isgn(size_t(n), 0);
+    auto run = [&](int t) {
+        SA_HIP(hipMemcpy(xd, x.data(), sizeof(double) * size_t(n), hipMemcpyHostToDevice));
+        solve(xd, t);
+        SA_HIP(hipMemcpy(x.data(), xd, sizeof(double) * size_t(n), hipMemcpyDeviceToHost));
+    };
+    auto asum = [&]() { double s = 0; for (double e : x) s += std::fabs(e); return s; };
+    auto imax = [&]() {
+        int64_t j = 0;
+        for (int64_t i = 1; i < n; ++i) if (std::fabs(x[size_t(i)]) > std::fabs(x[size_t(j)])) j = i;
+        return j;
+    };
+    run(0);
+    if (n == 1) return std::fabs(x[0]);
+    double est = asum();
+    for (int64_t i = 0; i < n; ++i) { isgn[size_t(i)] = x[size_t(i)] >= 0 ? 1 : -1; x[size_t(i)] = isgn[size_t(i)]; }
+    run(1);
+    int64_t j = imax();
+    for (int iter = 2;; ++iter) {
+        std::fill(x.begin(), x.end(), 0.0);
+        x[size_t(j)] = 1.0;
+        run(0);
+        const double estold = est;
+        est = asum();
+        bool repeated = true;
+        for (int64_t i = 0; i < n && repeated; ++i) repeated = (x[size_t(i)] >= 0 ? 1 : -1) == isgn[size_t(i)];
+        if (repeated || est <= estold) break;
+        for (int64_t i = 0; i < n; ++i) { isgn[size_t(i)] = x[size_t(i)] >= 0 ? 1 : -1; x[size_t(i)] = isgn[size_t(i)]; }
+        run(1);
+        const int64_t jlast = j;
+        j = imax();
+        if (!(x[size_t(jlast)] != std::fabs(x[size_t(j)]) && iter < 5)) break;
+    }
+    double alt = 1.0;
+    for (int64_t i = 0; i < n; ++i) { x[size_t(i)] = alt * (1.0 + double(i) / double(n - 1)); alt = -alt; }
+    run(0);
+    const double temp = 2.0 * asum() / double(3 * n);
+    return std::max(est, temp);
+}
+
+} // namespace sa
+
+namespace sa {
+
+// Spectrum-controlled test matrix, column-major n x n:
+//   A = U diag(sigma) V^T, U's reflectors the columns of randn(seed + 1) and
+//   V's those of randn(seed);  sym: A = V diag(sigma) V^T (SPD for cond >= 1).
+// Works on the row-major view R = A^T = V diag(sigma) U^T, applying the
+// compact-WY blocks Q_k = I - Y_k T_k Y_k^T of V = Q_1 ... Q_m from the left
+// and of U^T from the right, last block first; each block is 7 MFMA GEMMs.
+static void generate_svd(Context& c, int sdist, double cond, int64_t seed, int sym, int64_t n, double* R,
+                         int64_t ld)
+{
+    hipStream_t s = c.s0;
+    SA_HIP(hipMemset2DAsync(R, sizeof(double) * ld, 0, sizeof(double) * n, n, s));
+    sigma_diag(sdist, cond, n, R, ld, s);
+    const int64_t b = std::min<int64_t>(256, n);
+    c.work.ensure(sizeof(double) * size_t(3 * n * b + 2 * b * b + 64));
+    double* Y = c.work.as<double>();
+    double* W1 = Y + n * b;
+    double* W2 = W1 + n * b;
+    double* G = W2 + n * b;
+    double* T = G + b * b;
+    auto gm = [&](int64_t M, int64_t N, int64_t K, const double* A_, int64_t lda_, int alay, const double* B_,
+                  int64_t ldb_, int blay, double alpha, double beta, double* C_, int64_t ldc_) {
+        GemmArgs g{};
+        g.M = M; g.N = N; g.K = K; g.A = A_; g.lda = lda_; g.B = B_; g.ldb = ldb_;
+        g.C = C_; g.ldc = ldc_; g.alpha = alpha; g.beta = beta; g.uplo = 0;
+        gemm<double>(g, alay, blay, s);
+    };
+    // Y (n x bk, column-major), its T; Y(k, j) = Y[k + j n].
+    auto block = [&](int64_t sd, int64_t j0, int64_t bk) {
+        generate(3, sd, n, bk, 0, j0, Y, n, s);
+        gm(bk, bk, n, Y, n, A_KC, Y, n, B_KC, 1.0, 0.0, G, bk);          // G = Y^T Y
+        larft_from_gram(G, bk, T, s);
+    };
+    auto left = [&](int64_t bk) {                                        // R := Q R
+        gm(bk, n, n, Y, n, A_KC, R, ld, B_NC, 1.0, 0.0, W1, n);           // W1 = Y^T R
+        gm(bk, n, bk, T, bk, A_KC, W1, n, B_NC, 1.0, 0.0, W2, n);         // W2 = T W1
+        gm(n, n, bk, Y, n, A_MC, W2, n, B_NC, -1.0, 1.0, R, ld);          // R -= Y W2
+    };
+    auto right = [&](int64_t bk) {                                       // R := R Q^T
+        gm(n, bk, n, R, ld, A_KC, Y, n, B_KC, 1.0, 0.0, W1, bk);          // W1 = R Y
+        gm(n, bk, bk, W1, bk, A_KC, T, bk, B_KC, 1.0, 0.0, W2, bk);       // W2 = W1 T^T
+        gm(n, n, bk, W2, bk, A_KC, Y, n, B_NC, -1.0, 1.0, R, ld);         // R -= W2 Y^T
+    };
+    const int64_t nblk = ceildiv(n, b);
+    for (int64_t k = nblk - 1; k >= 0; --k) {
+        const int64_t j0 = k * b, bk = std::min(b, n - j0);
+        block(seed, j0, bk);
+        left(bk);
+        if (sym) right(bk);
+    }
+    if (!sym) {
+        for (int64_t k = nblk - 1; k >= 0; --k) {
+            const int64_t j0 = k * b, bk = std::min(b, n - j0);
+            block(seed + 1, j0, bk);
+            right(bk);
+        }
+    }
+}
+
+} // namespace sa
+
 namespace {
 void check_dims(int64_t n, int64_t lda) { SA_REQUIRE(n >= 0 && lda >= std::max<int64_t>(1, n), -3, "bad n/lda"); }
 void check_rhs(int64_t n, int64_t nrhs, int64_t ldb) { SA_REQUIRE(nrhs >= 0 && ldb >= std::max<int64_t>(1, n), -4, "bad nrhs/ldb"); }
@@ -1528,6 +1647,70 @@ int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double*
         c.work.ensure(sizeof(double) * (we + 8));
         double* w = c.work.as<double>();
         *result = sa::norm_host(c, dA, n, lda, sym, w, we, w + we);
+        return 0;
+    }));
+}
+
+// slate::gecondest (src/gecondest.cc): reciprocal condition number in the 1
+// (norm = 'O' or '1') or infinity (norm = 'I') norm from the LU factors.
+int64_t slate_amd_dgecondest(slate_amd_context ctx, int norm, int64_t n, const double* dLU, int64_t lda,
+                             const int64_t* ipiv, double anorm, double* rcond)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        SA_REQUIRE(norm == '1' || norm == 'O' || norm == 'o' || norm == 'I' || norm == 'i', -2, "invalid norm");
+        SA_REQUIRE(anorm >= 0 && !std::isnan(anorm), -7, "invalid Anorm");
+        SA_REQUIRE(rcond != nullptr, -8, "null rcond");
+        if (n <= 1) { *rcond = 1.0; return 0; }
+        if (anorm == 0.0) { *rcond = 0.0; return 0; }
+        SA_REQUIRE(ipiv != nullptr, -6, "null ipiv");
+        SA_ENTRY_SYNC();
+        int64_t* dip = sa::ipiv_from_host(c, ipiv, n);
+        c.basis.ensure(sizeof(double) * size_t(n));
+        const bool inf = (norm == 'I' || norm == 'i');
+        // ||A^{-1}||_inf = ||A^{-T}||_1: swap the two solves
+        double ainv = sa::norm1est_inv(n, c.basis.as<double>(), [&](double* x, int t) {
+            const int tr = (t == 1) != inf ? SLATE_AMD_TRANS : SLATE_AMD_NOTRANS;
+            sa::getrs_cm<double>(c, tr, dLU, lda, n, dip, x, n, 1);
+        });
+        *rcond = ainv != 0.0 ? (1.0 / ainv) / anorm : 0.0;
+        return 0;
+    });
+}
+
+// slate::pocondest (src/pocondest.cc): reciprocal condition number from the
+// Cholesky factor (A symmetric, so both norms coincide).
+int64_t slate_amd_dpocondest(slate_amd_context ctx, int uplo, int64_t n, const double* dL, int64_t lda,
+                             double anorm, double* rcond)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        SA_REQUIRE(uplo == SLATE_AMD_LOWER || uplo == SLATE_AMD_UPPER, -2, "invalid uplo");
+        SA_REQUIRE(anorm >= 0 && !std::isnan(anorm), -6, "invalid Anorm");
+        SA_REQUIRE(rcond != nullptr, -7, "null rcond");
+        if (n <= 1) { *rcond = 1.0; return 0; }
+        if (anorm == 0.0) { *rcond = 0.0; return 0; }
+        SA_ENTRY_SYNC();
+        c.basis.ensure(sizeof(double) * size_t(n));
+        double ainv = sa::norm1est_inv(n, c.basis.as<double>(), [&](double* x, int) {
+            sa::potrs_cm<double>(c, uplo, const_cast<double*>(dL), lda, n, x, n, 1);
+        });
+        *rcond = ainv != 0.0 ? (1.0 / ainv) / anorm : 0.0;
+        return 0;
+    });
+}
+
+int slate_amd_generate_svd(slate_amd_context ctx, int sigma_dist, double cond, int64_t seed, int sym,
+                           int64_t n, double* dA, int64_t lda)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(n >= 0 && lda >= std::max<int64_t>(1, n), -3, "bad n/lda");
+        SA_REQUIRE(sigma_dist >= 1 && sigma_dist <= 4, -2, "unknown sigma distribution");
+        SA_REQUIRE(cond >= 1.0, -2, "cond must be >= 1");
+        if (n == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::generate_svd(c, sigma_dist, cond, seed, sym != 0, n, dA, lda);
+        SA_HIP(hipStreamSynchronize(c.s0));
         return 0;
     }));
 }

@@ -36,6 +36,13 @@ void gemm_bf16s(int ns, int64_t M, int64_t N, int64_t Kp, const __bf16* A, int64
                 const __bf16* B, int64_t ldb, int64_t planeB, float* C, int64_t ldc, float alpha,
                 hipStream_t s, int role = 0, int uplo = 0);
 
+// Spectrum-controlled test matrices (matgen.hip): sigma_diag writes sigma_i
+// of distribution dist (1 arith, 2 geo, 3 cluster0, 4 cluster1) on the
+// diagonal; larft_from_gram builds the upper-triangular T of
+// H_1 ... H_b = I - Y T Y^T from G = Y^T Y (both b x b row-major).
+void sigma_diag(int dist, double cond, int64_t n, double* A, int64_t lda, hipStream_t s);
+void larft_from_gram(const double* G, int64_t b, double* T, hipStream_t s);
+
 // Workspace shared by the cross-workgroup panel kernel.
 struct PanelWork {
     void* slots;                 // tagged candidate words exchanged between workgroups

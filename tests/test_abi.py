@@ -10,10 +10,10 @@ import slate_amd
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "slate_amd.h")).read()
+def declared_symbols(header="slate_amd.h", pattern=r"\b(slate_amd_\w+)\s*\("):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(slate_amd_\w+)\s*\(", text)))
+    return sorted(set(re.findall(pattern, text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -23,6 +23,42 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in decl if not hasattr(lib, s)]
     assert missing == []
     assert sorted(slate_amd.C_SYMBOLS) == decl
+
+
+def test_library_exports_every_lapack_symbol():
+    lib = ctypes.CDLL(slate_amd.LIB_PATH)
+    decl = declared_symbols("slate_amd_lapack.h", r"\b(slate_\w+_)\s*\(")
+    assert len(decl) >= 11
+    missing = [s for s in decl if not hasattr(lib, s)]
+    assert missing == []
+
+
+def test_lapack_argument_errors_need_no_gpu():
+    """LAPACK-style argument checks run before any device call (info = -i)."""
+    lib = ctypes.CDLL(slate_amd.LIB_PATH)
+    i = ctypes.c_int
+    info = i(0)
+    n, nrhs, lda, ldb = i(4), i(1), i(2), i(4)
+    lib.slate_dgesv_(ctypes.byref(n), ctypes.byref(nrhs), None, ctypes.byref(lda), None, None,
+                     ctypes.byref(ldb), ctypes.byref(info))
+    assert info.value == -4
+    lda = i(4)
+    ldb = i(1)
+    lib.slate_dgesv_(ctypes.byref(n), ctypes.byref(nrhs), None, ctypes.byref(lda), None, None,
+                     ctypes.byref(ldb), ctypes.byref(info))
+    assert info.value == -7
+    uplo = ctypes.c_char(b"X")
+    lib.slate_dpotrf_(ctypes.byref(uplo), ctypes.byref(n), None, ctypes.byref(lda), ctypes.byref(info))
+    assert info.value == -1
+    trans = ctypes.c_char(b"Q")
+    lib.slate_dgetrs_(ctypes.byref(trans), ctypes.byref(n), ctypes.byref(nrhs), None, ctypes.byref(lda), None,
+                      None, ctypes.byref(lda), ctypes.byref(info))
+    assert info.value == -1
+    neg = i(-1)
+    it = i(0)
+    lib.slate_dsgesv_(ctypes.byref(neg), ctypes.byref(nrhs), None, ctypes.byref(lda), None, None,
+                      ctypes.byref(lda), None, ctypes.byref(lda), None, None, ctypes.byref(it), ctypes.byref(info))
+    assert info.value == -1
 
 
 def test_options_defaults_mirror_slate():

@@ -242,3 +242,49 @@ def test_scaled_residual_definition():
     B = A @ X + np.array([[0.0], [8.0 * EPS * 2 * 4 * 1]])
     # ||R||_1 = 64 eps, n = 2, ||A||_1 = 4, ||X||_1 = 2  ->  4 eps
     assert abs(oracle.scaled_residual(A, X, B) - 4.0) < 1e-9
+
+
+# ---- spectrum-controlled generator -----------------------------------------
+
+def test_sigma_closed_forms():
+    n, cond = 9, 1e6
+    g = oracle.sigma(2, cond, n)
+    assert g[0] == 1.0 and abs(g[-1] - 1 / cond) <= 1e-20
+    np.testing.assert_allclose(g[1:] / g[:-1], cond ** (-1.0 / (n - 1)), rtol=1e-14)
+    a = oracle.sigma(1, cond, n)
+    assert a[0] == 1.0 and abs(a[-1] - 1 / cond) < 1e-16
+    np.testing.assert_allclose(np.diff(a), -(1 - 1 / cond) / (n - 1), rtol=1e-12)
+    c0 = oracle.sigma(3, cond, n)
+    assert c0[0] == 1 and np.all(c0[1:] == 1 / cond)
+    c1 = oracle.sigma(4, cond, n)
+    assert np.all(c1[:-1] == 1) and c1[-1] == 1 / cond
+
+
+def test_reflector_product_invariants():
+    n = 40
+    Q = oracle.reflector_product(5, n)
+    np.testing.assert_allclose(Q.T @ Q, np.eye(n), atol=1e-13)
+    # each reflector has determinant -1: a dropped or doubled one flips the sign
+    assert np.sign(np.linalg.det(Q)) == (-1) ** n
+    # the last reflector applied to its own vector: H_n v_n = -v_n, so
+    # Q v_n = -(H_1 ... H_{n-1}) v_n; check with the product of the first n-1
+    V = oracle.randn(5, n, n)
+    Qm = np.eye(n)
+    for i in range(n - 2, -1, -1):
+        v = V[:, i]
+        Qm -= np.outer(v, (2.0 / (v @ v)) * (v @ Qm))
+    np.testing.assert_allclose(Q @ V[:, -1], -(Qm @ V[:, -1]), atol=1e-12)
+
+
+@pytest.mark.parametrize("sym", [False, True])
+@pytest.mark.parametrize("dist,cond", [(2, 1e6), (1, 1e2), (3, 1e10)])
+def test_generate_svd_spectrum(sym, dist, cond):
+    n = 60
+    A = oracle.generate_svd(dist, cond, 3, n, sym=sym)
+    sv = np.linalg.svd(A, compute_uv=False)
+    np.testing.assert_allclose(sv, np.sort(oracle.sigma(dist, cond, n))[::-1], rtol=0, atol=1e-14)
+    if sym:
+        np.testing.assert_allclose(A, A.T, atol=1e-15)
+        assert np.all(np.linalg.eigvalsh(A) > 0)
+    else:
+        assert not np.allclose(A, A.T)
