@@ -172,6 +172,41 @@ int64_t slate_amd_dgecondest(slate_amd_context ctx, int norm, int64_t n, const d
 int64_t slate_amd_dpocondest(slate_amd_context ctx, int uplo, int64_t n, const double* dL, int64_t lda,
                              double anorm, double* rcond);
 
+/* ---- tuning ---------------------------------------------------------------
+ * Block size nb per kind of factorization, used when options.block_size is 0
+ * (SLATE takes nb from the matrix tiling; the autotuner of
+ * slate_amd/autotune.py measures and installs this table).  Entry i applies
+ * to n <= n_upper[i] (ascending); count = 0 restores the built-in choice. */
+enum { SLATE_AMD_TUNE_LU_FP64 = 0, SLATE_AMD_TUNE_LU_FP32 = 1, SLATE_AMD_TUNE_LU_BF16 = 2,
+       SLATE_AMD_TUNE_CHOL = 3 };
+int slate_amd_tune_set_nb(slate_amd_context ctx, int kind, int count, const int64_t* n_upper,
+                          const int64_t* nb);
+
+/* ---- learned solver selection (no counterpart in the reference) ---------
+ * For callers that want the factorization precision / refinement scheme
+ * picked per system.  Features: 24 floats (log2 n, log10 of the infinity-norm
+ * condition estimate from the float factor, per-tile max |a| spread, a
+ * 16-bin exponent histogram, a constant).  The MLP (3 layers, 64 hidden,
+ * FP32; slate_amd_mlp_num_params() floats) maps them to a choice:
+ * 0 classic IR, 1 GMRES-IR, 2 plain FP64.  Training runs on the GPU with
+ * slate_amd_mlp_train_step (softmax cross-entropy, Adam).  Device pointers
+ * except where noted. */
+int slate_amd_mlp_num_params(void);
+int64_t slate_amd_selector_features(slate_amd_context ctx, int64_t n, const double* dA, int64_t lda,
+                                    const slate_amd_options* opts, float* features /* host, 24 */);
+int slate_amd_mlp_forward(slate_amd_context ctx, const float* dparams, const float* dX, int batch,
+                          float* dprobs);
+/* One step on a minibatch (batch <= 4096): dgrad <- d(mean loss)/dparams,
+ * then Adam (step >= 1) when lr > 0; *loss (host) <- mean loss. */
+int slate_amd_mlp_train_step(slate_amd_context ctx, float* dparams, float* dgrad, float* dm, float* dv,
+                             const float* dX, const int* dy, int batch, float lr, int step, float* loss);
+/* gesv_mixed with the selector: params (host) the trained MLP, or NULL for
+ * SLATE's default (classic IR); *choice the scheme used, *iter as in dsgesv,
+ * features (host, 24, may be NULL) the inputs it saw.  nrhs = 1. */
+int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int64_t lda, int64_t* ipiv,
+                              const double* dB, double* dX, const float* params, int* iter, int* choice,
+                              float* features, const slate_amd_options* opts);
+
 /* Test matrix with a prescribed spectrum: A = U diag(sigma) V^T, U and V
  * products of n Householder reflectors whose vectors are the columns of
  * matgen randn matrices (seed + 1 for U, seed for V); sym != 0 gives

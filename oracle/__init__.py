@@ -33,6 +33,8 @@ dsgesv_gmres      src/gesv_mixed_gmres.cc:111-410             same, plus Arnoldi
 rotg              blas::rotg (reference BLAS drotg)           closed form: c*a+s*b=r, -s*a+c*b=0
 sigma             matgen/generate_sigma.hh:61-111             closed forms (endpoints 1 and 1/cond,
                                                               geometric ratio, arithmetic step)
+mlp_*             (this library's selector network; no        finite differences of the loss
+                  reference counterpart)                      (tests/test_oracle.py), softmax sums
 generate_svd      (this library's generator; the reference's  invariants: singular values = sigma,
                   svd kind, generate_type_svd.hh:90-157,      orthogonality, det = (-1)^n of each
                   uses QR of a Gaussian instead)              n-reflector product, symmetry for sym
@@ -287,6 +289,74 @@ def generate_svd(sigma_dist, cond, seed, n, sym=False):
     V = reflector_product(seed, n)
     U = V if sym else reflector_product(seed + 1, n)
     return (U * sg) @ V.T
+
+
+# ---- selector MLP (float64 restatement of slate_amd/csrc/mlp.hip) ----------
+MLP_F, MLP_H, MLP_K = 24, 64, 3
+
+
+def mlp_unpack(P):
+    """Views of the parameter array: W1 (F x H), b1, W2 (H x H), b2, W3 (H x K), b3."""
+    F, H, K = MLP_F, MLP_H, MLP_K
+    o = 0
+    out = []
+    for shape in ((F, H), (H,), (H, H), (H,), (H, K), (K,)):
+        size = int(np.prod(shape))
+        out.append(P[o:o + size].reshape(shape))
+        o += size
+    assert o == P.size
+    return out
+
+
+def mlp_num_params():
+    return MLP_F * MLP_H + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H * MLP_K + MLP_K
+
+
+def mlp_forward(P, X):
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(np.asarray(P, dtype=np.float64))
+    h1 = np.maximum(X @ W1 + b1, 0)
+    h2 = np.maximum(h1 @ W2 + b2, 0)
+    z = h2 @ W3 + b3
+    z = z - z.max(axis=1, keepdims=True)
+    p = np.exp(z)
+    return p / p.sum(axis=1, keepdims=True)
+
+
+def mlp_loss_grad(P, X, y):
+    """Mean softmax cross-entropy and its gradient w.r.t. the flat parameters."""
+    P = np.asarray(P, dtype=np.float64)
+    X = np.asarray(X, dtype=np.float64)
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(P)
+    B = X.shape[0]
+    a1 = X @ W1 + b1
+    h1 = np.maximum(a1, 0)
+    a2 = h1 @ W2 + b2
+    h2 = np.maximum(a2, 0)
+    z = h2 @ W3 + b3
+    z = z - z.max(axis=1, keepdims=True)
+    p = np.exp(z)
+    p /= p.sum(axis=1, keepdims=True)
+    loss = -np.mean(np.log(np.maximum(p[np.arange(B), y], 1e-300)))
+    d3 = p.copy()
+    d3[np.arange(B), y] -= 1.0
+    d3 /= B
+    gW3 = h2.T @ d3
+    gb3 = d3.sum(axis=0)
+    d2 = (d3 @ W3.T) * (a2 > 0)
+    gW2 = h1.T @ d2
+    gb2 = d2.sum(axis=0)
+    d1 = (d2 @ W2.T) * (a1 > 0)
+    gW1 = X.T @ d1
+    gb1 = d1.sum(axis=0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+    return loss, g
+
+
+def adam_step(P, g, m, v, lr, t, b1=0.9, b2=0.999, eps=1e-8):
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    P = P - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+    return P, m, v
 
 
 def norm_inf(A):

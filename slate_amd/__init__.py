@@ -35,7 +35,9 @@ C_SYMBOLS = [
     "slate_amd_dgesv", "slate_amd_dpotrf", "slate_amd_spotrf", "slate_amd_dpotrs",
     "slate_amd_dposv", "slate_amd_dsgesv", "slate_amd_dsgesv_gmres", "slate_amd_dsposv",
     "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate", "slate_amd_generate_svd",
-    "slate_amd_dgecondest", "slate_amd_dpocondest",
+    "slate_amd_dgecondest", "slate_amd_dpocondest", "slate_amd_mlp_num_params",
+    "slate_amd_selector_features", "slate_amd_mlp_forward", "slate_amd_mlp_train_step",
+    "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb",
     "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_profile_enable",
     "slate_amd_profile_read",
 ]
@@ -106,6 +108,13 @@ def lib():
             "slate_amd_generate_svd": (i32, [vp, i32, dbl, i64, i32, i64, vp, i64]),
             "slate_amd_dgecondest": (i64, [vp, i32, i64, vp, i64, pi64, dbl, ctypes.POINTER(dbl)]),
             "slate_amd_dpocondest": (i64, [vp, i32, i64, vp, i64, dbl, ctypes.POINTER(dbl)]),
+            "slate_amd_mlp_num_params": (i32, []),
+            "slate_amd_tune_set_nb": (i32, [vp, i32, i32, pi64, pi64]),
+            "slate_amd_selector_features": (i64, [vp, i64, vp, i64, po, vp]),
+            "slate_amd_mlp_forward": (i32, [vp, vp, vp, i32, vp]),
+            "slate_amd_mlp_train_step": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_float, i32,
+                                               ctypes.POINTER(ctypes.c_float)]),
+            "slate_amd_dsgesv_auto": (i64, [vp, i64, vp, i64, pi64, vp, vp, vp, pint, pint, vp, po]),
             "slate_amd_profile_enable": (i32, [vp, i32]),
             "slate_amd_profile_read": (i32, [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl), pi64]),
             "slate_amd_dgemm": (i32, [vp, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
@@ -130,6 +139,8 @@ def context(device=0):
         if rc != 0:
             raise SlateError("slate_amd_init: " + last_error())
         _CTX[device] = h
+        from . import autotune
+        autotune.load(device=device)       # measured block sizes, when tuned on this machine
     return _CTX[device]
 
 
@@ -330,6 +341,66 @@ def norm_inf(A, sym=False):
     out = ctypes.c_double(0)
     _check(lib().slate_amd_dnorm_inf(context(dev), int(sym), A.shape[0], pa, lda, ctypes.byref(out)))
     return out.value
+
+
+# ---- learned selection ------------------------------------------------------
+SELECTOR_FEATURES = 24
+CHOICES = ("classic IR", "GMRES-IR", "FP64")
+
+
+def mlp_num_params():
+    return int(lib().slate_amd_mlp_num_params())
+
+
+def selector_features(A, opts=None):
+    """The selector's 24 features of a column-major float64 CUDA matrix
+    (factors A in float internally; A itself is not modified)."""
+    pa, lda, dev = _mat(A, ("float64",))
+    f = np.zeros(SELECTOR_FEATURES, dtype=np.float32)
+    info = _check(lib().slate_amd_selector_features(context(dev), A.shape[0], pa, lda, _opts(opts),
+                                                    f.ctypes.data_as(ctypes.c_void_p)))
+    return f, info
+
+
+def mlp_forward(params, X):
+    """Class probabilities (B x 3) for features X (B x 24), both CUDA float32."""
+    import torch
+    B = X.shape[0]
+    probs = torch.empty(B, len(CHOICES), dtype=torch.float32, device=X.device)
+    _check(lib().slate_amd_mlp_forward(context(X.device.index or 0), params.data_ptr(), X.data_ptr(), B,
+                                       probs.data_ptr()))
+    return probs
+
+
+def mlp_train_step(params, grad, m, v, X, y, lr, step):
+    """One minibatch step on the GPU; returns the mean loss.  lr = 0 only
+    computes the gradient into grad."""
+    loss = ctypes.c_float(0)
+    _check(lib().slate_amd_mlp_train_step(context(X.device.index or 0), params.data_ptr(), grad.data_ptr(),
+                                          m.data_ptr(), v.data_ptr(), X.data_ptr(), y.data_ptr(),
+                                          X.shape[0], float(lr), int(step), ctypes.byref(loss)))
+    return loss.value
+
+
+def gesv_auto(A, B, X, params=None, opts=None):
+    """gesv_mixed with the learned selector (params: numpy float32 MLP
+    parameters, or None for the default classic IR).  Returns
+    (info, iter, choice, ipiv, features)."""
+    pa, lda, dev = _mat(A, ("float64",))
+    pb, _, _ = _mat(B, ("float64",))
+    px, _, _ = _mat(X, ("float64",))
+    n = A.shape[0]
+    ipiv = _ipiv_out(n)
+    it, ch = ctypes.c_int(0), ctypes.c_int(-1)
+    feat = np.zeros(SELECTOR_FEATURES, dtype=np.float32)
+    pp = None
+    if params is not None:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        pp = params.ctypes.data_as(ctypes.c_void_p)
+    info = _check(lib().slate_amd_dsgesv_auto(context(dev), n, pa, lda, _p64(ipiv), pb, px, pp,
+                                              ctypes.byref(it), ctypes.byref(ch),
+                                              feat.ctypes.data_as(ctypes.c_void_p), _opts(opts)))
+    return info, it.value, ch.value, ipiv[:n], feat
 
 
 def gecondest(LU, ipiv, anorm, norm="1"):

@@ -1,0 +1,103 @@
+"""Block-size autotuner for the factorizations.
+
+SLATE takes the tile size nb from the matrix (the tester sweeps it by hand,
+test/test_gesv.cc:165-177).  Here nb is chosen per kind of factorization and
+problem size by measurement: for each n of a grid and each candidate nb the
+factorization is timed on a matgen matrix, and the fastest nb is installed in
+the library (slate_amd_tune_set_nb) for every n up to the next grid point.
+Results persist as JSON (tuning.json beside this file by default) and are
+applied to each new context by slate_amd.context() when the file exists.
+
+Usage (on a GPU):  python -m slate_amd.autotune --sizes 4096,8192,16384,32768
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_PATH = os.path.join(HERE, "tuning.json")
+KINDS = {"lu_fp64": 0, "lu_fp32": 1, "lu_bf16": 2, "chol": 3}
+CANDIDATES = (128, 256, 384, 512, 768, 1024)
+
+
+def install(table, device=0):
+    """table: {kind: [[n_upper, nb], ...]} -> the library context of `device`."""
+    import slate_amd as S
+    for kind, rows in table.items():
+        rows = sorted(rows)
+        nu = np.array([r[0] for r in rows], dtype=np.int64)
+        nb = np.array([r[1] for r in rows], dtype=np.int64)
+        S._check(S.lib().slate_amd_tune_set_nb(S.context(device), KINDS[kind], len(rows),
+                                               nu.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                               nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+
+
+def load(path=DEFAULT_PATH, device=0):
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)["table"]
+    install(table, device)
+    return table
+
+
+def _time_one(kind, n, nb, reps=2):
+    import torch
+    import slate_amd as S
+    best = float("inf")
+    for _ in range(reps):
+        A = S.empty_colmajor(n, n, torch.float64)
+        S.generate(A, S.DIST_RANDS, 0)
+        if kind == "chol":
+            A = S.colmajor(A.t() @ A / n + torch.eye(n, dtype=torch.float64, device=A.device))
+            S.potrf(A, opts=S.Options(block_size=nb))
+        elif kind == "lu_fp64":
+            S.getrf(A, S.Options(block_size=nb))
+        else:
+            A = S.colmajor(A.float())
+            prec = S.PREC_BF16X3 if kind == "lu_bf16" else S.PREC_FP32
+            S.getrf(A, S.Options(block_size=nb, factor_precision=prec))
+        best = min(best, S.last_time_ms())
+        del A
+    return best
+
+
+def tune(sizes, kinds=("lu_fp64", "lu_fp32", "lu_bf16", "chol"), log=None):
+    """Measure every (kind, n, nb); returns the table and the raw timings."""
+    table, raw = {}, {}
+    for kind in kinds:
+        rows = []
+        for n in sorted(sizes):
+            cands = [nb for nb in CANDIDATES if nb <= max(128, n // 4)]
+            times = {nb: _time_one(kind, n, nb) for nb in cands}
+            best = min(times, key=times.get)
+            raw["%s/%d" % (kind, n)] = times
+            rows.append([n, best])
+            if log:
+                log("autotune %s n=%d: %s -> nb %d" % (kind, n, {k: round(v, 2) for k, v in times.items()}, best))
+        # each entry covers sizes up to the next grid point; the last one all larger n
+        table[kind] = [[rows[i + 1][0] - 1 if i + 1 < len(rows) else 1 << 40, rows[i][1]]
+                       for i in range(len(rows))]
+    return table, raw
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--sizes", default="2048,4096,8192,16384,32768")
+    ap.add_argument("--kinds", default="lu_fp64,lu_fp32,lu_bf16,chol")
+    ap.add_argument("--out", default=DEFAULT_PATH)
+    args = ap.parse_args(argv)
+    sizes = [int(x) for x in args.sizes.split(",")]
+    table, raw = tune(sizes, args.kinds.split(","), log=lambda m: print(m, file=sys.stderr, flush=True))
+    with open(args.out, "w") as f:
+        json.dump({"table": table, "timings_ms": raw}, f, indent=1)
+    install(table)
+    print(json.dumps({"table": table}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -44,6 +44,8 @@ struct Context {
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
+    // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
+    std::vector<std::pair<int64_t, int64_t>> nb_table[4];
     // trailing-update GEMM timing (enabled by slate_amd_profile_enable)
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
@@ -120,9 +122,12 @@ static Opts read_opts(const slate_amd_options* o)
     return r;
 }
 
-static int64_t choose_nb(int64_t n, int64_t requested)
+// kind: SLATE_AMD_TUNE_* (LU FP64, LU float, LU split-BF16, Cholesky).
+static int64_t choose_nb(const Context& c, int kind, int64_t n, int64_t requested)
 {
     if (requested > 0) return requested;
+    for (const auto& e : c.nb_table[kind])
+        if (n <= e.first) return e.second;
     if (n <= 2048) return 128;
     if (n <= 16384) return 256;
     return 512;
@@ -291,7 +296,9 @@ template <class T>
 static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int64_t* ipiv,
                      int64_t* info)
 {
-    const int64_t nb = choose_nb(n, o.nb);
+    const int kind = sizeof(T) == 8 ? SLATE_AMD_TUNE_LU_FP64
+                   : (split_terms(o.prec) > 0 ? SLATE_AMD_TUNE_LU_BF16 : SLATE_AMD_TUNE_LU_FP32);
+    const int64_t nb = choose_nb(c, kind, n, o.nb);
     int64_t leaf_w = panel_leaf_width(int(sizeof(T)));
     if (o.ib > 0 && o.ib < leaf_w) leaf_w = o.ib;
     prepare_panel_work(c, n, leaf_w);
@@ -466,7 +473,7 @@ static void potrf_diag_rec(Context& c, T* M, int64_t ld, int64_t nb, int64_t col
 template <class T>
 static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int64_t* info)
 {
-    const int64_t nb = choose_nb(n, o.nb);
+    const int64_t nb = choose_nb(c, SLATE_AMD_TUNE_CHOL, n, o.nb);
     hipStream_t s = c.s0;
     // Float factor with split-BF16 updates: U12 (kb x r) is split and
     // transposed once per step into r x Kp planes, which are both operands
@@ -906,21 +913,23 @@ static int64_t factor_lo(Context& c, MixedState& st, const double* A, int64_t ld
 template <class T> static T* as_mut(const T* p) { return const_cast<T*>(p); }
 
 // gesv_mixed / posv_mixed: src/gesv_mixed.cc:106-290, src/posv_mixed.cc:105-292.
+// pre: a state whose float factor is already computed (info in pre_info),
+// or null to set up and factor here.
 static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, double* A, int64_t lda,
                         int64_t* ipiv_host, const double* B, int64_t ldb, double* X, int64_t ldx,
-                        int* iter, const Opts& o)
+                        int* iter, const Opts& o, const MixedState* pre = nullptr, int64_t pre_info = 0)
 {
     const double eps = std::numeric_limits<double>::epsilon();
     const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
     const int64_t itermax = o.itermax;
     *iter = 0;
     SA_REQUIRE(!(sym && uplo == SLATE_AMD_UPPER), -2, "posv_mixed: Upper storage not supported");
-    MixedState st = mixed_setup(c, n, nrhs);
+    MixedState st = pre ? *pre : mixed_setup(c, n, nrhs);
     const double Anorm = norm_host(c, A, n, lda, sym, st.work, st.work_elems, st.norms);
     const double cte = Anorm * tol;
     std::vector<double> nx, nr;
     bool converged = false;
-    int64_t info = factor_lo(c, st, A, lda, n, sym, o);
+    int64_t info = pre ? pre_info : factor_lo(c, st, A, lda, n, sym, o);
     if (info != 0) {
         *iter = -3;
     }
@@ -1007,17 +1016,17 @@ static inline void rot(double* x, double* y, double cc, double ss)
 // matrix and Givens rotations are on the host like the reference (:168-177).
 static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, int64_t lda,
                            int64_t* ipiv_host, const double* B, double* X, int* iter, const Opts& o,
-                           int64_t nb_hint)
+                           int64_t nb_hint, const MixedState* pre = nullptr, int64_t pre_info = 0)
 {
     const double eps = std::numeric_limits<double>::epsilon();
     const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
     const int64_t itermax = o.itermax;
-    const int64_t nb = choose_nb(n, nb_hint);
+    const int64_t nb = choose_nb(c, sym ? SLATE_AMD_TUNE_CHOL : SLATE_AMD_TUNE_LU_FP32, n, nb_hint);
     const int64_t restart = std::min<int64_t>(std::min<int64_t>(30, itermax), std::min(nb, n) - 1);
     SA_REQUIRE(restart >= 1, -2, "gmres: restart length < 1");
     SA_REQUIRE(!(sym && uplo == SLATE_AMD_UPPER), -2, "posv_mixed_gmres: Upper storage not supported");
     *iter = 0;
-    MixedState st = mixed_setup(c, n, 1);
+    MixedState st = pre ? *pre : mixed_setup(c, n, 1);
     const int64_t R1 = restart + 1;
     c.basis.ensure(sizeof(double) * size_t(n) * size_t(2 * R1) + sizeof(double) * 256);
     double* V = c.basis.as<double>();
@@ -1028,7 +1037,7 @@ static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, 
     std::vector<double> H(R1 * R1, 0.0), S(R1, 0.0), ga(restart), gb(restart), hz(R1), zz(R1);
     std::vector<double> nx(1), nr(1);
     bool converged = false;
-    int64_t info = factor_lo(c, st, A, lda, n, sym, o);
+    int64_t info = pre ? pre_info : factor_lo(c, st, A, lda, n, sym, o);
     if (info != 0) {
         *iter = -3;
     }
@@ -1122,6 +1131,172 @@ static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, 
     return info;
 }
 
+// Estimate of ||A^{-1}||_1 by Hager's method with Higham's refinements, the
+// steps of LAPACK dlacn2 that SLATE's internal::norm1est restates
+// (src/internal/internal_norm1est.cc): x = 1/n, then alternately
+// y = A^{-1} x and z = A^{-T} sign(y), moving x to the unit vector of the
+// largest |z_j| until the sign vector repeats, the estimate stops growing or
+// five iterations pass; finally the alternating vector
+// x_i = (-1)^i (1 + i/(n-1)) guards against unlucky starts.  The vector logic
+// runs on the host (n doubles per step); solve(x, t) overwrites the device
+// vector x with A^{-1} x (t = 0) or A^{-T} x (t = 1).
+template <class Solve>
+static double norm1est_inv(int64_t n, double* xd, Solve solve)
+{
+    std::vector<double> x(size_t(n), 1.0 / double(n));
+    std::vector<int> isgn(size_t(n), 0);
+    auto run = [&](int t) {
+        SA_HIP(hipMemcpy(xd, x.data(), sizeof(double) * size_t(n), hipMemcpyHostToDevice));
+        solve(xd, t);
+        SA_HIP(hipMemcpy(x.data(), xd, sizeof(double) * size_t(n), hipMemcpyDeviceToHost));
+    };
+    auto asum = [&]() { double s = 0; for (double e : x) s += std::fabs(e); return s; };
+    auto imax = [&]() {
+        int64_t j = 0;
+        for (int64_t i = 1; i < n; ++i) if (std::fabs(x[size_t(i)]) > std::fabs(x[size_t(j)])) j = i;
+        return j;
+    };
+    run(0);
+    if (n == 1) return std::fabs(x[0]);
+    double est = asum();
+    for (int64_t i = 0; i < n; ++i) { isgn[size_t(i)] = x[size_t(i)] >= 0 ? 1 : -1; x[size_t(i)] = isgn[size_t(i)]; }
+    run(1);
+    int64_t j = imax();
+    for (int iter = 2;; ++iter) {
+        std::fill(x.begin(), x.end(), 0.0);
+        x[size_t(j)] = 1.0;
+        run(0);
+        const double estold = est;
+        est = asum();
+        bool repeated = true;
+        for (int64_t i = 0; i < n && repeated; ++i) repeated = (x[size_t(i)] >= 0 ? 1 : -1) == isgn[size_t(i)];
+        if (repeated || est <= estold) break;
+        for (int64_t i = 0; i < n; ++i) { isgn[size_t(i)] = x[size_t(i)] >= 0 ? 1 : -1; x[size_t(i)] = isgn[size_t(i)]; }
+        run(1);
+        const int64_t jlast = j;
+        j = imax();
+        if (!(x[size_t(jlast)] != std::fabs(x[size_t(j)]) && iter < 5)) break;
+    }
+    double alt = 1.0;
+    for (int64_t i = 0; i < n; ++i) { x[size_t(i)] = alt * (1.0 + double(i) / double(n - 1)); alt = -alt; }
+    run(0);
+    const double temp = 2.0 * asum() / double(3 * n);
+    return std::max(est, temp);
+}
+
+//------------------------------------------------------------------------------
+// Learned selection (no counterpart in the reference).  Features of a general
+// FP64 matrix, computed on the device: log2 n, the 1-norm condition estimate
+// from the float factor, the spread of the per-tile max |a_ij| and a 16-bin
+// histogram of binary exponents relative to the largest.  The float factor
+// is left in st for the solve that follows.
+static constexpr int SEL_F = 24;
+
+static void selector_features(Context& c, MixedState& st, int64_t n, const double* A, int64_t lda,
+                              const Opts& o, int64_t* info, float* f)
+{
+    const int64_t t = ceildiv(n, 256);
+    c.flags.ensure(sizeof(double) * size_t(t * t) + sizeof(unsigned long long) * 16 + 256);
+    double* tmax = c.flags.as<double>();
+    unsigned long long* hist = reinterpret_cast<unsigned long long*>(tmax + t * t);
+    feature_tiles(A, lda, n, tmax, c.s0);
+    std::vector<double> tm(size_t(t * t));
+    SA_HIP(hipMemcpyAsync(tm.data(), tmax, sizeof(double) * tm.size(), hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+    double mx = 0, mn = std::numeric_limits<double>::infinity(), s1 = 0, s2 = 0;
+    int64_t cnt = 0;
+    for (double v : tm) {
+        mx = std::max(mx, v);
+        if (v > 0) { mn = std::min(mn, v); double l = std::log2(v); s1 += l; s2 += l * l; ++cnt; }
+    }
+    int emax = 0;
+    std::frexp(mx > 0 ? mx : 1.0, &emax);
+    zero_u32(reinterpret_cast<unsigned*>(hist), 32, c.s0);
+    feature_hist(A, lda, n, emax, hist, c.s0);
+    unsigned long long h[16];
+    SA_HIP(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, c.s0));
+    *info = factor_lo(c, st, A, lda, n, 0, o);
+    double rcond = 0;
+    if (*info == 0) {
+        const double anorm1 = [&] {
+            // ||A||_1 from the device (column sums): norm_inf of A^T is the
+            // row-sum norm of the transpose; use the residual workspace
+            return norm_host(c, A, n, lda, 0, st.work, st.work_elems, st.norms);
+        }();
+        // the infinity norm pairs with ||A^{-1}||_inf: solves with A^T and A swapped
+        const double ainv = norm1est_inv(n, st.R, [&](double* x, int tr) {
+            if (tr == 0) {
+                // x := A^{-T} x with the float factor (infinity-norm estimate)
+                copy_convert<double, float>(x, n, st.xlo, n, n, 1, c.s0);
+                getrs_vec<float>(c, st.Alo, n, 0, n, st.perm, st.xlo, st.xtmp, 1, c.s0);
+                copy_convert<float, double>(st.xlo, n, x, n, n, 1, c.s0);
+            }
+            else {
+                lo_solve_vec(c, st, 0, n, x, x, false);
+            }
+            SA_HIP(hipStreamSynchronize(c.s0));
+        });
+        rcond = (ainv > 0 && anorm1 > 0) ? 1.0 / (ainv * anorm1) : 0.0;
+    }
+    SA_HIP(hipStreamSynchronize(c.s0));
+    const double total = double(n) * double(n);
+    for (int k = 0; k < SEL_F; ++k) f[k] = 0.f;
+    f[0] = float(std::log2(double(n)) / 16.0);
+    f[1] = float((rcond > 0 ? std::log10(1.0 / rcond) : 20.0) / 16.0);
+    f[2] = float((cnt > 0 && mn < std::numeric_limits<double>::infinity()) ? std::log2(mx / mn) / 64.0 : 1.0);
+    const double mean = cnt ? s1 / double(cnt) : 0.0;
+    f[3] = float(cnt ? std::sqrt(std::max(0.0, s2 / double(cnt) - mean * mean)) / 16.0 : 0.0);
+    for (int b = 0; b < 16; ++b) f[4 + b] = float(double(h[b]) / total);
+    f[20] = 1.0f;                                   // bias-like constant input
+}
+
+// Solve with the float factor chosen by the selector (src/gesv_mixed.cc and
+// src/gesv_mixed_gmres.cc flows): choice 0 classic IR, 1 GMRES-IR, 2 plain
+// FP64 getrf + getrs.  params: the MLP parameters (host, mlp_num_params()
+// floats) or null for SLATE's default (classic IR).  A refinement that misses
+// the tolerance falls back to FP64 as in the reference.
+static int64_t mixed_auto(Context& c, int64_t n, double* A, int64_t lda, int64_t* ipiv_host, const double* B,
+                          double* X, int* iter, int* choice, const float* params, float* feat_out, const Opts& o)
+{
+    MixedState st = mixed_setup(c, n, 1);
+    int64_t finfo = 0;
+    float f[SEL_F];
+    selector_features(c, st, n, A, lda, o, &finfo, f);
+    if (feat_out) std::copy(f, f + SEL_F, feat_out);
+    int ch = 0;
+    if (params && finfo == 0) {
+        const int np = mlp_num_params();
+        c.tinv.ensure(sizeof(float) * size_t(np + SEL_F + 16));
+        float* dp = c.tinv.as<float>();
+        float* dx = dp + np;
+        float* dprob = dx + SEL_F;
+        SA_HIP(hipMemcpyAsync(dp, params, sizeof(float) * np, hipMemcpyHostToDevice, c.s0));
+        SA_HIP(hipMemcpyAsync(dx, f, sizeof(float) * SEL_F, hipMemcpyHostToDevice, c.s0));
+        mlp_forward(dp, dx, 1, dprob, c.s0);
+        float pr[3];
+        SA_HIP(hipMemcpyAsync(pr, dprob, sizeof(pr), hipMemcpyDeviceToHost, c.s0));
+        SA_HIP(hipStreamSynchronize(c.s0));
+        ch = 0;
+        for (int k = 1; k < 3; ++k) if (pr[k] > pr[ch]) ch = k;
+    }
+    *choice = ch;
+    if (ch == 2) {
+        *iter = 0;
+        c.ipiv.ensure(sizeof(int64_t) * n);
+        int64_t* dip = c.ipiv.as<int64_t>();
+        int64_t info = getrf_cm<double>(c, A, n, lda, o, dip);
+        if (info == 0) {
+            copy_convert<double, double>(B, n, X, n, n, 1, c.s0);
+            getrs_cm<double>(c, 0, A, lda, n, dip, X, n, 1);
+        }
+        ipiv_to_host(c, dip, n, ipiv_host);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return info;
+    }
+    if (ch == 1) return mixed_gmres(c, 0, SLATE_AMD_LOWER, n, A, lda, ipiv_host, B, X, iter, o, o.nb, &st, finfo);
+    return mixed_ir(c, 0, SLATE_AMD_LOWER, n, 1, A, lda, ipiv_host, B, n, X, n, iter, o, &st, finfo);
+}
+
 } // namespace sa
 
 //==============================================================================
@@ -1203,58 +1378,6 @@ static int gemm_bf16_entry(slate_amd_context ctx, int terms, int64_t m, int64_t 
 
 namespace sa {
 
-// Estimate of ||A^{-1}||_1 by Hager's method with Higham's refinements, the
-// steps of LAPACK dlacn2 that SLATE's internal::norm1est restates
-// (src/internal/internal_norm1est.cc): x = 1/n, then alternately
-// y = A^{-1} x and z = A^{-T} sign(y), moving x to the unit vector of the
-// largest |z_j| until the sign vector repeats, the estimate stops growing or
-// five iterations pass; finally the alternating vector
-// x_i = (-1)^i (1 + i/(n-1)) guards against unlucky starts.  The vector logic
-// runs on the host (n doubles per step); solve(x, t) overwrites the device
-// vector x with A^{-1} x (t = 0) or A^{-T} x (t = 1).
-template <class Solve>
-static double norm1est_inv(int64_t n, double* xd, Solve solve)
-{
-    std::vector<double> x(size_t(n), 1.0 / double(n));
-    std::vector<int> isgn(size_t(n), 0);
-    auto run = [&](int t) {
-        SA_HIP(hipMemcpy(xd, x.data(), sizeof(double) * size_t(n), hipMemcpyHostToDevice));
-        solve(xd, t);
-        SA_HIP(hipMemcpy(x.data(), xd, sizeof(double) * size_t(n), hipMemcpyDeviceToHost));
-    };
-    auto asum = [&]() { double s = 0; for (double e : x) s += std::fabs(e); return s; };
-    auto imax = [&]() {
-        int64_t j = 0;
-        for (int64_t i = 1; i < n; ++i) if (std::fabs(x[size_t(i)]) > std::fabs(x[size_t(j)])) j = i;
-        return j;
-    };
-    run(0);
-    if (n == 1) return std::fabs(x[0]);
-    double est = asum();
-    for (int64_t i = 0; i < n; ++i) { isgn[size_t(i)] = x[size_t(i)] >= 0 ? 1 : -1; x[size_t(i)] = isgn[size_t(i)]; }
-    run(1);
-    int64_t j = imax();
-    for (int iter = 2;; ++iter) {
-        std::fill(x.begin(), x.end(), 0.0);
-        x[size_t(j)] = 1.0;
-        run(0);
-        const double estold = est;
-        est = asum();
-        bool repeated = true;
-        for (int64_t i = 0; i < n && repeated; ++i) repeated = (x[size_t(i)] >= 0 ? 1 : -1) == isgn[size_t(i)];
-        if (repeated || est <= estold) break;
-        for (int64_t i = 0; i < n; ++i) { isgn[size_t(i)] = x[size_t(i)] >= 0 ? 1 : -1; x[size_t(i)] = isgn[size_t(i)]; }
-        run(1);
-        const int64_t jlast = j;
-        j = imax();
-        if (!(x[size_t(jlast)] != std::fabs(x[size_t(j)]) && iter < 5)) break;
-    }
-    double alt = 1.0;
-    for (int64_t i = 0; i < n; ++i) { x[size_t(i)] = alt * (1.0 + double(i) / double(n - 1)); alt = -alt; }
-    run(0);
-    const double temp = 2.0 * asum() / double(3 * n);
-    return std::max(est, temp);
-}
 
 } // namespace sa
 
@@ -1649,6 +1772,89 @@ int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double*
         *result = sa::norm_host(c, dA, n, lda, sym, w, we, w + we);
         return 0;
     }));
+}
+
+int slate_amd_tune_set_nb(slate_amd_context ctx, int kind, int count, const int64_t* n_upper,
+                          const int64_t* nb)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(kind >= 0 && kind < 4, -2, "unknown tuning kind");
+        SA_REQUIRE(count >= 0 && (count == 0 || (n_upper && nb)), -3, "bad table");
+        std::vector<std::pair<int64_t, int64_t>> t;
+        for (int i = 0; i < count; ++i) {
+            SA_REQUIRE(nb[i] >= 16 && nb[i] <= 4096, -5, "nb must be in [16, 4096]");
+            SA_REQUIRE(i == 0 || n_upper[i] > n_upper[i - 1], -4, "n_upper must increase");
+            t.emplace_back(n_upper[i], nb[i]);
+        }
+        c.nb_table[kind] = t;
+        return 0;
+    }));
+}
+
+// Learned selection: features, MLP forward / training step, and the solve
+// driven by it (no counterpart in the reference).
+int slate_amd_mlp_num_params(void) { return sa::mlp_num_params(); }
+
+int64_t slate_amd_selector_features(slate_amd_context ctx, int64_t n, const double* dA, int64_t lda,
+                                    const slate_amd_options* opts, float* features)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        SA_REQUIRE(features != nullptr, -6, "null features");
+        SA_REQUIRE(n >= 2, -3, "selector features need n >= 2");
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::MixedState st = sa::mixed_setup(c, n, 1);
+        int64_t info = 0;
+        sa::selector_features(c, st, n, dA, lda, o, &info, features);
+        sa::check_kernel_error(c);
+        return info;
+    });
+}
+
+int slate_amd_mlp_forward(slate_amd_context ctx, const float* dparams, const float* dX, int batch, float* dprobs)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(batch >= 0, -4, "bad batch");
+        SA_ENTRY_SYNC();
+        sa::mlp_forward(dparams, dX, batch, dprobs, c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return 0;
+    }));
+}
+
+int slate_amd_mlp_train_step(slate_amd_context ctx, float* dparams, float* dgrad, float* dm, float* dv,
+                             const float* dX, const int* dy, int batch, float lr, int step, float* loss)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(batch >= 1 && batch <= 4096, -8, "batch must be 1..4096");
+        SA_REQUIRE(step >= 1, -10, "step counts from 1");
+        SA_ENTRY_SYNC();
+        c.work.ensure(sizeof(float) * size_t(batch) * size_t(2 * 64 + 3) + sizeof(float) * 64);
+        float* scratch = c.work.as<float>();
+        float* dloss = scratch + size_t(batch) * size_t(2 * 64 + 3);
+        sa::mlp_train(dparams, dgrad, dm, dv, dX, dy, batch, scratch, lr, step, dloss, c.s0);
+        if (loss) SA_HIP(hipMemcpyAsync(loss, dloss, sizeof(float), hipMemcpyDeviceToHost, c.s0));
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return 0;
+    }));
+}
+
+int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int64_t lda, int64_t* ipiv,
+                              const double* dB, double* dX, const float* params, int* iter, int* choice,
+                              float* features, const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        SA_REQUIRE(n >= 2, -2, "n >= 2 required");
+        SA_REQUIRE(ipiv && iter && choice, -5, "null output");
+        SA_ENTRY_SYNC();
+        sa::Timer t(c);
+        sa::Opts o = sa::read_opts(opts);
+        int64_t info = sa::mixed_auto(c, n, dA, lda, ipiv, dB, dX, iter, choice, params, features, o);
+        t.stop();
+        return info;
+    });
 }
 
 // slate::gecondest (src/gecondest.cc): reciprocal condition number in the 1

@@ -43,6 +43,16 @@ void gemm_bf16s(int ns, int64_t M, int64_t N, int64_t Kp, const __bf16* A, int64
 void sigma_diag(int dist, double cond, int64_t n, double* A, int64_t lda, hipStream_t s);
 void larft_from_gram(const double* G, int64_t b, double* T, hipStream_t s);
 
+// Selector (mlp.hip): per-tile max |a| (tiles of 256, t x t row-major into
+// tmax), 16-bin exponent histogram (hist zeroed by the caller), and the MLP
+// forward pass / one training step (parameters in one float array).
+int mlp_num_params();
+void feature_tiles(const double* A, int64_t lda, int64_t n, double* tmax, hipStream_t s);
+void feature_hist(const double* A, int64_t lda, int64_t n, int emax, unsigned long long* hist, hipStream_t s);
+void mlp_forward(const float* P, const float* X, int B, float* probs, hipStream_t s);
+void mlp_train(float* P, float* grad, float* m, float* v, const float* X, const int* y, int B, float* scratch,
+               float lr, int t, float* loss, hipStream_t s);
+
 // Workspace shared by the cross-workgroup panel kernel.
 struct PanelWork {
     void* slots;                 // tagged candidate words exchanged between workgroups

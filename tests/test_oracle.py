@@ -288,3 +288,31 @@ def test_generate_svd_spectrum(sym, dist, cond):
         assert np.all(np.linalg.eigvalsh(A) > 0)
     else:
         assert not np.allclose(A, A.T)
+
+
+# ---- selector MLP -------------------------------------------------------------
+
+def test_mlp_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    P = rng.standard_normal(oracle.mlp_num_params()) * 0.3
+    X = rng.standard_normal((7, oracle.MLP_F))
+    y = rng.integers(0, oracle.MLP_K, 7)
+    loss, g = oracle.mlp_loss_grad(P, X, y)
+    idx = rng.choice(P.size, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += h
+        Pm[i] -= h
+        fd = (oracle.mlp_loss_grad(Pp, X, y)[0] - oracle.mlp_loss_grad(Pm, X, y)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+    p = oracle.mlp_forward(P, X)
+    np.testing.assert_allclose(p.sum(axis=1), 1.0, rtol=1e-12)
+    assert abs(loss + np.mean(np.log(p[np.arange(7), y]))) < 1e-12
+
+
+def test_adam_first_step_moves_by_lr():
+    # at t = 1 the bias-corrected step is lr * g / (|g| + eps): magnitude lr
+    g = np.array([0.5, -2.0, 1e-3])
+    P, m, v = oracle.adam_step(np.zeros(3), g, np.zeros(3), np.zeros(3), 0.01, 1)
+    np.testing.assert_allclose(P, -0.01 * np.sign(g), rtol=1e-4)
