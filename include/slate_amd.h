@@ -52,7 +52,13 @@ typedef struct {
     double  tolerance;         /* Option::Tolerance; <= 0 -> eps * sqrt(n)       */
     int32_t use_fallback;      /* Option::UseFallbackSolver (default 1)          */
     int32_t factor_precision;  /* SLATE_AMD_PREC_* for the mixed drivers         */
+    int32_t method_lu;         /* Option::MethodLU: SLATE_AMD_LU_* (default PP)  */
+    int32_t reserved;
 } slate_amd_options;
+
+/* slate::MethodLU (include/slate/enums.hh:294-301) for getrf/gesv and the
+ * float factor of the mixed drivers. */
+enum { SLATE_AMD_LU_PARTIALPIV = 0, SLATE_AMD_LU_NOPIV = 1 };
 
 /* ---- context ------------------------------------------------------------ */
 /* Creates the per-device state (two HIP streams, workspaces).  No counterpart

@@ -26,6 +26,7 @@ LOWER, UPPER = 0, 1
 NOTRANS, TRANS = 0, 1
 DIST_UNIFORM, DIST_RANDS, DIST_RANDN = 1, 2, 3
 SIGMA_ARITH, SIGMA_GEO, SIGMA_CLUSTER0, SIGMA_CLUSTER1 = 1, 2, 3, 4
+LU_PARTIALPIV, LU_NOPIV = 0, 1
 
 # Public C symbols (the C-ABI test checks that the library exports each one).
 C_SYMBOLS = [
@@ -58,6 +59,8 @@ class Options(ctypes.Structure):
         ("tolerance", ctypes.c_double),
         ("use_fallback", ctypes.c_int32),
         ("factor_precision", ctypes.c_int32),
+        ("method_lu", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def __init__(self, **kw):

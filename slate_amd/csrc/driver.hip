@@ -41,6 +41,7 @@ struct Context {
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
     Buf bfl, bfu0, bfu1, bfg;    // split-BF16 operand planes (LU update, gemm entry)
+    Buf npu, npt;                // no-pivoting panel: U11^{-1} and staging
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
@@ -93,6 +94,7 @@ struct Opts {
     double threshold = 1.0, tol = 0;
     bool fallback = true;
     int prec = SLATE_AMD_PREC_FP32;
+    int method = SLATE_AMD_LU_PARTIALPIV;
 };
 
 // Number of bf16 terms per operand of the float LU's trailing update
@@ -119,6 +121,9 @@ static Opts read_opts(const slate_amd_options* o)
     r.tol = o->tolerance;
     r.fallback = o->use_fallback != 0;
     r.prec = o->factor_precision;
+    r.method = o->method_lu;
+    SA_REQUIRE(r.method == SLATE_AMD_LU_PARTIALPIV || r.method == SLATE_AMD_LU_NOPIV, -2,
+               "unknown method_lu");
     return r;
 }
 
@@ -287,6 +292,30 @@ static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64
     laswp<T>(A, ld, n, c0, c0 + w1, c0 + w1, c0 + w, ipiv, plan, s);
 }
 
+// Panel of the no-pivoting LU (src/getrf_nopiv.cc): the kb x kb diagonal
+// block is factored without interchanges (the leaf kernel with threshold 0
+// keeps every diagonal pivot), then L21 = A21 U11^{-1} with the inverse of the
+// upper triangle applied by one GEMM.  No cross-workgroup pivot search over
+// the tall panel.  ipiv is left as the identity.
+template <class T>
+static void panel_nopiv(Context& c, T* A, int64_t ld, int64_t n, int64_t k0, int64_t kb, int64_t* ipiv,
+                        int64_t* info, int64_t leaf_w, int64_t* plan, T* uinv, T* tmp, hipStream_t s)
+{
+    panel_rec<T>(c, A, ld, k0 + kb, k0, kb, ipiv, info, 0.0, leaf_w, plan, s);
+    const int64_t m = n - k0 - kb;
+    if (m <= 0) return;
+    T* Akk = A + k0 * ld + k0;
+    set_identity<T>(uinv, kb, kb, s);
+    trsm_rec<T>(Akk, ld, 0, 0, 0, kb, uinv, kb, kb, s);          // U11^{-1}
+    GemmArgs g{};
+    g.M = m; g.N = kb; g.K = kb;
+    g.A = Akk + kb * ld; g.lda = ld; g.B = uinv; g.ldb = kb; g.C = tmp; g.ldc = kb;
+    g.alpha = 1.0; g.beta = 0.0; g.uplo = 0;
+    gemm<T>(g, A_KC, B_NC, s);
+    SA_HIP(hipMemcpy2DAsync(Akk + kb * ld, sizeof(T) * ld, tmp, sizeof(T) * kb, sizeof(T) * kb, m,
+                            hipMemcpyDeviceToDevice, s));
+}
+
 // Right-looking blocked LU of a row-major n x n matrix with lookahead 1 on a
 // second stream: the panel of step k+1 is factored on s1 while s0 applies
 // step k to the columns beyond k+1 (src/getrf.cc:85-235, with the OpenMP
@@ -383,9 +412,31 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     const int64_t plan_stride = 4 * nb + 8;
     c.plans.ensure(sizeof(int64_t) * size_t(plan_stride) * size_t(nsteps + 1));
     auto step_plan = [&](int64_t k) { return c.plans.as<int64_t>() + k * plan_stride; };
+    // MethodLU::NoPiv (src/getrf_nopiv.cc): panels without interchanges and
+    // no row swaps; the pivot vector stays the identity.
+    const bool nopiv = o.method == SLATE_AMD_LU_NOPIV;
+    T* npu = nullptr;
+    T* npt = nullptr;
+    if (nopiv) {
+        c.npu.ensure(sizeof(T) * size_t(nb) * size_t(nb));
+        c.npt.ensure(sizeof(T) * size_t(nb) * size_t(std::max<int64_t>(n, 1)));
+        npu = c.npu.as<T>();
+        npt = c.npt.as<T>();
+    }
+    auto factor_panel = [&](int64_t k, int64_t p0, int64_t pw) {       // on s1
+        if (nopiv) {
+            panel_nopiv<T>(c, A, ld, n, p0, pw, ipiv, info, leaf_w, plan1, npu, npt, s1);
+        }
+        else {
+            panel_rec<T>(c, A, ld, n, p0, pw, ipiv, info, o.threshold, leaf_w, plan1, s1);
+            laswp_plan(p0, p0 + pw, n, ipiv, step_plan(k), s1);
+        }
+    };
+    auto swap = [&](int64_t a0, int64_t a1, int64_t p0, int64_t p1, int64_t* pk, hipStream_t st) {
+        if (!nopiv) laswp<T>(A, ld, n, a0, a1, p0, p1, ipiv, pk, st, true);
+    };
     // panel 0
-    panel_rec<T>(c, A, ld, n, 0, std::min(nb, n), ipiv, info, o.threshold, leaf_w, plan1, s1);
-    laswp_plan(0, std::min(nb, n), n, ipiv, step_plan(0), s1);
+    factor_panel(0, 0, std::min(nb, n));
     unit_lower_inverse<T>(A, ld, std::min(nb, n), step_inv(0), s1);
     split_panel(0, 0, std::min(nb, n), s1);
     SA_HIP(hipEventRecord(ev_panel(0), s1));
@@ -396,23 +447,22 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         SA_HIP(hipStreamWaitEvent(s0, ev_panel(k), 0));
         // pivots of step k applied to the columns on the left (finished L) on s0
         int64_t* pk = step_plan(k);
-        laswp<T>(A, ld, n, 0, k0, k0, c1, ipiv, pk, s0, true);
+        swap(0, k0, k0, c1, pk, s0);
         if (r <= 0) break;
         const int64_t nb1 = std::min(nb, r);          // lookahead column block width
         if (la && r > nb1) {
             // s1: step k on column block k+1, then panel k+1
             SA_HIP(hipStreamWaitEvent(s1, k > 0 ? ev_trail(k) : ev_panel(0), 0));
-            laswp<T>(A, ld, n, c1, c1 + nb1, k0, c1, ipiv, pk, s1, true);
+            swap(c1, c1 + nb1, k0, c1, pk, s1);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, nb1, tmp1, s1);
             update(k, k0, kb, c1, nb1, true, s1, 0);
-            panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
-            laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
+            factor_panel(k + 1, c1, nb1);
             unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             split_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
             // s0: step k on the columns beyond k+1
             const int64_t c2 = c1 + nb1;
-            laswp<T>(A, ld, n, c2, n, k0, c1, ipiv, pk, s0, true);
+            swap(c2, n, k0, c1, pk, s0);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c2, ld, n - c2, tmp0, s0);
             prof_begin(c, s0);
             update(k, k0, kb, c2, n - c2, false, s0, 1);
@@ -421,7 +471,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         }
         else {
             // no lookahead: whole trailing matrix on s0, then the next panel
-            laswp<T>(A, ld, n, c1, n, k0, c1, ipiv, pk, s0, true);
+            swap(c1, n, k0, c1, pk, s0);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, r, tmp0, s0);
             prof_begin(c, s0);
             update(k, k0, kb, c1, r, false, s0, 1);
@@ -430,8 +480,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
                 SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
                 SA_HIP(hipStreamWaitEvent(s1, ev_trail(k + 1), 0));
             }
-            panel_rec<T>(c, A, ld, n, c1, nb1, ipiv, info, o.threshold, leaf_w, plan1, s1);
-            laswp_plan(c1, c1 + nb1, n, ipiv, step_plan(k + 1), s1);
+            factor_panel(k + 1, c1, nb1);
             unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             split_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
@@ -1474,7 +1523,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     sa::Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
-    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
+    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
@@ -1499,6 +1548,8 @@ void slate_amd_options_init(slate_amd_options* o)
     o->tolerance = 0.0;
     o->use_fallback = 1;
     o->factor_precision = SLATE_AMD_PREC_FP32;
+    o->method_lu = SLATE_AMD_LU_PARTIALPIV;
+    o->reserved = 0;
 }
 
 double slate_amd_last_time_ms(slate_amd_context ctx) { return ctx ? ctx->c.last_ms : 0.0; }
