@@ -58,7 +58,7 @@ typedef struct {
 
 /* slate::MethodLU (include/slate/enums.hh:294-301) for getrf/gesv and the
  * float factor of the mixed drivers. */
-enum { SLATE_AMD_LU_PARTIALPIV = 0, SLATE_AMD_LU_NOPIV = 1 };
+enum { SLATE_AMD_LU_PARTIALPIV = 0, SLATE_AMD_LU_NOPIV = 1, SLATE_AMD_LU_CALU = 2 };
 
 /* ---- context ------------------------------------------------------------ */
 /* Creates the per-device state (two HIP streams, workspaces).  No counterpart

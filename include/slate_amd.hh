@@ -58,7 +58,7 @@ enum class Op : char { NoTrans = 'N', Trans = 'T', ConjTrans = 'C' };
 // scalar_lo template argument, the BF16 choice has no SLATE counterpart.
 enum class FactorPrecision : int { FP32 = SLATE_AMD_PREC_FP32, BF16 = SLATE_AMD_PREC_BF16 };
 
-enum class MethodLU : char { PartialPiv = 'P', NoPiv = 'N' };   // include/slate/enums.hh:294-301
+enum class MethodLU : char { PartialPiv = 'P', CALU = 'C', NoPiv = 'N' };   // include/slate/enums.hh:294-301
 
 enum class Option : char {
     Lookahead, InnerBlocking, MaxPanelThreads, PivotThreshold, Tolerance, MaxIterations,
@@ -321,8 +321,10 @@ inline slate_amd_options to_c(const Options& o, int64_t nb)
     c.use_fallback = int32_t(opt_i(o, Option::UseFallbackSolver, 1));
     c.factor_precision = int32_t(opt_i(o, Option::FactorPrecision, int64_t(FactorPrecision::FP32)));
     auto m = MethodLU(opt_i(o, Option::MethodLU, int64_t(MethodLU::PartialPiv)));
-    if (m != MethodLU::PartialPiv && m != MethodLU::NoPiv) throw NotImplemented("MethodLU");
-    c.method_lu = m == MethodLU::NoPiv ? SLATE_AMD_LU_NOPIV : SLATE_AMD_LU_PARTIALPIV;
+    if (m != MethodLU::PartialPiv && m != MethodLU::NoPiv && m != MethodLU::CALU)
+        throw NotImplemented("MethodLU");
+    c.method_lu = m == MethodLU::NoPiv ? SLATE_AMD_LU_NOPIV
+                : m == MethodLU::CALU ? SLATE_AMD_LU_CALU : SLATE_AMD_LU_PARTIALPIV;
     return c;
 }
 

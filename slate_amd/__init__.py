@@ -26,7 +26,7 @@ LOWER, UPPER = 0, 1
 NOTRANS, TRANS = 0, 1
 DIST_UNIFORM, DIST_RANDS, DIST_RANDN = 1, 2, 3
 SIGMA_ARITH, SIGMA_GEO, SIGMA_CLUSTER0, SIGMA_CLUSTER1 = 1, 2, 3, 4
-LU_PARTIALPIV, LU_NOPIV = 0, 1
+LU_PARTIALPIV, LU_NOPIV, LU_CALU = 0, 1, 2
 
 # Public C symbols (the C-ABI test checks that the library exports each one).
 C_SYMBOLS = [

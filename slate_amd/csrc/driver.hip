@@ -45,6 +45,7 @@ struct Context {
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
+    bool calu = false;           // leaves use tournament pivoting (MethodLU::CALU)
     // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
     std::vector<std::pair<int64_t, int64_t>> nb_table[4];
     // trailing-update GEMM timing (enabled by slate_amd_profile_enable)
@@ -122,8 +123,8 @@ static Opts read_opts(const slate_amd_options* o)
     r.fallback = o->use_fallback != 0;
     r.prec = o->factor_precision;
     r.method = o->method_lu;
-    SA_REQUIRE(r.method == SLATE_AMD_LU_PARTIALPIV || r.method == SLATE_AMD_LU_NOPIV, -2,
-               "unknown method_lu");
+    SA_REQUIRE(r.method == SLATE_AMD_LU_PARTIALPIV || r.method == SLATE_AMD_LU_NOPIV
+               || r.method == SLATE_AMD_LU_CALU, -2, "unknown method_lu");
     return r;
 }
 
@@ -144,8 +145,9 @@ static int64_t choose_nb(const Context& c, int kind, int64_t n, int64_t requeste
 static void prepare_panel_work(Context& c, int64_t n, int64_t W)
 {
     (void)W;
-    const size_t bytes = size_t(1) << 22;
+    const size_t bytes = std::max<size_t>(size_t(1) << 22, size_t(calu_slot_bytes(8, n)) + 4096);
     const bool fresh = c.slots.bytes < bytes;
+    if (fresh) c.slots.release();
     c.slots.ensure(bytes);
     if (fresh || c.leaf_tag > 0xF0000000u - uint32_t(std::min<int64_t>(n, 1 << 26))) {
         zero_u32(static_cast<unsigned*>(c.slots.p), int64_t(c.slots.bytes / 4), c.s0);
@@ -271,8 +273,14 @@ static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64
                       int64_t* plan, hipStream_t s)
 {
     if (w <= leaf_w) {
-        panel_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, thr, panel_work(c), c.leaf_tag, s);
-        c.leaf_tag += uint32_t(w);
+        if (c.calu) {
+            calu_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, panel_work(c), c.leaf_tag, s);
+            c.leaf_tag += 16;
+        }
+        else {
+            panel_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, thr, panel_work(c), c.leaf_tag, s);
+            c.leaf_tag += uint32_t(w);
+        }
         return;
     }
     int64_t w1 = ((w / 2 + leaf_w - 1) / leaf_w) * leaf_w;
@@ -328,7 +336,8 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     const int kind = sizeof(T) == 8 ? SLATE_AMD_TUNE_LU_FP64
                    : (split_terms(o.prec) > 0 ? SLATE_AMD_TUNE_LU_BF16 : SLATE_AMD_TUNE_LU_FP32);
     const int64_t nb = choose_nb(c, kind, n, o.nb);
-    int64_t leaf_w = panel_leaf_width(int(sizeof(T)));
+    c.calu = o.method == SLATE_AMD_LU_CALU;
+    int64_t leaf_w = c.calu ? calu_leaf_width(int(sizeof(T))) : panel_leaf_width(int(sizeof(T)));
     if (o.ib > 0 && o.ib < leaf_w) leaf_w = o.ib;
     prepare_panel_work(c, n, leaf_w);
     c.plan1.ensure(sizeof(int64_t) * (4 * 2048 + 8));

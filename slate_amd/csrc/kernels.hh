@@ -72,6 +72,15 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
                 uint32_t tag0, hipStream_t s);
 int panel_leaf_width(int elem_bytes);   // largest leaf width the kernel supports
 
+// Tournament-pivoting leaf (MethodLU::CALU): same contract as panel_leaf
+// with the pivots chosen by a butterfly tournament over the workgroups; uses
+// tags tag0 .. tag0 + 8.  calu_slot_bytes: exchange area it needs for m rows.
+template <class T>
+void calu_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w, int64_t* ipiv, int64_t* info,
+               const PanelWork& pw, uint32_t tag0, hipStream_t s);
+int calu_leaf_width(int elem_bytes);
+int64_t calu_slot_bytes(int elem_bytes, int64_t m);
+
 // Row interchanges: apply swaps (i, ipiv[i]) for i in [k1, k2), in order, to
 // columns [c0, c1) of a row-major matrix with nrows rows.  Two launches: a
 // one-block plan that turns the sequence into a (dst <- src) row list, then a
