@@ -178,6 +178,19 @@ int64_t slate_amd_dgecondest(slate_amd_context ctx, int norm, int64_t n, const d
 int64_t slate_amd_dpocondest(slate_amd_context ctx, int uplo, int64_t n, const double* dL, int64_t lda,
                              double anorm, double* rcond);
 
+/* ---- batched uneven solves (no counterpart in the reference) ------------
+ * count systems A_i x_i = b_i, n[i] in [1, 1024] (host array), packed on the
+ * device back to back: A_i column-major n_i x n_i at dA + sum_{k<i} n_k^2,
+ * b_i and x_i at dB / dX + sum_{k<i} n_k.  One persistent launch (a work
+ * queue over the systems, largest first).  factor_precision FP32: float
+ * factor + FP64 refinement to SLATE's stopping rule with the FP64 fallback
+ * (mode[i] = 1, host, may be NULL, sends system i straight to FP64); FP64:
+ * plain FP64 LU.  Outputs (host): iters[i] as in dsgesv (0 for FP64),
+ * info[i], berr[i] = ||b - A x||_inf / (||A||_inf ||x||_inf). */
+int64_t slate_amd_dgesv_batched(slate_amd_context ctx, int count, const int* n, const double* dA,
+                                const double* dB, double* dX, const int* mode, int factor_precision,
+                                int* iters, int* info, double* berr, const slate_amd_options* opts);
+
 /* ---- tuning ---------------------------------------------------------------
  * Block size nb per kind of factorization, used when options.block_size is 0
  * (SLATE takes nb from the matrix tiling; the autotuner of

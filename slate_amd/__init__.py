@@ -38,7 +38,7 @@ C_SYMBOLS = [
     "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate", "slate_amd_generate_svd",
     "slate_amd_dgecondest", "slate_amd_dpocondest", "slate_amd_mlp_num_params",
     "slate_amd_selector_features", "slate_amd_mlp_forward", "slate_amd_mlp_train_step",
-    "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb",
+    "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb", "slate_amd_dgesv_batched",
     "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_profile_enable",
     "slate_amd_profile_read",
 ]
@@ -113,6 +113,7 @@ def lib():
             "slate_amd_dpocondest": (i64, [vp, i32, i64, vp, i64, dbl, ctypes.POINTER(dbl)]),
             "slate_amd_mlp_num_params": (i32, []),
             "slate_amd_tune_set_nb": (i32, [vp, i32, i32, pi64, pi64]),
+            "slate_amd_dgesv_batched": (i64, [vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, po]),
             "slate_amd_selector_features": (i64, [vp, i64, vp, i64, po, vp]),
             "slate_amd_mlp_forward": (i32, [vp, vp, vp, i32, vp]),
             "slate_amd_mlp_train_step": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_float, i32,
@@ -404,6 +405,25 @@ def gesv_auto(A, B, X, params=None, opts=None):
                                               ctypes.byref(it), ctypes.byref(ch),
                                               feat.ctypes.data_as(ctypes.c_void_p), _opts(opts)))
     return info, it.value, ch.value, ipiv[:n], feat
+
+
+def gesv_batched(ns, A, B, X, factor_precision=PREC_FP32, mode=None, opts=None):
+    """Batched uneven solves: ns (sizes), A/B/X packed float64 CUDA buffers
+    (A_i column-major back to back, then b_i / x_i).  Returns (iters, info,
+    berr) numpy arrays."""
+    ns = np.ascontiguousarray(ns, dtype=np.int32)
+    cnt = int(ns.size)
+    iters = np.zeros(cnt, dtype=np.int32)
+    info = np.zeros(cnt, dtype=np.int32)
+    berr = np.zeros(cnt, dtype=np.float64)
+    md = None if mode is None else np.ascontiguousarray(mode, dtype=np.int32)
+    dev = A.device.index or 0
+    vp = ctypes.c_void_p
+    _check(lib().slate_amd_dgesv_batched(context(dev), cnt, ns.ctypes.data_as(vp), A.data_ptr(), B.data_ptr(),
+                                         X.data_ptr(), None if md is None else md.ctypes.data_as(vp),
+                                         int(factor_precision), iters.ctypes.data_as(vp),
+                                         info.ctypes.data_as(vp), berr.ctypes.data_as(vp), _opts(opts)))
+    return iters, info, berr
 
 
 def gecondest(LU, ipiv, anorm, norm="1"):

@@ -79,6 +79,38 @@ __device__ inline uint32_t tw_wait(const uint64_t* p, uint32_t tag, int* error, 
         }
     }
 }
+// Batched wait: the N words p + off[u] (u with valid[u]) are loaded together
+// and reloaded together until every one carries `tag`, so one iteration costs
+// one memory round trip however many words are still missing (a word-by-word
+// wait would serialise the round trips).  Bounded like tw_wait.
+template <int N>
+__device__ inline void tw_gather(const uint64_t* p, const int64_t (&off)[N], const bool (&valid)[N],
+                                 uint32_t tag, uint32_t (&out)[N], int* error, int code)
+{
+    uint64_t w[N];
+    #pragma unroll
+    for (int u = 0; u < N; ++u) w[u] = tw_load(p + off[u]);
+    uint64_t t0 = 0;
+    bool started = false;
+    while (true) {
+        bool ok = true;
+        #pragma unroll
+        for (int u = 0; u < N; ++u) ok = ok && (!valid[u] || uint32_t(w[u] >> 32) == tag);
+        if (ok) break;
+        if (!started) { t0 = wall_clock64(); started = true; }
+        else if (wall_clock64() - t0 > 400000000ull) {
+            __hip_atomic_store(error, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        __builtin_amdgcn_s_sleep(1);
+        #pragma unroll
+        for (int u = 0; u < N; ++u) w[u] = tw_load(p + off[u]);
+    }
+    #pragma unroll
+    for (int u = 0; u < N; ++u) out[u] = uint32_t(w[u]);
+}
+
 template <class T> struct TwWords { static constexpr int n = sizeof(T) / 4; };
 template <class T>
 __device__ inline void tw_store_val(uint64_t* p, T v, uint32_t tag)

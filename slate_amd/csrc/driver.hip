@@ -42,6 +42,7 @@ struct Context {
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
     Buf bfl, bfu0, bfu1, bfg;    // split-BF16 operand planes (LU update, gemm entry)
     Buf npu, npt;                // no-pivoting panel: U11^{-1} and staging
+    Buf bmeta, bwork;            // batched solves: per-system metadata, per-workgroup workspace
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
@@ -1532,7 +1533,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     sa::Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
-    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
+    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
@@ -1914,6 +1915,69 @@ int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int6
         int64_t info = sa::mixed_auto(c, n, dA, lda, ipiv, dB, dX, iter, choice, params, features, o);
         t.stop();
         return info;
+    });
+}
+
+// Batched uneven solves (no counterpart in the reference; SLATE solves one
+// distributed system per call).
+int64_t slate_amd_dgesv_batched(slate_amd_context ctx, int count, const int* n, const double* dA,
+                                const double* dB, double* dX, const int* mode, int factor_precision,
+                                int* iters, int* info, double* berr, const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(count >= 0, -2, "bad count");
+        if (count == 0) return 0;
+        SA_REQUIRE(n && dA && dB && dX && iters && info && berr, -3, "null argument");
+        SA_REQUIRE(factor_precision == SLATE_AMD_PREC_FP32 || factor_precision == SLATE_AMD_PREC_FP64, -8,
+                   "factor_precision must be FP32 or FP64");
+        const int maxn = sa::batched_max_n();
+        std::vector<int64_t> offA(count), offB(count);
+        std::vector<int> order(count), md(count, 0);
+        int64_t a = 0, b = 0;
+        for (int i = 0; i < count; ++i) {
+            SA_REQUIRE(n[i] >= 1 && n[i] <= maxn, -3, "system sizes must be in [1, 1024]");
+            offA[i] = a; offB[i] = b;
+            a += int64_t(n[i]) * n[i];
+            b += n[i];
+            order[i] = i;
+            if (mode) { SA_REQUIRE(mode[i] == 0 || mode[i] == 1, -7, "mode must be 0 or 1"); md[i] = mode[i]; }
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+        sa::Opts o = sa::read_opts(opts);
+        SA_ENTRY_SYNC();
+        sa::Timer t(c);
+        // metadata: offA, offB (int64) | n, mode, order, iters, info (int) | berr (double) | queue
+        const size_t meta = sizeof(int64_t) * 2 * count + sizeof(int) * 5 * size_t(count)
+                            + sizeof(double) * count + 1024;
+        c.bmeta.ensure(meta);
+        char* p = static_cast<char*>(c.bmeta.p);
+        int64_t* d_offA = reinterpret_cast<int64_t*>(p);
+        int64_t* d_offB = d_offA + count;
+        int* d_n = reinterpret_cast<int*>(d_offB + count);
+        int* d_mode = d_n + count;
+        int* d_order = d_mode + count;
+        int* d_iters = d_order + count;
+        int* d_info = d_iters + count;
+        double* d_berr = reinterpret_cast<double*>((reinterpret_cast<uintptr_t>(d_info + count) + 7) & ~uintptr_t(7));
+        int* d_queue = reinterpret_cast<int*>(d_berr + count);
+        SA_HIP(hipMemcpyAsync(d_offA, offA.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice, c.s0));
+        SA_HIP(hipMemcpyAsync(d_offB, offB.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice, c.s0));
+        SA_HIP(hipMemcpyAsync(d_n, n, sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
+        SA_HIP(hipMemcpyAsync(d_mode, md.data(), sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
+        SA_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
+        int cus = 0;
+        SA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+        const int eb = factor_precision == SLATE_AMD_PREC_FP64 ? 8 : 4;
+        const int grid = std::max(1, std::min(count, cus));
+        c.bwork.ensure(size_t(grid) * size_t(sa::batched_work_stride(eb)));
+        sa::batched_solve(eb, count, d_order, d_n, d_mode, d_offA, d_offB, dA, dB, dX, d_iters, d_info, d_berr,
+                          d_queue, static_cast<char*>(c.bwork.p), grid, int(o.itermax), o.tol, c.s0);
+        SA_HIP(hipMemcpyAsync(iters, d_iters, sizeof(int) * count, hipMemcpyDeviceToHost, c.s0));
+        SA_HIP(hipMemcpyAsync(info, d_info, sizeof(int) * count, hipMemcpyDeviceToHost, c.s0));
+        SA_HIP(hipMemcpyAsync(berr, d_berr, sizeof(double) * count, hipMemcpyDeviceToHost, c.s0));
+        SA_HIP(hipStreamSynchronize(c.s0));
+        t.stop();
+        return 0;
     });
 }
 

@@ -53,6 +53,17 @@ void mlp_forward(const float* P, const float* X, int B, float* probs, hipStream_
 void mlp_train(float* P, float* grad, float* m, float* v, const float* X, const int* y, int B, float* scratch,
                float lr, int t, float* loss, hipStream_t s);
 
+// Batched uneven solves (batched.hip): one persistent launch of `grid`
+// workgroups; elem_bytes 4 = float factor + FP64 refinement (mode[i] = 1
+// forces FP64 for system i), 8 = FP64 factor.  Device arrays throughout;
+// work: grid * batched_work_stride(elem_bytes) bytes; queue: one int.
+int64_t batched_work_stride(int elem_bytes);
+int batched_max_n();
+void batched_solve(int elem_bytes, int count, const int* order, const int* n, const int* mode,
+                   const int64_t* offA, const int64_t* offB, const double* A, const double* B, double* X,
+                   int* iters, int* info, double* berr, int* queue, char* work, int grid, int itermax,
+                   double tol, hipStream_t s);
+
 // Workspace shared by the cross-workgroup panel kernel.
 struct PanelWork {
     void* slots;                 // tagged candidate words exchanged between workgroups

@@ -168,24 +168,24 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
         }
 
         // (3) gather every word of this column: all loads issued at once
-        // (fixed trip count, clamped addresses), then only the words whose
-        // tag is not there yet are polled (bounded wait).
+        // (fixed trip count, clamped addresses) and re-polled together until
+        // every tag has arrived (one round trip per poll).
         {
             constexpr int NCR = (SL::MAXWORDS + LEAF_THREADS - 1) / LEAF_THREADS;
-            uint64_t wv[NCR];
+            int64_t off[NCR];
+            bool val[NCR];
+            uint32_t got[NCR];
             #pragma unroll
             for (int u = 0; u < NCR; ++u) {
-                int e = tid + u * LEAF_THREADS;
-                wv[u] = tw_load(buf + (e < total ? e : total - 1));
+                const int e = tid + u * LEAF_THREADS;
+                val[u] = e < total;
+                off[u] = e < total ? e : total - 1;
             }
+            tw_gather<NCR>(buf, off, val, tag, got, error, 1);
             #pragma unroll
             for (int u = 0; u < NCR; ++u) {
-                int e = tid + u * LEAF_THREADS;
-                if (e < total) {
-                    uint32_t x = uint32_t(wv[u]);
-                    if (uint32_t(wv[u] >> 32) != tag) x = tw_wait(buf + e, tag, error, 1);
-                    s_words[e] = x;
-                }
+                const int e = tid + u * LEAF_THREADS;
+                if (e < total) s_words[e] = got[u];
             }
         }
         __syncthreads();

@@ -45,6 +45,13 @@ struct CaluSlots {
 };
 
 template <class T>
+__device__ inline T raw_val(const uint32_t* w)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, (uint64_t(w[1]) << 32) | w[0]);
+    else return __builtin_bit_cast(T, w[0]);
+}
+
+template <class T>
 __device__ inline bool calu_better(T a, int ia, T b, int ib)
 {
     return a > b || (a == b && ia < ib);
@@ -141,7 +148,7 @@ void calu_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total
     __shared__ int s_dn_pos[W], s_dn_row[W];
     __shared__ int s_ndn;
     __shared__ T s_top[W][W];
-
+    __shared__ uint32_t s_raw[SL::SET];   // payloads of a partner's set
     auto rowp = [&](int64_t i) { return A + (r0 + i) * ld + c0; };
     T v[RPT][W];
     #pragma unroll
@@ -236,15 +243,34 @@ void calu_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total
         if (partner < G) {                            // uniform over the workgroup
             const uint64_t* theirs = slots + SL::round_base(r, G) + int64_t(partner) * SL::SET;
             const bool lo_first = partner < g;        // lower subtree's rows first
+            {
+                // all words of the partner's set in one batched wait
+                constexpr int NSW = int((SL::SET + CALU_THREADS - 1) / CALU_THREADS);
+                int64_t off[NSW];
+                bool val[NSW];
+                uint32_t got[NSW];
+                #pragma unroll
+                for (int u = 0; u < NSW; ++u) {
+                    const int64_t e = tid + int64_t(u) * CALU_THREADS;
+                    val[u] = e < SL::SET;
+                    off[u] = e < SL::SET ? e : SL::SET - 1;
+                }
+                tw_gather<NSW>(theirs, off, val, tag, got, error, 1);
+                #pragma unroll
+                for (int u = 0; u < NSW; ++u) {
+                    const int64_t e = tid + int64_t(u) * CALU_THREADS;
+                    if (e < SL::SET) s_raw[e] = got[u];
+                }
+            }
+            __syncthreads();
             for (int e = tid; e < W + W * W; e += CALU_THREADS) {
                 if (e < W) {
-                    const int id = int(tw_wait(theirs + e, tag, error, 1));
-                    s_mid[lo_first ? e : W + e] = id;
+                    s_mid[lo_first ? e : W + e] = int(s_raw[e]);
                     s_mid[lo_first ? W + e : e] = s_sid[e];
                 }
                 else {
                     const int j = (e - W) / W, c = (e - W) % W;
-                    const T x = tw_wait_val<T>(theirs + W + int64_t(e - W) * NWV, tag, error, 1);
+                    const T x = raw_val<T>(&s_raw[W + (e - W) * NWV]);
                     s_m[lo_first ? j : W + j][c] = x;
                     s_m[lo_first ? W + j : j][c] = s_set[j][c];
                 }
@@ -318,12 +344,29 @@ void calu_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total
         }
     }
     __syncthreads();
-    // original top rows for the displaced positions this workgroup owns
-    for (int k = 0; k < s_ndn; ++k) {
-        const int64_t pos = s_dn_pos[k];
-        if (pos >= gbase && pos < gbase + R) {
-            for (int c = tid; c < W; c += CALU_THREADS)
-                s_top[k][c] = tw_wait_val<T>(slots + (int64_t(s_dn_row[k]) * W + c) * NWV, tag0, error, 1);
+    // original top rows for the displaced positions this workgroup owns, in
+    // one batched wait
+    {
+        constexpr int TW = W * NWV;                   // words per top row
+        constexpr int NTW = (W * TW + CALU_THREADS - 1) / CALU_THREADS;
+        int64_t off[NTW];
+        bool val[NTW];
+        uint32_t got[NTW];
+        const int ndn = s_ndn;
+        #pragma unroll
+        for (int u = 0; u < NTW; ++u) {
+            const int e = tid + u * CALU_THREADS;
+            const int k = e / TW;
+            const bool mine = k < ndn && int64_t(s_dn_pos[k < W ? k : 0]) >= gbase
+                              && int64_t(s_dn_pos[k < W ? k : 0]) < gbase + R;
+            val[u] = e < W * TW && mine;
+            off[u] = val[u] ? int64_t(s_dn_row[k]) * TW + e % TW : 0;
+        }
+        tw_gather<NTW>(slots, off, val, tag0, got, error, 1);
+        #pragma unroll
+        for (int u = 0; u < NTW; ++u) {
+            const int e = tid + u * CALU_THREADS;
+            if (val[u]) reinterpret_cast<uint32_t*>(&s_top[0][0])[(e / TW) * TW + e % TW] = got[u];
         }
     }
     __syncthreads();
