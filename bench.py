@@ -56,6 +56,55 @@ def cpu_baseline(n_cpu):
     }
 
 
+def bench_batched(args):
+    """Uneven batch: n uniform in [32, 1024], half general (kappa log-uniform
+    in [1e1, 1e8], geometric spectrum) and half SPD, one right-hand side each;
+    float factor + FP64 refinement vs the plain FP64 batched path."""
+    import numpy as np
+    import torch
+    import slate_amd as S
+    rng = np.random.default_rng(2024)
+    cnt = args.batch
+    ns = rng.integers(32, 1025, cnt).astype(np.int32)
+    kappa = 10.0 ** rng.uniform(1.0, 8.0, cnt)
+    log("bench: generating %d systems (sum n^2 = %.3g)" % (cnt, float(np.sum(ns.astype(np.float64) ** 2))))
+    offs = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)])
+    A = torch.empty(int(offs[-1]), dtype=torch.float64, device="cuda")
+    for i in range(cnt):
+        n = int(ns[i])
+        Ai = A[offs[i]:offs[i + 1]].view(n, n).t()          # column-major view
+        S.generate_svd(Ai, S.SIGMA_GEO, float(kappa[i]), 1000 + i, sym=bool(i % 2))
+    B = torch.empty(int(ns.sum()), dtype=torch.float64, device="cuda")
+    S.generate(B, S.DIST_RANDS, 7)
+    X = torch.zeros_like(B)
+    res = {}
+    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_FP32)):
+        S.gesv_batched(ns, A, B, X, prec)                    # warmup
+        times = []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            iters, info, berr = S.gesv_batched(ns, A, B, X, prec)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        ms = 1e3 * min(times)
+        res[name] = {"ms": round(ms, 3), "systems_per_s": round(cnt / (ms * 1e-3), 1),
+                     "max_berr": float(berr.max()), "median_berr": float(np.median(berr)),
+                     "info_nonzero": int(np.sum(info != 0)),
+                     "iters_hist": np.bincount(np.clip(iters, -1, 31) + 1).tolist()}
+        log("bench: batched %s %.1f ms, %.0f systems/s, max berr %.3g" % (name, ms, cnt / (ms * 1e-3), berr.max()))
+    out = {
+        "metric": "batched uneven solves, systems/s (FP64-accurate)", "value": res["mixed"]["systems_per_s"],
+        "unit": "systems/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": res["mixed"]["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 factor + f64 refinement", "data": "synthetic",
+        "config": {"workload": "%d systems, n uniform [32,1024], half general kappa log-uniform [1e1,1e8], "
+                               "half SPD, nrhs=1, one persistent launch" % cnt,
+                   "mixed": res["mixed"], "plain_fp64": res["fp64"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +116,9 @@ def main():
     ap.add_argument("--factor-precision", default="fp32", choices=sorted(PRECS),
                     help="factor precision of the mixed solve (bf16x3 = float-accurate split-BF16)")
     ap.add_argument("--gmres", action="store_true", help="GMRES-IR instead of classic IR")
+    ap.add_argument("--workload", default="gesv", choices=["gesv", "batched"],
+                    help="batched: many uneven systems in one launch instead of the headline solve")
+    ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -80,6 +132,9 @@ def main():
 
     import torch
     import slate_amd as S
+
+    if args.workload == "batched":
+        return bench_batched(args)
 
     if args.copy_probe_mib > 0:
         cnt = args.copy_probe_mib * (1 << 20) // 4

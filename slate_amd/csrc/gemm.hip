@@ -43,8 +43,11 @@ template <> struct Mf<float> {
 // B layout: B_NC = N-contiguous (B[k*ldb + n]), B_KC = K-contiguous (B[k + n*ldb]).
 // ROLE only names the instantiation: 1 = the trailing-matrix update of the
 // blocked LU (profiled on its own), 0 = every other product.
+// PF: prefetch depth in K slices.  PF = 2 keeps two register sets, so a
+// slice's global loads are issued 1.5 slices before its LDS store (more
+// latency cover at the cost of one more set of staging registers).
 template <class T, int ALAY, int BLAY, int BM, int BN, int BK, int WM = 2, int WN = 2, int MINB = 1,
-          int ROLE = 0>
+          int ROLE = 0, int PF = 1>
 __global__ __launch_bounds__(WM * WN * 64, MINB)
 void gemm_kernel(GemmArgs p)
 {
@@ -95,9 +98,9 @@ void gemm_kernel(GemmArgs p)
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
 
-    T ra[A_PER], rb[B_PER];
+    T ra[PF][A_PER], rb[PF][B_PER];
 
-    auto load_tiles = [&](int64_t k0) {
+    auto load_tiles = [&](int64_t k0, int set) {
         #pragma unroll
         for (int i = 0; i < A_PER; ++i) {
             int e = tid + i * NT;
@@ -110,7 +113,8 @@ void gemm_kernel(GemmArgs p)
             int64_t cm_ = gm_ < M ? gm_ : M - 1, ck_ = gk_ < K ? gk_ : K - 1;
             int64_t off = (ALAY == A_KC) ? cm_ * lda + ck_ : cm_ + ck_ * lda;
             T v = A[off];
-            ra[i] = ok ? v : T(0);
+            #pragma unroll
+            for (int q = 0; q < PF; ++q) if (q == set) ra[q][i] = ok ? v : T(0);
         }
         #pragma unroll
         for (int i = 0; i < B_PER; ++i) {
@@ -123,19 +127,24 @@ void gemm_kernel(GemmArgs p)
             int64_t cn_ = gn_ < N ? gn_ : N - 1, ck_ = gk_ < K ? gk_ : K - 1;
             int64_t off = (BLAY == B_NC) ? ck_ * ldb + cn_ : ck_ + cn_ * ldb;
             T v = B[off];
-            rb[i] = ok ? v : T(0);
+            #pragma unroll
+            for (int q = 0; q < PF; ++q) if (q == set) rb[q][i] = ok ? v : T(0);
         }
     };
-    auto store_tiles = [&](int buf) {
+    auto store_tiles = [&](int buf, int set) {
         #pragma unroll
-        for (int i = 0; i < A_PER; ++i) {
-            int e = tid + i * NT;
-            As[buf][(e / A_CD) * A_LD + (e % A_CD)] = ra[i];
-        }
-        #pragma unroll
-        for (int i = 0; i < B_PER; ++i) {
-            int e = tid + i * NT;
-            Bs[buf][(e / B_CD) * B_LD + (e % B_CD)] = rb[i];
+        for (int q = 0; q < PF; ++q) {
+            if (q != set) continue;
+            #pragma unroll
+            for (int i = 0; i < A_PER; ++i) {
+                int e = tid + i * NT;
+                As[buf][(e / A_CD) * A_LD + (e % A_CD)] = ra[q][i];
+            }
+            #pragma unroll
+            for (int i = 0; i < B_PER; ++i) {
+                int e = tid + i * NT;
+                Bs[buf][(e / B_CD) * B_LD + (e % B_CD)] = rb[q][i];
+            }
         }
     };
     auto a_at = [&](int buf, int m, int k) -> T {
@@ -154,21 +163,19 @@ void gemm_kernel(GemmArgs p)
             for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
 
     const int64_t ktiles = ceildiv(K, BK);
-    load_tiles(0);
-    store_tiles(0);
-    __syncthreads();
-
     const int fr = lane & 15, fk = lane >> 4;
-    for (int64_t kt = 0; kt < ktiles; ++kt) {
+    // One K slice: MFMAs on LDS buffer kt&1; the slice kt+1 (register set
+    // `sn`) is written to the other buffer half-way through, so the LDS writes
+    // overlap the remaining MFMAs (that buffer was last read before the
+    // previous barrier); with PF = 2 the loads of slice kt+2 go out first.
+    auto slice = [&](int64_t kt, int sn) {
         const int cur = kt & 1;
         const bool more = kt + 1 < ktiles;
-        if (more) load_tiles((kt + 1) * BK);
+        if (PF == 1) { if (more) load_tiles((kt + 1) * BK, 0); }
+        else if (kt + 2 < ktiles) load_tiles((kt + 2) * BK, 1 - sn);
         #pragma unroll
         for (int kk = 0; kk < BK; kk += KM) {
-            // Stage the next slice half-way through this one, so the LDS writes
-            // overlap the remaining MFMAs instead of draining the matrix pipe
-            // (the other buffer was last read before the previous barrier).
-            if (kk == BK / 2 && more) store_tiles(cur ^ 1);
+            if (kk == BK / 2 && more) store_tiles(cur ^ 1, PF == 1 ? 0 : sn);
             T af[FM], bf[FN];
             #pragma unroll
             for (int i = 0; i < FM; ++i) af[i] = a_at(cur, wm * TM + i * 16 + fr, kk + fk);
@@ -180,6 +187,19 @@ void gemm_kernel(GemmArgs p)
                 for (int j = 0; j < FN; ++j) acc[i][j] = Mf<T>::op(af[i], bf[j], acc[i][j]);
         }
         if (more) __syncthreads();
+    };
+    load_tiles(0, 0);
+    store_tiles(0, 0);
+    if (PF == 2 && ktiles > 1) load_tiles(BK, 1);
+    __syncthreads();
+    if (PF == 1) {
+        for (int64_t kt = 0; kt < ktiles; ++kt) slice(kt, 0);
+    }
+    else {
+        for (int64_t kt = 0; kt < ktiles; kt += 2) {
+            slice(kt, 1);
+            if (kt + 1 < ktiles) slice(kt + 1, 0);
+        }
     }
 
     const T alpha = T(p.alpha), beta = T(p.beta);

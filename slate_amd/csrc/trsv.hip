@@ -124,7 +124,6 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
     const int rows = int((n - r0 < BS) ? n - r0 : BS);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    __shared__ T sy[BS];
     __shared__ T sr[BS];
     __shared__ T spart[NPART][BS];
     __shared__ T sred[4][RPW][65];
@@ -153,7 +152,10 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
                 uint64_t t0 = wall_clock64();
                 unsigned v;
                 while ((v = __hip_atomic_load(ndone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < unsigned(t + 1)) {
-                    __builtin_amdgcn_s_sleep(1);
+                    // blocks far from their turn poll rarely, so the few that
+                    // are next see the counter move without contention
+                    if (unsigned(t + 1) - v > 2) __builtin_amdgcn_s_sleep(16);
+                    else __builtin_amdgcn_s_sleep(1);
                     if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { v = unsigned(ndep); break; }
                     if (wall_clock64() - t0 > 400000000ull) {
                         __hip_atomic_store(error, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -175,20 +177,19 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
         // values are zeroed afterwards), so the compiler issues a whole tile's
         // loads back to back instead of one guarded round trip per element.
         if (CM) {
-            if (tid < BS) {
-                T v = X[c0 + (tid < cols ? tid : cols - 1)];
-                sy[tid] = (tid < cols) ? v : T(0);
-            }
-            __syncthreads();
+            // each thread reads the x entries of its own columns (the same
+            // addresses across the BS threads of a column group: one
+            // broadcast request), so a tile needs no LDS staging or barrier
             const int64_t row = r0 + (it < rows ? it : rows - 1);
             const T* mp = M + row + c0 * ld;
             #pragma unroll 16
             for (int c = 0; c < KP; ++c) {
                 const int cc = ih * KP + c;
-                T a = mp[int64_t(cc < cols ? cc : cols - 1) * ld];
-                acc_cm += a * sy[cc];                 // sy is zero past cols
+                const int ccl = cc < cols ? cc : cols - 1;
+                const T a = mp[int64_t(ccl) * ld];
+                const T xv = X[c0 + ccl];
+                acc_cm += (cc < cols) ? a * xv : T(0);
             }
-            __syncthreads();
         }
         else {
             // lanes run along the columns (BS/64 per lane), the wave over RPW rows

@@ -100,7 +100,7 @@ def test_sgetrf_split_updates(gpu):
     assert f32 <= 1e-3
     assert be[slate_amd.PREC_BF16X3] <= 3 * f32          # float-accurate update
     assert f32 < be[slate_amd.PREC_BF16X2] <= 2000 * f32  # ~16-bit products
-    assert be[slate_amd.PREC_BF16X2] < be[slate_amd.PREC_BF16] <= 0.5
+    assert be[slate_amd.PREC_BF16X2] < be[slate_amd.PREC_BF16] <= 4.0   # one bf16 term: O(1) in max norm
 
 
 @pytest.mark.parametrize("prec", [slate_amd.PREC_BF16X3, slate_amd.PREC_BF16X2, slate_amd.PREC_BF16])

@@ -51,6 +51,6 @@ def test_pocondest_matches_lapack_dpocon(gpu):
     anorm = np.abs(S0).sum(axis=0).max()
     rc = slate_amd.pocondest(S, anorm)
     c, _ = sl.lapack.dpotrf(S0, lower=1)
-    rc_ref, _ = sl.lapack.dpocon(c, anorm, lower=1)
+    rc_ref, _ = sl.lapack.dpocon(c, anorm, uplo="L")
     say("pocondest %.6g  LAPACK dpocon %.6g" % (rc, rc_ref))
     assert abs(rc - rc_ref) <= 1e-6 * rc_ref
