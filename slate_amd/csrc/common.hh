@@ -112,6 +112,13 @@ __device__ inline void tw_gather(const uint64_t* p, const int64_t (&off)[N], con
 }
 
 template <class T> struct TwWords { static constexpr int n = sizeof(T) / 4; };
+// A value from consecutive 32-bit payload words (as gathered into LDS).
+template <class T>
+__device__ inline T lds_word_val(const uint32_t* w)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, (uint64_t(w[1]) << 32) | w[0]);
+    else return __builtin_bit_cast(T, w[0]);
+}
 template <class T>
 __device__ inline void tw_store_val(uint64_t* p, T v, uint32_t tag)
 {

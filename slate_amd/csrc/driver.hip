@@ -46,6 +46,8 @@ struct Context {
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
+    Buf tvw;                     // trsv look-back exchange words (+ progress word)
+    uint32_t tv_tag = 0;         // next free tag of tvw
     bool calu = false;           // leaves use tournament pivoting (MethodLU::CALU)
     // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
     std::vector<std::pair<int64_t, int64_t>> nb_table[4];
@@ -377,37 +379,57 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     T* tmp1 = c.stmp1.as<T>();
 
     // Float factor with split-BF16 updates (factor_precision BF16, BF16X2,
-    // BF16X3): the L panel of each step is split once on s1 into one of two
-    // alternating slots (s0 is done with slot k%2 before s1 refills it at
-    // step k+2), and each U row block right before its update.
+    // BF16X3).  With 16-byte aligned rows the update reads the float L panel
+    // directly and a transposed float copy of the U row block (k contiguous),
+    // splitting both into bf16 terms inside the GEMM (gemm_s32).  Otherwise
+    // the L panel of each step is split once on s1 into one of two
+    // alternating bf16-plane slots (s0 is done with slot k%2 before s1
+    // refills it at step k+2), and each U row block right before its update.
     const int ns = std::is_same<T, float>::value ? split_terms(o.prec) : 0;
     const int64_t Kp = ceildiv(nb, 32) * 32;
+    const bool direct = ns > 0 && ld % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0;
     const int64_t lplane = n * Kp, uplane0 = n * Kp, uplane1 = nb * Kp;
     if (ns > 0) {
-        c.bfl.ensure(sizeof(__bf16) * size_t(2 * ns) * size_t(lplane));
-        c.bfu0.ensure(sizeof(__bf16) * size_t(ns) * size_t(uplane0));
-        c.bfu1.ensure(sizeof(__bf16) * size_t(ns) * size_t(uplane1));
+        const size_t ub = direct ? sizeof(float) : sizeof(__bf16) * size_t(ns);
+        if (!direct) c.bfl.ensure(sizeof(__bf16) * size_t(2 * ns) * size_t(lplane));
+        c.bfu0.ensure(ub * size_t(uplane0));
+        c.bfu1.ensure(ub * size_t(uplane1));
     }
     auto lslot = [&](int64_t k) { return c.bfl.as<__bf16>() + (k % 2) * ns * lplane; };
     auto split_panel = [&](int64_t k, int64_t k0, int64_t kb, hipStream_t st) {
         if constexpr (std::is_same<T, float>::value) {
-            if (ns > 0)
+            if (ns > 0 && !direct)
                 split_rows(ns, A + (k0 + kb) * ld + k0, ld, n - k0 - kb, kb, lslot(k), Kp, lplane, st);
         }
     };
     // Update of columns [cb, cb + ncols) by step k (rows below the panel):
     // C -= L21 U12, U12 already multiplied by L11^{-1}.
+    // role 1 (the trailing update on s0) is the one timed by the profiling
+    // events, around the GEMM launch alone.
     auto update = [&](int64_t k, int64_t k0, int64_t kb, int64_t cb, int64_t ncols, bool lookahead_block,
                       hipStream_t st, int role) {
         const int64_t c1 = k0 + kb;
         if (ncols <= 0 || n - c1 <= 0) return;
+        const double flops = 2.0 * double(n - c1) * double(ncols) * double(kb);
+        auto timed = [&](auto&& launch) {
+            if (role == 1) prof_begin(c, st);
+            launch();
+            if (role == 1) prof_end(c, st, flops);
+        };
         if constexpr (std::is_same<T, float>::value) {
+            if (ns > 0 && direct) {
+                float* ut = lookahead_block ? c.bfu1.as<float>() : c.bfu0.as<float>();
+                transpose_f32(A + k0 * ld + cb, ld, kb, ncols, ut, Kp, st);
+                timed([&] { gemm_s32(ns, n - c1, ncols, kb, A + c1 * ld + k0, ld, ut, Kp, A + c1 * ld + cb, ld,
+                                     -1.0f, st, role, 0); });
+                return;
+            }
             if (ns > 0) {
                 __bf16* uh = lookahead_block ? c.bfu1.as<__bf16>() : c.bfu0.as<__bf16>();
                 const int64_t up = lookahead_block ? uplane1 : uplane0;
                 split_cols_t(ns, A + k0 * ld + cb, ld, kb, ncols, uh, Kp, up, st);
-                gemm_bf16s(ns, n - c1, ncols, Kp, lslot(k), Kp, lplane, uh, Kp, up, A + c1 * ld + cb, ld,
-                           -1.0f, st, role);
+                timed([&] { gemm_bf16s(ns, n - c1, ncols, Kp, lslot(k), Kp, lplane, uh, Kp, up,
+                                       A + c1 * ld + cb, ld, -1.0f, st, role); });
                 return;
             }
         }
@@ -417,7 +439,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         g.B = A + k0 * ld + cb; g.ldb = ld;
         g.C = A + c1 * ld + cb; g.ldc = ld;
         g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
-        gemm<T>(g, A_KC, B_NC, st, role);
+        timed([&] { gemm<T>(g, A_KC, B_NC, st, role); });
     };
     const int64_t plan_stride = 4 * nb + 8;
     c.plans.ensure(sizeof(int64_t) * size_t(plan_stride) * size_t(nsteps + 1));
@@ -474,18 +496,14 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             const int64_t c2 = c1 + nb1;
             swap(c2, n, k0, c1, pk, s0);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c2, ld, n - c2, tmp0, s0);
-            prof_begin(c, s0);
             update(k, k0, kb, c2, n - c2, false, s0, 1);
-            prof_end(c, s0, 2.0 * double(r) * double(n - c2) * double(kb));
             SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
         }
         else {
             // no lookahead: whole trailing matrix on s0, then the next panel
             swap(c1, n, k0, c1, pk, s0);
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, r, tmp0, s0);
-            prof_begin(c, s0);
             update(k, k0, kb, c1, r, false, s0, 1);
-            prof_end(c, s0, 2.0 * double(r) * double(r) * double(kb));
             if (la) {
                 SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
                 SA_HIP(hipStreamWaitEvent(s1, ev_trail(k + 1), 0));
@@ -539,7 +557,7 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     // of the triangular update C -= U12^T U12.
     const int ns = std::is_same<T, float>::value ? split_terms(o.prec) : 0;
     const int64_t Kp = ceildiv(nb, 32) * 32, plane = n * Kp;
-    if (ns > 0) c.bfu0.ensure(sizeof(__bf16) * size_t(ns) * size_t(plane));
+    if (ns > 0) c.bfu0.ensure(std::max(sizeof(__bf16) * size_t(ns), sizeof(float)) * size_t(plane));
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
         int64_t kb = std::min(nb, n - k0);
         T* Akk = A + k0 * ld + k0;
@@ -549,10 +567,10 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         trsm_wide<T>(c, Akk, ld, 1, 1, 0, kb, Akk + kb, ld, r, s);
         if constexpr (std::is_same<T, float>::value) {
             if (ns > 0) {
-                __bf16* u = c.bfu0.as<__bf16>();
-                split_cols_t(ns, Akk + kb, ld, kb, r, u, Kp, plane, s);
-                gemm_bf16s(ns, r, r, Kp, u, Kp, plane, u, Kp, plane, A + (k0 + kb) * ld + k0 + kb, ld,
-                           -1.0f, s, 0, 1);
+                // both operands are U12^T (r x kb): one transposed copy
+                float* ut = c.bfu0.as<float>();
+                transpose_f32(Akk + kb, ld, kb, r, ut, Kp, s);
+                gemm_s32(ns, r, r, kb, ut, Kp, ut, Kp, A + (k0 + kb) * ld + k0 + kb, ld, -1.0f, s, 0, 1);
                 continue;
             }
         }
@@ -601,6 +619,26 @@ static int* err_flag(Context& c)
     return c.err.as<int>();
 }
 
+// Exchange words for one trsv_lookback launch over n rows: a fresh tag per
+// launch; the words are zeroed when (re)allocated and when the tags wrap.
+static TvSync tv_next(Context& c, int64_t n)
+{
+    const size_t words = size_t(2 * std::max<int64_t>(n, 1)) + 32;
+    const bool fresh = c.tvw.bytes < words * sizeof(uint64_t);
+    if (fresh || c.tv_tag == 0 || c.tv_tag > 0xFFFFFF00u) {
+        c.tvw.ensure(words * sizeof(uint64_t));
+        zero_u32(c.tvw.as<unsigned>(), int64_t(c.tvw.bytes / sizeof(unsigned)), c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        c.tv_tag = 1;
+    }
+    TvSync t;
+    t.words = c.tvw.as<uint64_t>();
+    t.progress = t.words + (c.tvw.bytes / sizeof(uint64_t) - 16);
+    t.error = err_flag(c);
+    t.tag = c.tv_tag++;
+    return t;
+}
+
 // Single right-hand side: x := op(LU)^{-1} x with x contiguous, using the
 // one-launch look-back triangular solves.  perm is the getrf permutation.
 // Workspace for the two inverted-diagonal-block sets of a triangular pair.
@@ -619,8 +657,6 @@ template <class T>
 static void getrs_vec(Context& c, const T* LU, int64_t ld, int cm, int64_t n, const int64_t* perm,
                       T* x, T* tmp, int trans, hipStream_t s)
 {
-    unsigned* ctr = tv_counter(c);
-    int* err = err_flag(c);
     const int64_t bs = trsv_block(int(sizeof(T)));
     T* d0 = dinv_pair<T>(c, n);
     T* d1 = d0 + ceildiv(n, bs) * bs * bs;
@@ -628,15 +664,15 @@ static void getrs_vec(Context& c, const T* LU, int64_t ld, int cm, int64_t n, co
         trsv_diag_inverse<T>(LU, ld, cm, 1, 1, n, d0, s);
         trsv_diag_inverse<T>(LU, ld, cm, 0, 0, n, d1, s);
         permute_convert<T, T>(x, tmp, perm, n, 0, s);
-        trsv_lookback<T>(LU, ld, cm, 1, n, d0, tmp, ctr, err, s);
-        trsv_lookback<T>(LU, ld, cm, 0, n, d1, tmp, ctr, err, s);
+        trsv_lookback<T>(LU, ld, cm, 1, n, d0, tmp, tv_next(c, n), s);
+        trsv_lookback<T>(LU, ld, cm, 0, n, d1, tmp, tv_next(c, n), s);
         copy_convert<T, T>(tmp, n, x, n, n, 1, s);
     }
     else {
         trsv_diag_inverse<T>(LU, ld, 1 - cm, 1, 0, n, d0, s);
         trsv_diag_inverse<T>(LU, ld, 1 - cm, 0, 1, n, d1, s);
-        trsv_lookback<T>(LU, ld, 1 - cm, 1, n, d0, x, ctr, err, s);
-        trsv_lookback<T>(LU, ld, 1 - cm, 0, n, d1, x, ctr, err, s);
+        trsv_lookback<T>(LU, ld, 1 - cm, 1, n, d0, x, tv_next(c, n), s);
+        trsv_lookback<T>(LU, ld, 1 - cm, 0, n, d1, x, tv_next(c, n), s);
         permute_convert<T, T>(x, tmp, perm, n, 1, s);
         copy_convert<T, T>(tmp, n, x, n, n, 1, s);
     }
@@ -793,8 +829,6 @@ template <class T>
 static void potrs_cm(Context& c, int uplo, T* A, int64_t lda, int64_t n, T* B, int64_t ldb, int64_t nrhs)
 {
     if (nrhs == 1) {
-        unsigned* ctr = tv_counter(c);
-        int* err = err_flag(c);
         // Lower column-major L (A = L L^T): L y = b, L^T x = y.
         // Upper column-major U (A = U^T U): U^T y = b, U x = y.
         const int cmL = (uplo == SLATE_AMD_UPPER) ? 0 : 1;
@@ -803,8 +837,8 @@ static void potrs_cm(Context& c, int uplo, T* A, int64_t lda, int64_t n, T* B, i
         T* d1 = d0 + ceildiv(n, bs) * bs * bs;
         trsv_diag_inverse<T>(A, lda, cmL, 1, 0, n, d0, c.s0);
         trsv_diag_inverse<T>(A, lda, 1 - cmL, 0, 0, n, d1, c.s0);
-        trsv_lookback<T>(A, lda, cmL, 1, n, d0, B, ctr, err, c.s0);
-        trsv_lookback<T>(A, lda, 1 - cmL, 0, n, d1, B, ctr, err, c.s0);
+        trsv_lookback<T>(A, lda, cmL, 1, n, d0, B, tv_next(c, n), c.s0);
+        trsv_lookback<T>(A, lda, 1 - cmL, 0, n, d1, B, tv_next(c, n), c.s0);
         SA_HIP(hipStreamSynchronize(c.s0));
         check_kernel_error(c);
         return;
@@ -876,17 +910,15 @@ static void lo_solve(Context& c, const MixedState& st, int sym, int64_t n, int64
 static void lo_solve_vec(Context& c, const MixedState& st, int sym, int64_t n, const double* src,
                          double* dst, bool accumulate)
 {
-    unsigned* ctr = tv_counter(c);
-    int* err = err_flag(c);
     if (sym) {
         copy_convert<double, float>(src, n, st.xlo, n, n, 1, c.s0);
-        trsv_lookback<float>(st.Alo, n, 1, 1, n, st.dinvL, st.xlo, ctr, err, c.s0);   // U^T y = b
-        trsv_lookback<float>(st.Alo, n, 0, 0, n, st.dinvU, st.xlo, ctr, err, c.s0);   // U x = y
+        trsv_lookback<float>(st.Alo, n, 1, 1, n, st.dinvL, st.xlo, tv_next(c, n), c.s0);   // U^T y = b
+        trsv_lookback<float>(st.Alo, n, 0, 0, n, st.dinvU, st.xlo, tv_next(c, n), c.s0);   // U x = y
     }
     else {
         permute_convert<double, float>(src, st.xlo, st.perm, n, 0, c.s0);
-        trsv_lookback<float>(st.Alo, n, 0, 1, n, st.dinvL, st.xlo, ctr, err, c.s0);   // L y = P b
-        trsv_lookback<float>(st.Alo, n, 0, 0, n, st.dinvU, st.xlo, ctr, err, c.s0);   // U x = y
+        trsv_lookback<float>(st.Alo, n, 0, 1, n, st.dinvL, st.xlo, tv_next(c, n), c.s0);   // L y = P b
+        trsv_lookback<float>(st.Alo, n, 0, 0, n, st.dinvU, st.xlo, tv_next(c, n), c.s0);   // U x = y
     }
     if (accumulate) add_convert<float>(st.xlo, dst, n, c.s0);
     else copy_convert<float, double>(st.xlo, n, dst, n, n, 1, c.s0);
@@ -1424,12 +1456,21 @@ static int gemm_bf16_entry(slate_amd_context ctx, int terms, int64_t m, int64_t 
         sa::Timer t(c);
         const int64_t Kp = sa::ceildiv(k, 32) * 32;
         const int64_t pa = n * Kp, pb = m * Kp;
-        c.bfg.ensure(sizeof(__bf16) * size_t(terms) * size_t(pa + pb));
-        __bf16* a = c.bfg.as<__bf16>();
-        __bf16* b = a + terms * pa;
-        sa::split_rows(terms, B, ldb, n, k, a, Kp, pa, c.s0);
-        sa::split_cols_t(terms, A, lda, k, m, b, Kp, pb, c.s0);
-        sa::gemm_bf16s(terms, n, m, Kp, a, Kp, pa, b, Kp, pb, C, ldc, alpha, c.s0, 0, 0);
+        if (k >= 4 && ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0) {
+            // B^T (n x k) is k-contiguous in place; A^T needs a transposed copy
+            c.bfg.ensure(sizeof(float) * size_t(pb));
+            float* at = c.bfg.as<float>();
+            sa::transpose_f32(A, lda, k, m, at, Kp, c.s0);
+            sa::gemm_s32(terms, n, m, k, B, ldb, at, Kp, C, ldc, alpha, c.s0);
+        }
+        else {
+            c.bfg.ensure(sizeof(__bf16) * size_t(terms) * size_t(pa + pb));
+            __bf16* a = c.bfg.as<__bf16>();
+            __bf16* b = a + terms * pa;
+            sa::split_rows(terms, B, ldb, n, k, a, Kp, pa, c.s0);
+            sa::split_cols_t(terms, A, lda, k, m, b, Kp, pb, c.s0);
+            sa::gemm_bf16s(terms, n, m, Kp, a, Kp, pa, b, Kp, pb, C, ldc, alpha, c.s0, 0, 0);
+        }
         t.stop();
         return 0;
     }));

@@ -30,7 +30,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // Splitting.  Each thread handles 8 consecutive k of one row (one 16-byte
 // store per plane).
 template <int NS>
-__device__ inline void split8(const float (&x)[8], bf16x8 (&h)[NS])
+__device__ inline void split8(const float (&x)[8], bf16x8 (&h)[NS > 0 ? NS : 1])
 {
     #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -57,7 +57,7 @@ void split_rows_kernel(const float* __restrict__ src, int64_t ld, int64_t M, int
         float x[8];
         #pragma unroll
         for (int e = 0; e < 8; ++e) x[e] = (k0 + e < K) ? src[m * ld + k0 + e] : 0.0f;
-        bf16x8 h[NS];
+        bf16x8 h[NS > 0 ? NS : 1];
         split8<NS>(x, h);
         #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -90,11 +90,19 @@ void split_cols_t_kernel(const float* __restrict__ src, int64_t ld, int64_t K, i
         float x[8];
         #pragma unroll
         for (int e = 0; e < 8; ++e) x[e] = t[kg + e][nn];
-        bf16x8 h[NS];
-        split8<NS>(x, h);
-        #pragma unroll
-        for (int s = 0; s < NS; ++s)
-            *reinterpret_cast<bf16x8*>(dst + s * plane + n * Kp + k0 + kg) = h[s];
+        if constexpr (NS == 0) {
+            // plain transposed float copy (dst is float, plane unused)
+            float* d = reinterpret_cast<float*>(dst) + n * Kp + k0 + kg;
+            *reinterpret_cast<v4f*>(d) = v4f{x[0], x[1], x[2], x[3]};
+            *reinterpret_cast<v4f*>(d + 4) = v4f{x[4], x[5], x[6], x[7]};
+        }
+        else {
+            bf16x8 h[NS > 0 ? NS : 1];
+            split8<NS>(x, h);
+            #pragma unroll
+            for (int s = 0; s < NS; ++s)
+                *reinterpret_cast<bf16x8*>(dst + s * plane + n * Kp + k0 + kg) = h[s];
+        }
     }
 }
 
@@ -220,6 +228,217 @@ void gemm_bf16s_kernel(int64_t M, int64_t N, int64_t Kp, const __bf16* __restric
                     *c += alpha * acc[i][j][r];
                 }
             }
+}
+
+//------------------------------------------------------------------------------
+// The same update from float operands split inside the kernel: C (row-major)
+// += alpha * L R^T, L: M x K and R: N x K, both float with k contiguous, with
+// no pre-split planes (half the operand traffic of three bf16 planes and no
+// separate split pass over the L panel).  Each float element is split once per
+// workgroup, by the thread that stages it into LDS, rather than by every wave
+// that reads it (each A row is read by two waves, each B row by two).  LDS
+// holds NS bf16 planes per operand, a 128 x 32 tile each as 128 rows of four
+// 16-byte chunks (8 consecutive k); chunk c of row r sits in slot
+// c ^ ((r >> 2) & 3), so the 16 rows of a fragment read and the 4 rows x 4
+// chunks of a staging write both cover all 64 banks.  One buffer (16 KB per
+// term), refilled between two barriers while the next slice's global loads are
+// in flight; MINB workgroups share a CU to cover the barrier gaps.
+
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// 8 floats -> NS packed bf16x8 terms (round to nearest, residual carried).
+template <int NS>
+__device__ inline void split8_packed(float (&r)[8], u32x4 (&h)[NS])
+{
+    #pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        #pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bf16x2_t b = __builtin_convertvector((v2f_t{r[2 * e], r[2 * e + 1]}), bf16x2_t);
+            const unsigned u = __builtin_bit_cast(unsigned, b);
+            h[s][e] = u;
+            if (s + 1 < NS) {
+                r[2 * e] -= __uint_as_float(u << 16);
+                r[2 * e + 1] -= __uint_as_float(u & 0xffff0000u);
+            }
+        }
+    }
+}
+
+template <int NS, int ROLE, int MINB>
+__global__ __launch_bounds__(256, MINB)
+void gemm_sp_kernel(int64_t M, int64_t N, int64_t K, const float* __restrict__ L, int64_t ldl,
+                    const float* __restrict__ R, int64_t ldr, float* __restrict__ C, int64_t ldc, float alpha,
+                    int uplo)
+{
+    constexpr int BM = 128, BN = 128, BK = 32;
+    __shared__ u32x4 As[NS][BM * 4];
+    __shared__ u32x4 Bs[NS][BN * 4];
+
+    const int64_t tiles_m = ceildiv(M, BM), tiles_n = ceildiv(N, BN);
+    const int64_t ntiles = tiles_m * tiles_n;
+    int64_t t = xcd_remap(blockIdx.x, ntiles);
+    constexpr int64_t GM = 8;
+    int64_t group = t / (GM * tiles_n);
+    int64_t first_m = group * GM;
+    int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
+    int64_t within = t - group * GM * tiles_n;
+    const int64_t m0 = (first_m + within % gm) * BM;
+    const int64_t n0 = (within / gm) * BN;
+    if (uplo && n0 + BN <= m0) return;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // staging: rows lr and lr + 64 of each operand tile, k chunk lc (8 floats
+    // as two float4); rows past M (N) clamped (not stored), k past K zeroed
+    const int lr = tid >> 2, lc = tid & 3;
+    int64_t aoff[2], boff[2];
+    #pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int64_t r = m0 + lr + 64 * h, c = n0 + lr + 64 * h;
+        aoff[h] = (r < M ? r : M - 1) * ldl;
+        boff[h] = (c < N ? c : N - 1) * ldr;
+    }
+    v4f ra[2][2], rb[2][2];
+    auto load = [&](int64_t k0) {
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int64_t k = k0 + lc * 8 + q * 4;
+            const bool full = k + 4 <= K;
+            const int64_t kc = full ? k : (K >= 4 ? K - 4 : 0);
+            #pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                v4f a = *reinterpret_cast<const v4f*>(L + aoff[h] + kc);
+                v4f b = *reinterpret_cast<const v4f*>(R + boff[h] + kc);
+                if (!full) {
+                    #pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int64_t ke = k + e;
+                        const int src = int(ke - kc);
+                        float av = 0.f, bv = 0.f;
+                        #pragma unroll
+                        for (int w = 0; w < 4; ++w) if (w == src) { av = a[w]; bv = b[w]; }
+                        a[e] = (ke < K) ? av : 0.f;
+                        b[e] = (ke < K) ? bv : 0.f;
+                    }
+                }
+                ra[h][q] = a;
+                rb[h][q] = b;
+            }
+        }
+    };
+    auto stage = [&]() {
+        #pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = lr + 64 * h;
+            const int slot = row * 4 + (lc ^ ((row >> 2) & 3));
+            float x[8] = {ra[h][0][0], ra[h][0][1], ra[h][0][2], ra[h][0][3],
+                          ra[h][1][0], ra[h][1][1], ra[h][1][2], ra[h][1][3]};
+            u32x4 p[NS];
+            split8_packed<NS>(x, p);
+            #pragma unroll
+            for (int s = 0; s < NS; ++s) As[s][slot] = p[s];
+            float y[8] = {rb[h][0][0], rb[h][0][1], rb[h][0][2], rb[h][0][3],
+                          rb[h][1][0], rb[h][1][1], rb[h][1][2], rb[h][1][3]};
+            split8_packed<NS>(y, p);
+            #pragma unroll
+            for (int s = 0; s < NS; ++s) Bs[s][slot] = p[s];
+        }
+    };
+
+    v4f acc[4][4];
+    #pragma unroll
+    for (int i = 0; i < 4; ++i)
+        #pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    const int fr = lane & 15, fslot = (lane >> 4) ^ ((fr >> 2) & 3);
+    auto compute = [&]() {
+        bf16x8 bs[4][NS];
+        #pragma unroll
+        for (int j = 0; j < 4; ++j)
+            #pragma unroll
+            for (int s = 0; s < NS; ++s)
+                bs[j][s] = __builtin_bit_cast(bf16x8, Bs[s][(wn * 64 + j * 16 + fr) * 4 + fslot]);
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            bf16x8 as[NS];
+            #pragma unroll
+            for (int s = 0; s < NS; ++s)
+                as[s] = __builtin_bit_cast(bf16x8, As[s][(wm * 64 + i * 16 + fr) * 4 + fslot]);
+            #pragma unroll
+            for (int sa_ = NS - 1; sa_ >= 0; --sa_)
+                #pragma unroll
+                for (int sb = NS - 1 - sa_; sb >= 0; --sb)
+                    #pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as[sa_], bs[j][sb], acc[i][j], 0, 0, 0);
+        }
+    };
+
+    const int64_t ktiles = ceildiv(K, BK);
+    load(0);
+    stage();
+    __syncthreads();
+    for (int64_t kt = 0; kt < ktiles; ++kt) {
+        const bool more = kt + 1 < ktiles;
+        if (more) load((kt + 1) * BK);
+        compute();
+        if (more) {
+            __syncthreads();
+            stage();
+            __syncthreads();
+        }
+    }
+
+    #pragma unroll
+    for (int i = 0; i < 4; ++i)
+        #pragma unroll
+        for (int j = 0; j < 4; ++j)
+            #pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gm_ = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const int64_t gn_ = n0 + wn * 64 + j * 16 + fr;
+                if (gm_ < M && gn_ < N && (uplo == 0 || gn_ >= gm_)) {
+                    float* c = C + gm_ * ldc + gn_;
+                    *c += alpha * acc[i][j][r];
+                }
+            }
+}
+
+void transpose_f32(const float* src, int64_t ld, int64_t K, int64_t N, float* dst, int64_t Kp, hipStream_t s)
+{
+    if (N <= 0) return;
+    SA_REQUIRE(Kp % 32 == 0 && Kp >= K, -2, "transpose_f32: bad Kp");
+    dim3 grid(unsigned(ceildiv(N, 64)), unsigned(Kp / 32));
+    hipLaunchKernelGGL(split_cols_t_kernel<0>, grid, dim3(256), 0, s, src, ld, K, N,
+                       reinterpret_cast<__bf16*>(dst), Kp, int64_t(0));
+    SA_CHECK_LAUNCH();
+}
+
+void gemm_s32(int ns, int64_t M, int64_t N, int64_t K, const float* L, int64_t ldl, const float* R, int64_t ldr,
+              float* C, int64_t ldc, float alpha, hipStream_t s, int role, int uplo)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    SA_REQUIRE(ldl % 4 == 0 && ldr % 4 == 0 && reinterpret_cast<uintptr_t>(L) % 16 == 0
+               && reinterpret_cast<uintptr_t>(R) % 16 == 0, -2, "gemm_s32: operands must be 16-byte aligned");
+    SA_REQUIRE(K >= 4, -2, "gemm_s32: K must be at least 4");
+    int64_t nt = ceildiv(M, 128) * ceildiv(N, 128);
+    SA_REQUIRE(nt < (int64_t(1) << 31), -2, "gemm_s32: grid too large");
+    // split at staging, three workgroups per CU (48 KB of LDS each at NS = 3)
+#define SA_S32(NSV, RV) hipLaunchKernelGGL((gemm_sp_kernel<NSV, RV, 3>), dim3(unsigned(nt)), dim3(256), 0, s, \
+                                           M, N, K, L, ldl, R, ldr, C, ldc, alpha, uplo)
+    if (role == 1) {
+        switch (ns) { case 1: SA_S32(1, 1); break; case 2: SA_S32(2, 1); break; case 3: SA_S32(3, 1); break;
+                      default: SA_REQUIRE(false, -2, "gemm_s32: ns must be 1..3"); }
+    }
+    else {
+        switch (ns) { case 1: SA_S32(1, 0); break; case 2: SA_S32(2, 0); break; case 3: SA_S32(3, 0); break;
+                      default: SA_REQUIRE(false, -2, "gemm_s32: ns must be 1..3"); }
+    }
+#undef SA_S32
+    SA_CHECK_LAUNCH();
 }
 
 //------------------------------------------------------------------------------

@@ -64,6 +64,14 @@ void batched_solve(int elem_bytes, int count, const int* order, const int* n, co
                    int* iters, int* info, double* berr, int* queue, char* work, int grid, int itermax,
                    double tol, hipStream_t s);
 
+// gemm_s32: C (fp32 row-major) += alpha * L R^T with float L (M x K) and
+// R (N x K), k contiguous, rows 16-byte aligned; each operand split into ns
+// bf16 terms inside the kernel.  transpose_f32: K x N row-major float ->
+// N x Kp (zero padded past K).
+void gemm_s32(int ns, int64_t M, int64_t N, int64_t K, const float* L, int64_t ldl, const float* R, int64_t ldr,
+              float* C, int64_t ldc, float alpha, hipStream_t s, int role = 0, int uplo = 0);
+void transpose_f32(const float* src, int64_t ld, int64_t K, int64_t N, float* dst, int64_t Kp, hipStream_t s);
+
 // Workspace shared by the cross-workgroup panel kernel.
 struct PanelWork {
     void* slots;                 // tagged candidate words exchanged between workgroups
@@ -114,18 +122,25 @@ void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
                T* B, int64_t ldb, int64_t N, hipStream_t s);
 
 // One-launch triangular solve of tri * x = b (one right-hand side, x overwrites
-// b, contiguous) with a completion-counter chain between blocks of
-// trsv_block() rows.  Dinv holds the inverted diagonal blocks made by
-// trsv_diag_inverse (ceil(n/bs) * bs^2 elements; the unit/non-unit choice is
-// folded into it).  ndone: one counter, zeroed on launch; error: set to 2 if
-// a wait times out.
+// b, contiguous) with a look-back chain between blocks of trsv_block() rows.
+// Dinv holds the inverted diagonal blocks made by trsv_diag_inverse
+// (ceil(n/bs) * bs^2 elements; the unit/non-unit choice is folded into it).
+// TvSync: words = 2n tagged exchange words, progress = one tagged word, both
+// zeroed when allocated; tag = nonzero and unused by earlier launches on those
+// words; error set to 2 if a wait times out.
+struct TvSync {
+    uint64_t* words;
+    uint64_t* progress;
+    int* error;
+    uint32_t tag;
+};
 int64_t trsv_block(int elem_bytes);
 template <class T>
 void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* Dinv,
                        hipStream_t s);
 template <class T>
 void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T* Dinv, T* X,
-                   unsigned* ndone, int* error, hipStream_t s);
+                   const TvSync& sync, hipStream_t s);
 
 // Cholesky leaf on an nb x nb row-major block (nb <= 64), upper factor U with
 // A = U^T U stored in the upper triangle.  Sets *info = col0 + j + 1 at the

@@ -116,14 +116,19 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
         // (1) local candidate: first row i >= j of maximal |a_ij|
         T babs = T(-1);
         int64_t bidx = INT64_MAX;
+        // Column selects are written as value selects (v_cndmask), not as
+        // statements: with the column index uniform, statement form compiles
+        // to one scalar branch per register column.
         #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
             T x = T(0);
             #pragma unroll
-            for (int c = 0; c < W; ++c) if (c == j) x = v[q][c];
+            for (int c = 0; c < W; ++c) x = (c == j) ? v[q][c] : x;
             T a = fabs(x);
-            if (i >= j && i < m && a > babs) { babs = a; bidx = i; }
+            const bool take = i >= j && i < m && a > babs;
+            babs = take ? a : babs;
+            bidx = take ? i : bidx;
         }
         #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -245,16 +250,16 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
         #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
-            if (i > j && i < m) {
-                T l = T(0);
-                #pragma unroll
-                for (int c = 0; c < W; ++c) if (c == j) l = v[q][c];
-                if (pv != T(0)) l = tiny ? l / pv : l * inv;
-                #pragma unroll
-                for (int c = 0; c < W; ++c) {
-                    if (c == j) v[q][c] = l;
-                    else if (c > j) v[q][c] -= l * s_piv[c];
-                }
+            const bool below = i > j && i < m;
+            T l = T(0);
+            #pragma unroll
+            for (int c = 0; c < W; ++c) l = (c == j) ? v[q][c] : l;
+            if (pv != T(0)) l = tiny ? l / pv : l * inv;
+            l = below ? l : T(0);
+            #pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const T u = v[q][c] - l * s_piv[c];
+                v[q][c] = below ? ((c == j) ? l : ((c > j) ? u : v[q][c])) : v[q][c];
             }
         }
         // No barrier here: the next writes of s_wabs, s_words, s_piv and
@@ -286,6 +291,279 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
     }
 }
 
+//------------------------------------------------------------------------------
+// Leaf with rotated register columns.  v[q][c] holds column (j + c) mod W at
+// step j, so the active column is always v[q][0] and every other access uses
+// compile-time register indices (no per-element column selects); one
+// circular shift per step keeps the invariant.  Rows are published and kept
+// in LDS in the same rotated order.  The exchange is in two stages: every
+// workgroup gathers only the G (|a|, row) heads and row j (G*(NWV+1) + W*NWV
+// words, about one per thread), decides the pivot, and then wave 0 reads the
+// winner's row, which the winner published in the same round (no second read
+// when the diagonal is kept).
+template <class T, int W, int RPT, int VEC>
+__global__ __launch_bounds__(LEAF_THREADS)
+void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0,
+                           int w, int64_t* __restrict__ ipiv, int64_t* __restrict__ info, T threshold,
+                           uint32_t tag0, uint64_t* slots, int* error)
+{
+    using SL = LeafSlots<T, W>;
+    constexpr int NWV = SL::NWV, CW = SL::CW;
+    constexpr int HW = NWV + 1;                 // head words of an entry: |a|, row
+    constexpr int AMAX = LEAF_GMAX * HW + W * NWV;
+    constexpr int NCR = (AMAX + LEAF_THREADS - 1) / LEAF_THREADS;
+    static_assert((W & (W - 1)) == 0 && W % 4 == 0, "W must be a power of two");
+    typedef T vec4_t __attribute__((ext_vector_type(4)));
+    const int G = gridDim.x;
+    const int g = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t m = m_total - r0;
+    constexpr int64_t R = int64_t(LEAF_THREADS) * RPT;
+    const int64_t gbase = int64_t(g) * R;
+    const int totalA = G * HW + W * NWV;
+    const int64_t per = SL::per(G);
+
+    __shared__ T s_wabs[4];
+    __shared__ uint32_t s_widx[4];
+    __shared__ uint32_t s_words[AMAX];
+    __shared__ __attribute__((aligned(16))) T s_rowj[W];   // row j, rotated
+    __shared__ __attribute__((aligned(16))) T s_piv[W];    // pivot row, rotated
+    __shared__ __attribute__((aligned(16))) T s_prot[W];   // pivot row, rotated, 0 outside columns j+1 .. w-1
+    __shared__ int64_t s_p;
+
+    constexpr int VE = 16 / int(sizeof(T));
+    T v[RPT][W];
+    #pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+        const T* rowp = A + (r0 + (i < m ? i : 0)) * ld + c0;
+        if (VEC && w == W) {
+            #pragma unroll
+            for (int c = 0; c < W; c += VE) {
+                typedef T vec_t __attribute__((ext_vector_type(VE)));
+                vec_t x = *reinterpret_cast<const vec_t*>(rowp + c);
+                #pragma unroll
+                for (int e = 0; e < VE; ++e) v[q][c + e] = (i < m) ? x[e] : T(0);
+            }
+        }
+        else {
+            #pragma unroll
+            for (int c = 0; c < W; ++c)
+                v[q][c] = (i < m && c < w) ? rowp[c] : T(0);
+        }
+    }
+    const T safe_min = std::numeric_limits<T>::min();
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    auto better = [](T a, uint32_t ia, T b, uint32_t ib) { return a > b || (a == b && ia < ib); };
+
+    #pragma unroll 1
+    for (int j = 0; j < w; ++j) {
+        uint64_t* buf = slots + (j & 1) * per;
+        const uint32_t tag = tag0 + uint32_t(j);
+
+        // (1) local candidate: first row i >= j of maximal |a_ij|
+        T babs = T(-1);
+        uint32_t bidx = NONE;
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            const T a = fabs(v[q][0]);
+            const bool take = i >= j && i < m && a > babs;
+            babs = take ? a : babs;
+            bidx = take ? uint32_t(i) : bidx;
+        }
+        #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const T oa = __shfl_xor(babs, o, 64);
+            const uint32_t oi = __shfl_xor(bidx, o, 64);
+            const bool b = better(oa, oi, babs, bidx);
+            babs = b ? oa : babs;
+            bidx = b ? oi : bidx;
+        }
+        if (lane == 0) { s_wabs[wave] = babs; s_widx[wave] = bidx; }
+        __syncthreads();
+        babs = s_wabs[0]; bidx = s_widx[0];
+        #pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            const bool b = better(s_wabs[k], s_widx[k], babs, bidx);
+            babs = b ? s_wabs[k] : babs;
+            bidx = b ? s_widx[k] : bidx;
+        }
+
+        // (2) publish the head (|a|, row) and the candidate row and, from the
+        // owner of row j, row j (rows in rotated order)
+        uint64_t* ent = buf + int64_t(g) * CW;
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            if (bidx != NONE && i == int64_t(bidx)) {
+                #pragma unroll
+                for (int c = 0; c < W; ++c) tw_store_val<T>(ent + HW + c * NWV, v[q][c], tag);
+            }
+            if (i == j) {
+                #pragma unroll
+                for (int c = 0; c < W; ++c) tw_store_val<T>(buf + int64_t(G) * CW + c * NWV, v[q][c], tag);
+            }
+        }
+        if (tid == 0) {
+            tw_store_val<T>(ent, babs, tag);
+            tw_store(ent + NWV, bidx, tag);
+        }
+
+        // (3a) gather the heads of all workgroups and row j
+        {
+            int64_t off[NCR];
+            bool val[NCR];
+            uint32_t got[NCR];
+            #pragma unroll
+            for (int u = 0; u < NCR; ++u) {
+                const int e = tid + u * LEAF_THREADS;
+                const int ee = e < totalA ? e : totalA - 1;
+                val[u] = e < totalA;
+                off[u] = ee < G * HW ? int64_t(ee / HW) * CW + ee % HW : int64_t(G) * CW + (ee - G * HW);
+            }
+            tw_gather<NCR>(buf, off, val, tag, got, error, 1);
+            #pragma unroll
+            for (int u = 0; u < NCR; ++u) {
+                const int e = tid + u * LEAF_THREADS;
+                if (e < totalA) s_words[e] = got[u];
+            }
+        }
+        __syncthreads();
+        // (3b) decide; wave 0 fetches the winner's row unless the diagonal stays
+        if (wave == 0) {
+            T ba = T(-1);
+            uint32_t bi = NONE;
+            int bg = -1;
+            for (int e = lane; e < G; e += 64) {
+                const T a = lds_word_val<T>(&s_words[e * HW]);
+                const uint32_t x = s_words[e * HW + NWV];
+                const bool b = better(a, x, ba, bi);
+                ba = b ? a : ba;
+                bi = b ? x : bi;
+                bg = b ? e : bg;
+            }
+            #pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const T oa = __shfl_xor(ba, o, 64);
+                const uint32_t oi = __shfl_xor(bi, o, 64);
+                const int og = __shfl_xor(bg, o, 64);
+                const bool b = better(oa, oi, ba, bi);
+                ba = b ? oa : ba;
+                bi = b ? oi : bi;
+                bg = b ? og : bg;
+            }
+            const int lc = lane < W ? lane : 0;
+            const T rj = lds_word_val<T>(&s_words[G * HW + lc * NWV]);
+            const T ajj = lds_word_val<T>(&s_words[G * HW]);
+            // Tile_getrf.hh:283-289: keep the diagonal if it is large enough.
+            const bool diag = bi == NONE || bg < 0 || fabs(ajj) >= threshold * ba;
+            int64_t p = diag ? int64_t(j) : int64_t(bi);
+            if (p < j || p >= m) p = j;            // never trust a row outside the leaf
+            T pr = rj;
+            if (p != j) {
+                // the winner's row (published with its head)
+                int64_t off[1] = {int64_t(bg) * CW + HW + (lane < W * NWV ? lane : 0)};
+                bool val[1] = {lane < W * NWV};
+                uint32_t got[1];
+                tw_gather<1>(buf, off, val, tag, got, error, 1);
+                if (lane < W * NWV) s_words[lane] = got[0];   // the heads are consumed
+                __builtin_amdgcn_wave_barrier();
+                pr = lds_word_val<T>(&s_words[lc * NWV]);
+            }
+            if (lane < W) {
+                s_rowj[lane] = rj;
+                s_piv[lane] = pr;
+                s_prot[lane] = (lane >= 1 && lane < w - j) ? pr : T(0);
+            }
+            if (lane == 0) s_p = p;
+        }
+        __syncthreads();
+        const int64_t p = s_p;
+
+        // (4) interchange rows j and p among the rows this thread owns
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            if (i == j) {
+                #pragma unroll
+                for (int c = 0; c < W; c += 4) {
+                    const vec4_t x = *reinterpret_cast<const vec4_t*>(&s_piv[c]);
+                    v[q][c] = x[0]; v[q][c + 1] = x[1]; v[q][c + 2] = x[2]; v[q][c + 3] = x[3];
+                }
+            }
+            else if (i == p) {
+                #pragma unroll
+                for (int c = 0; c < W; c += 4) {
+                    const vec4_t x = *reinterpret_cast<const vec4_t*>(&s_rowj[c]);
+                    v[q][c] = x[0]; v[q][c + 1] = x[1]; v[q][c + 2] = x[2]; v[q][c + 3] = x[3];
+                }
+            }
+        }
+        const T pv = s_piv[0];
+        if (g == 0 && tid == 0) {
+            ipiv[c0 + j] = r0 + p;
+            if (pv == T(0) && *info == 0) *info = c0 + j + 1;
+        }
+        // (5) scale the column below the pivot and update columns j+1 .. w-1
+        // (rotated 1 .. w-j-1; the others see a zero in s_prot and are kept
+        // by the select, so finished columns stay bit-identical)
+        const bool tiny = fabs(pv) < safe_min;
+        const T inv = T(1) / pv;
+        const int lim = w - j;
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            if (i > j && i < m) {
+                T l = v[q][0];
+                if (pv != T(0)) l = tiny ? l / pv : l * inv;
+                v[q][0] = l;
+                #pragma unroll
+                for (int c = 0; c < W; c += 4) {
+                    const vec4_t x = *reinterpret_cast<const vec4_t*>(&s_prot[c]);
+                    #pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (c + e >= 1) v[q][c + e] = (c + e < lim) ? v[q][c + e] - l * x[e] : v[q][c + e];
+                }
+            }
+        }
+        // (6) rotate: column j + 1 becomes v[q][0]
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const T t0 = v[q][0];
+            #pragma unroll
+            for (int c = 0; c < W - 1; ++c) v[q][c] = v[q][c + 1];
+            v[q][W - 1] = t0;
+        }
+    }
+
+    // after w steps v[q][c] holds column (w + c) mod W
+    #pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+        if (i < m) {
+            T* rowp = A + (r0 + i) * ld + c0;
+            if (VEC && w == W) {
+                #pragma unroll
+                for (int c = 0; c < W; c += VE) {
+                    typedef T vec_t __attribute__((ext_vector_type(VE)));
+                    vec_t x;
+                    #pragma unroll
+                    for (int e = 0; e < VE; ++e) x[e] = v[q][c + e];
+                    *reinterpret_cast<vec_t*>(rowp + c) = x;
+                }
+            }
+            else {
+                #pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    const int col = (w + c) & (W - 1);
+                    if (col < w) rowp[col] = v[q][c];
+                }
+            }
+        }
+    }
+}
+
 template <class T>
 void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
                 int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
@@ -306,7 +584,10 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
     constexpr int VE = 16 / int(sizeof(T));
     const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (ld % VE == 0) && (c0 % VE == 0);
     uint64_t* slots = static_cast<uint64_t*>(pw.slots);
-#define SA_LEAF(R, V) hipLaunchKernelGGL((panel_leaf_kernel<T, W, R, V>), dim3(G), dim3(LEAF_THREADS), 0, s, \
+#ifndef SA_LEAF_KERNEL
+#define SA_LEAF_KERNEL panel_leaf_kernel
+#endif
+#define SA_LEAF(R, V) hipLaunchKernelGGL((SA_LEAF_KERNEL<T, W, R, V>), dim3(G), dim3(LEAF_THREADS), 0, s, \
                                           A, ld, r0, m_total, c0, w, ipiv, info, thr, tag0, slots, pw.error)
     if (vec) {
         switch (rpt) { case 1: SA_LEAF(1, 1); break; case 2: SA_LEAF(2, 1); break; default: SA_LEAF(4, 1); break; }

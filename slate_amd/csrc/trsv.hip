@@ -109,12 +109,29 @@ void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int6
 }
 
 //------------------------------------------------------------------------------
+// Look-back sweep.  Block q publishes its solved piece twice: plainly into X
+// (the result) and as tagged words (32-bit payload + the launch's tag) into
+// xw, which is what later blocks read: a reader that sees the tag has the
+// value, so there is no completion counter on the critical path and no
+// cache-wide fence.  A separate progress word (tag, blocks done) only paces
+// the waiting: blocks far from their turn sleep in proportion to the
+// distance instead of polling.
+//
+// Each dependency's BS x BS tile of the factor is loaded into registers
+// before its x arrives (it does not depend on x), so after the wait only the
+// x words, one LDS pass and the FMAs remain.
+template <class T>
+__device__ inline void tv_put(uint64_t* p, T v, uint32_t tag) { tw_store_val<T>(p, v, tag); }
+
 template <class T, int BS, int LOWER, int CM>
 __global__ __launch_bounds__(256)
 void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const T* __restrict__ Dinv,
-                          T* __restrict__ X, unsigned* ndone, int* error)
+                          T* __restrict__ X, uint64_t* __restrict__ xw, uint32_t tag,
+                          uint64_t* __restrict__ progress, int* error)
 {
+    constexpr int NWV = int(sizeof(T)) / 4;
     constexpr int RPW = BS / 4;                 // rows per wave (row-major product)
+    constexpr int CPL = BS / 64;                // columns per lane (row-major product)
     constexpr int NPART = 256 / BS;             // threads per row in the column-major product
     constexpr int KP = BS / NPART;              // columns per thread
     const int nblk = int(ceildiv(n, BS));
@@ -124,10 +141,10 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
     const int rows = int((n - r0 < BS) ? n - r0 : BS);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
+    __shared__ uint32_t sxw[2][BS * NWV];       // x words of the dependency being applied
     __shared__ T sr[BS];
     __shared__ T spart[NPART][BS];
     __shared__ T sred[4][RPW][65];
-    __shared__ unsigned s_ready;
 
     // Prefetch this block's inverse: thread (it, ih) holds inv(it, ih*KP .. ih*KP+KP-1).
     const int it = tid % BS, ih = tid / BS;
@@ -138,79 +155,96 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
         for (int k = 0; k < KP; ++k) dinv[k] = dp[int64_t(ih * KP + k) * BS + it];
     }
 
-    // ---- products with the already-solved blocks
     const int ndep = LOWER ? b : nblk - 1 - b;
     T acc_rm[RPW];
     T acc_cm = T(0);
     #pragma unroll
     for (int r = 0; r < RPW; ++r) acc_rm[r] = T(0);
 
-    unsigned ready = 0;                         // uniform private copy of s_ready
-    for (int t = 0; t < ndep; ++t) {
-        if (t >= int(ready)) {
-            if (tid == 0) {
-                uint64_t t0 = wall_clock64();
-                unsigned v;
-                while ((v = __hip_atomic_load(ndone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < unsigned(t + 1)) {
-                    // blocks far from their turn poll rarely, so the few that
-                    // are next see the counter move without contention
-                    if (unsigned(t + 1) - v > 2) __builtin_amdgcn_s_sleep(16);
-                    else __builtin_amdgcn_s_sleep(1);
-                    if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { v = unsigned(ndep); break; }
-                    if (wall_clock64() - t0 > 400000000ull) {
-                        __hip_atomic_store(error, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        v = unsigned(ndep);
-                        break;
-                    }
-                }
-                sa_publish_fence();
-                s_ready = v;
-            }
-            __syncthreads();
-            ready = s_ready;
-            __syncthreads();                    // s_ready is rewritten only after everyone read it
-        }
+    // Tile registers.  Loads use clamped addresses (edge values are zeroed
+    // when applied), so a tile's loads issue back to back.
+    T mrm[CM ? 1 : RPW][CM ? 1 : CPL];
+    T mcm[CM ? KP : 1];
+    auto load_tile = [&](int t) {
         const int j = LOWER ? t : nblk - 1 - t;
         const int64_t c0 = int64_t(j) * BS;
         const int cols = int((n - c0 < BS) ? n - c0 : BS);
-        // All loads below are unconditional with clamped addresses (edge
-        // values are zeroed afterwards), so the compiler issues a whole tile's
-        // loads back to back instead of one guarded round trip per element.
-        if (CM) {
-            // each thread reads the x entries of its own columns (the same
-            // addresses across the BS threads of a column group: one
-            // broadcast request), so a tile needs no LDS staging or barrier
+        if constexpr (CM) {
             const int64_t row = r0 + (it < rows ? it : rows - 1);
             const T* mp = M + row + c0 * ld;
-            #pragma unroll 16
+            #pragma unroll
             for (int c = 0; c < KP; ++c) {
                 const int cc = ih * KP + c;
-                const int ccl = cc < cols ? cc : cols - 1;
-                const T a = mp[int64_t(ccl) * ld];
-                const T xv = X[c0 + ccl];
-                acc_cm += (cc < cols) ? a * xv : T(0);
+                mcm[c] = mp[int64_t(cc < cols ? cc : cols - 1) * ld];
             }
         }
         else {
-            // lanes run along the columns (BS/64 per lane), the wave over RPW rows
-            constexpr int CPL = BS / 64;
-            T y[CPL];
-            int cl[CPL];
-            #pragma unroll
-            for (int u = 0; u < CPL; ++u) {
-                const int c = CPL * lane + u;
-                cl[u] = c < cols ? c : cols - 1;
-                T v = X[c0 + cl[u]];
-                y[u] = (c < cols) ? v : T(0);
-            }
             #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 const int i = wave * RPW + r;
                 const T* mp = M + (r0 + (i < rows ? i : rows - 1)) * ld + c0;
                 #pragma unroll
-                for (int u = 0; u < CPL; ++u) acc_rm[r] += mp[cl[u]] * y[u];
+                for (int u = 0; u < CPL; ++u) {
+                    const int c = CPL * lane + u;
+                    mrm[r][u] = mp[c < cols ? c : cols - 1];
+                }
             }
         }
+    };
+
+    const uint64_t t_start = wall_clock64();
+    bool failed = false;
+    if (ndep > 0) load_tile(0);
+    #pragma unroll 1
+    for (int t = 0; t < ndep; ++t) {
+        const int j = LOWER ? t : nblk - 1 - t;
+        const int64_t c0 = int64_t(j) * BS;
+        const int cols = int((n - c0 < BS) ? n - c0 : BS);
+        const int buf = t & 1;
+        // pace: sleep while the chain is more than two blocks away from t
+        if (tid == 0 && !failed) {
+            while (true) {
+                const uint64_t pw = tw_load(progress);
+                const unsigned done = (uint32_t(pw >> 32) == tag) ? uint32_t(pw) : 0u;
+                if (done + 2 >= unsigned(t + 1)) break;
+                const unsigned dist = unsigned(t + 1) - done;
+                const unsigned naps = dist > 16 ? 8 : dist / 2;
+                for (unsigned z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(32);
+                if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0
+                    || wall_clock64() - t_start > 400000000ull) {
+                    __hip_atomic_store(error, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        // the dependency's x words (bounded spin inside tw_wait)
+        if (tid < BS * NWV) {
+            uint32_t wv = 0;
+            if (tid < cols * NWV) wv = tw_wait(xw + c0 * NWV + tid, tag, error, 2);
+            sxw[buf][tid] = wv;
+        }
+        __syncthreads();
+        if (CM) {
+            #pragma unroll
+            for (int c = 0; c < KP; ++c) {
+                const int cc = ih * KP + c;
+                const T xv = lds_word_val<T>(&sxw[buf][cc * NWV]);
+                acc_cm += (cc < cols) ? mcm[c] * xv : T(0);
+            }
+        }
+        else {
+            #pragma unroll
+            for (int u = 0; u < CPL; ++u) {
+                const int c = CPL * lane + u;
+                const T xv = lds_word_val<T>(&sxw[buf][c * NWV]);
+                const T y = (c < cols) ? xv : T(0);
+                #pragma unroll
+                for (int r = 0; r < RPW; ++r) acc_rm[r] += mrm[r][u] * y;
+            }
+        }
+        if (t + 1 < ndep) load_tile(t + 1);
+        failed = failed || (tid == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0);
     }
     // ---- residual of this block's rows
     if (CM) {
@@ -247,32 +281,31 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
         spart[ih][it] = acc;
     }
     __syncthreads();
-    if (tid < BS) {
+    if (tid < rows) {
         T yv = T(0);
         #pragma unroll
         for (int p = 0; p < NPART; ++p) yv += spart[p][tid];
-        if (tid < rows) X[r0 + tid] = yv;
+        tv_put<T>(xw + (r0 + tid) * NWV, yv, tag);
+        X[r0 + tid] = yv;
     }
-    __syncthreads();
-    if (tid == 0) {
-        sa_publish_fence();
-        __hip_atomic_store(ndone, unsigned(q + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (tid == 0)
+        __hip_atomic_fetch_max(progress, (uint64_t(tag) << 32) | uint32_t(q + 1), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class T>
 void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T* Dinv, T* X,
-                   unsigned* ndone, int* error, hipStream_t s)
+                   const TvSync& sync, hipStream_t s)
 {
     if (n <= 0) return;
     constexpr int BS = TvBS<T>::v;
-    zero_u32(ndone, 1, s);
+    SA_REQUIRE(sync.tag != 0, -2, "trsv_lookback: tag 0 is reserved");
     unsigned nblk = unsigned(ceildiv(n, BS));
     int key = (lower ? 2 : 0) | (cm ? 1 : 0);
 #define SA_TV_CASE(K, LO, CMV)                                                                      \
     case K:                                                                                         \
         hipLaunchKernelGGL((trsv_lookback_kernel<T, BS, LO, CMV>), dim3(nblk), dim3(256), 0, s,    \
-                           M, ld, n, Dinv, X, ndone, error);                                        \
+                           M, ld, n, Dinv, X, sync.words, sync.tag, sync.progress, sync.error);    \
         break;
     switch (key) {
         SA_TV_CASE(0, 0, 0)
@@ -288,8 +321,8 @@ void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T
 template void trsv_diag_inverse<double>(const double*, int64_t, int, int, int, int64_t, double*, hipStream_t);
 template void trsv_diag_inverse<float>(const float*, int64_t, int, int, int, int64_t, float*, hipStream_t);
 template void trsv_lookback<double>(const double*, int64_t, int, int, int64_t, const double*, double*,
-                                    unsigned*, int*, hipStream_t);
+                                    const TvSync&, hipStream_t);
 template void trsv_lookback<float>(const float*, int64_t, int, int, int64_t, const float*, float*,
-                                   unsigned*, int*, hipStream_t);
+                                   const TvSync&, hipStream_t);
 
 } // namespace sa
