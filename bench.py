@@ -1,9 +1,11 @@
 """Benchmark: FP64-accurate gesv TFLOP/s, mixed precision + iterative refinement
 vs plain FP64 LU, n = 65536, one MI355X.
 
-One step = one slate_amd.gesv_mixed call (float LU factorization with the
-MFMA trailing updates + FP64 refinement to SLATE's stopping criterion) on a
-matrix already resident in HBM.  TFLOP/s counts FP64-equivalent LU flops
+One step = one slate_amd.gesv_mixed call (float LU factorization + FP64
+refinement to SLATE's stopping criterion) on a matrix already resident in
+HBM.  By default the float factorization's trailing updates run as
+three-term split-BF16 products (FP32-accurate, factor_precision BF16X3) on
+v_mfma_f32_16x16x32_bf16; --factor-precision fp32 uses FP32 MFMA instead.  TFLOP/s counts FP64-equivalent LU flops
 (2 n^3 / 3), whatever precision executes them.  The same input is also solved
 once with the plain FP64 gesv for the "vs plain FP64 LU" comparison, and the
 oracle's CPU restatement of the HostTask path is timed on a bounded sample.
@@ -37,6 +39,27 @@ def scaled_residual_dev(A, x, b):
     r = b - A @ x
     a1 = A.abs().sum(dim=0).max()
     return float(r.abs().sum() / (n * a1 * x.abs().sum()) / torch.finfo(torch.float64).eps)
+
+
+def measured_traffic(kernel_key, factor_precision):
+    """HBM bytes per launch of the trailing-update kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "trailing_update_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None, None
+    fields = {}
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#") or not line.strip():
+                continue
+            k, _, v = line.strip().partition(" ")
+            fields[k] = v.strip()
+    if fields.get("kernel") != kernel_key or fields.get("factor_precision") != factor_precision:
+        return None, None
+    try:
+        return float(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    except (KeyError, ValueError):
+        return None, None
 
 
 def cpu_baseline(n_cpu):
@@ -113,7 +136,7 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--nb", type=int, default=0)
     ap.add_argument("--cpu-n", type=int, default=12288)
-    ap.add_argument("--factor-precision", default="fp32", choices=sorted(PRECS),
+    ap.add_argument("--factor-precision", default="bf16x3", choices=sorted(PRECS),
                     help="factor precision of the mixed solve (bf16x3 = float-accurate split-BF16)")
     ap.add_argument("--gmres", action="store_true", help="GMRES-IR instead of classic IR")
     ap.add_argument("--workload", default="gesv", choices=["gesv", "batched"],
@@ -204,16 +227,22 @@ def main():
     mfma_flops = gemm_flops * (products if products else 1)
     achieved = mfma_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     if products:
-        peak, kname = PEAK["bf16"], ("gemm_bf16s_kernel<%d,1> trailing update (v_mfma_f32_16x16x32_bf16, "
-                                     "%d products per update)" % ({1: 1, 3: 2, 6: 3}[products], products))
+        peak, kname = PEAK["bf16"], ("gemm_sp_kernel<%d,1,3> trailing update (float operands split into bf16 "
+                                     "terms at LDS staging, v_mfma_f32_16x16x32_bf16, %d products per update)"
+                                     % ({1: 1, 3: 2, 6: 3}[products], products))
     else:
-        peak, kname = PEAK["fp32"], "gemm_kernel<float,0,0,128,128,32,2,2,2,1> trailing update (v_mfma_f32_16x16x4f32)"
+        peak, kname = PEAK["fp32"], "gemm_kernel<float,0,0,128,128,32,2,2,2,1,1> trailing update (v_mfma_f32_16x16x4f32)"
+    kernel_key = kname.split(" ")[0]
+    traffic, traffic_src = measured_traffic(kernel_key, args.factor_precision) if n == 65536 else (None, None)
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-        "frac": round(achieved / peak, 4), "traffic": None,
+        "frac": round(achieved / peak, 4), "traffic": traffic,
         "kernel": kname,
         "launches": int(launches), "avg_launch_ms": round(gemm_ms / max(launches, 1), 4),
+        "algorithmic_flops_per_launch": round(gemm_flops / max(launches, 1), 1),
     }
+    if traffic_src:
+        roofline["traffic_source"] = traffic_src
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,

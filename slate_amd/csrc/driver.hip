@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -38,6 +39,9 @@ struct Context {
     int device = 0;
     hipStream_t s0 = nullptr;          // trailing updates, solves
     hipStream_t s1 = nullptr;          // panel + lookahead column
+    hipStream_t s2 = nullptr;          // trailing updates on a CU subset (when a panel reserve is set)
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    int64_t panel_switch[2] = {0, 0};  // trailing size below which s2 takes the updates (float, double)
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
     Buf bfl, bfu0, bfu1, bfg;    // split-BF16 operand planes (LU update, gemm entry)
@@ -354,6 +358,14 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             c.evs.push_back(e);
         }
     }
+    // With a panel reserve the trailing updates run on s2, whose CU mask
+    // leaves the reserved CUs to the panel stream: a trailing GEMM of
+    // thousands of workgroups otherwise holds every CU slot and the panel's
+    // small launches (and the leaf, whose workgroups must be co-resident)
+    // wait for slots at every step.
+    // The switch happens once, when the trailing size drops below
+    // c.panel_switch (the steps where the panel, not the update, is the
+    // critical path).
     hipStream_t s0 = c.s0, s1 = la ? c.s1 : c.s0;
     // s1 must see the counter reset done on s0
     if (la) {
@@ -476,6 +488,11 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         const int64_t k0 = k * nb, kb = std::min(nb, n - k0);
         const int64_t c1 = k0 + kb;                   // first column after the panel
         const int64_t r = n - c1;                     // trailing size
+        if (la && c.s2 && s0 == c.s0 && r < c.panel_switch[sizeof(T) == 8 ? 1 : 0]) {
+            SA_HIP(hipEventRecord(c.ev_in, s0));
+            SA_HIP(hipStreamWaitEvent(c.s2, c.ev_in, 0));
+            s0 = c.s2;
+        }
         SA_HIP(hipStreamWaitEvent(s0, ev_panel(k), 0));
         // pivots of step k applied to the columns on the left (finished L) on s0
         int64_t* pk = step_plan(k);
@@ -517,6 +534,10 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     if (la) {
         SA_HIP(hipEventRecord(c.evs[1], s1));
         SA_HIP(hipStreamWaitEvent(s0, c.evs[1], 0));
+    }
+    if (s0 != c.s0) {
+        SA_HIP(hipEventRecord(c.ev_out, s0));
+        SA_HIP(hipStreamWaitEvent(c.s0, c.ev_out, 0));
     }
     if (c.prof) {
         SA_HIP(hipStreamSynchronize(s0));
@@ -1559,6 +1580,32 @@ int slate_amd_init(slate_amd_context* out, int device)
         h->c.device = device;
         SA_HIP(hipStreamCreateWithFlags(&h->c.s0, hipStreamNonBlocking));
         SA_HIP(hipStreamCreateWithFlags(&h->c.s1, hipStreamNonBlocking));
+        SA_HIP(hipEventCreateWithFlags(&h->c.ev_in, hipEventDisableTiming));
+        SA_HIP(hipEventCreateWithFlags(&h->c.ev_out, hipEventDisableTiming));
+        // Panel reserve: the float LU's trailing updates run on s2, whose CU
+        // mask leaves `reserve` CUs (64 of 256 by default) to the panel
+        // stream.  With split-BF16 updates the panel is the critical path and
+        // the reserve shortens the n = 65536 factorization by about a quarter;
+        // the FP64 LU is update-bound and keeps every CU (switch 0).
+        // SLATE_AMD_PANEL_CUS overrides the reserve (0 = none) and
+        // SLATE_AMD_PANEL_SWITCH the trailing size below which both
+        // precisions use it.
+        {
+            hipDeviceProp_t prop;
+            SA_HIP(hipGetDeviceProperties(&prop, device));
+            const int ncu = prop.multiProcessorCount;
+            int reserve = ncu / 4;
+            if (const char* e = std::getenv("SLATE_AMD_PANEL_CUS")) reserve = std::atoi(e);
+            if (reserve > 0 && reserve < ncu) {
+                std::vector<uint32_t> mask(size_t((ncu + 31) / 32), 0u);
+                for (int cu = reserve; cu < ncu; ++cu) mask[size_t(cu / 32)] |= 1u << (cu % 32);
+                SA_HIP(hipExtStreamCreateWithCUMask(&h->c.s2, uint32_t(mask.size()), mask.data()));
+                h->c.panel_switch[0] = std::numeric_limits<int64_t>::max();
+                h->c.panel_switch[1] = 0;
+                if (const char* sw = std::getenv("SLATE_AMD_PANEL_SWITCH"))
+                    h->c.panel_switch[0] = h->c.panel_switch[1] = std::atoll(sw);
+            }
+        }
         SA_HIP(hipEventCreate(&h->c.t_start));
         SA_HIP(hipEventCreate(&h->c.t_stop));
         *out = h;
@@ -1575,13 +1622,17 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
-                       &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv})
+                       &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
+                       &c.tvw})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);
     (void)hipEventDestroy(c.t_stop);
+    (void)hipEventDestroy(c.ev_in);
+    (void)hipEventDestroy(c.ev_out);
     (void)hipStreamDestroy(c.s0);
     (void)hipStreamDestroy(c.s1);
+    if (c.s2) (void)hipStreamDestroy(c.s2);
     delete ctx;
     return 0;
 }

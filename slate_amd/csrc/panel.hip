@@ -7,9 +7,10 @@
 // of rows in registers.  Per column, each workgroup finds its local pivot
 // candidate with wavefront shuffles and publishes it (value, row, and the row's
 // W entries) as tagged words to a double-buffered exchange area; every
-// workgroup reads all G candidates (one memory round trip once they have
-// landed, with no grid barrier and no fences), reduces them itself, applies
-// the interchange to the rows it owns, scales and performs the rank-1 update.
+// workgroup reads the G (value, row) heads (one memory round trip once they
+// have landed, with no grid barrier and no fences), reduces them itself,
+// fetches the winning row, applies the interchange to the rows it owns, scales
+// and performs the rank-1 update.
 // Pivot rules follow src/internal/Tile_getrf.hh:197-366: largest |a_ij| with
 // the smallest row winning exact ties, the diagonal kept when
 // |a_jj| >= threshold * max, scaling by the reciprocal unless the pivot is
@@ -20,9 +21,7 @@
 
 namespace sa {
 
-static constexpr int LEAF_THREADS = 256;
 static constexpr int LEAF_GMAX = 128;     // workgroups per leaf launch
-static constexpr int LEAF_RPT_MAX = 4;    // rows per thread
 
 template <class T> struct LeafW;
 template <> struct LeafW<double> { static constexpr int W = 16; };
@@ -57,240 +56,6 @@ __device__ inline bool better(T a, int64_t ia, T b, int64_t ib)
     return a > b || (a == b && ia < ib);
 }
 
-template <class T, int W, int RPT, int VEC>
-__global__ __launch_bounds__(LEAF_THREADS)
-void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0,
-                       int w, int64_t* __restrict__ ipiv, int64_t* __restrict__ info, T threshold,
-                       uint32_t tag0, uint64_t* slots, int* error)
-{
-    using SL = LeafSlots<T, W>;
-    constexpr int NWV = SL::NWV, CW = SL::CW;
-    const int G = gridDim.x;
-    const int g = blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t m = m_total - r0;
-    constexpr int64_t R = int64_t(LEAF_THREADS) * RPT;
-    const int64_t gbase = int64_t(g) * R;
-    const int total = G * CW + W * NWV;     // words exchanged per column
-    const int64_t per = SL::per(G);
-
-    __shared__ T s_wabs[4];
-    __shared__ int64_t s_widx[4];
-    __shared__ uint32_t s_words[SL::MAXWORDS];
-    __shared__ T s_rowj[W];
-    __shared__ T s_piv[W];
-    __shared__ int64_t s_p;
-
-    // Each thread's rows are read with 16-byte vector loads when the leaf is
-    // aligned (VEC), so a row segment costs W*sizeof(T)/16 requests.
-    constexpr int VE = 16 / int(sizeof(T));
-    T v[RPT][W];
-    #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
-        const T* rowp = A + (r0 + (i < m ? i : 0)) * ld + c0;
-        if (VEC && w == W) {
-            #pragma unroll
-            for (int c = 0; c < W; c += VE) {
-                typedef T vec_t __attribute__((ext_vector_type(VE)));
-                vec_t x = *reinterpret_cast<const vec_t*>(rowp + c);
-                #pragma unroll
-                for (int e = 0; e < VE; ++e) v[q][c + e] = (i < m) ? x[e] : T(0);
-            }
-        }
-        else {
-            #pragma unroll
-            for (int c = 0; c < W; ++c)
-                v[q][c] = (i < m && c < w) ? rowp[c] : T(0);
-        }
-    }
-    const T safe_min = std::numeric_limits<T>::min();
-
-    // The column loop stays rolled; register columns are addressed with
-    // compile-time indices under predicates so v[][] never spills.
-    #pragma unroll 1
-    for (int j = 0; j < w; ++j) {
-        uint64_t* buf = slots + (j & 1) * per;
-        const uint32_t tag = tag0 + uint32_t(j);
-
-        // (1) local candidate: first row i >= j of maximal |a_ij|
-        T babs = T(-1);
-        int64_t bidx = INT64_MAX;
-        // Column selects are written as value selects (v_cndmask), not as
-        // statements: with the column index uniform, statement form compiles
-        // to one scalar branch per register column.
-        #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
-            T x = T(0);
-            #pragma unroll
-            for (int c = 0; c < W; ++c) x = (c == j) ? v[q][c] : x;
-            T a = fabs(x);
-            const bool take = i >= j && i < m && a > babs;
-            babs = take ? a : babs;
-            bidx = take ? i : bidx;
-        }
-        #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            T oa = __shfl_xor(babs, o, 64);
-            int64_t oi = __shfl_xor(bidx, o, 64);
-            if (better(oa, oi, babs, bidx)) { babs = oa; bidx = oi; }
-        }
-        if (lane == 0) { s_wabs[wave] = babs; s_widx[wave] = bidx; }
-        __syncthreads();
-        babs = s_wabs[0]; bidx = s_widx[0];
-        #pragma unroll
-        for (int k = 1; k < 4; ++k)
-            if (better(s_wabs[k], s_widx[k], babs, bidx)) { babs = s_wabs[k]; bidx = s_widx[k]; }
-
-        // (2) publish the candidate (|a|, row, row values) and, from the owner
-        // of row j, row j itself, all as tagged words: no fence, no barrier.
-        uint64_t* ent = buf + int64_t(g) * CW;
-        if (bidx != INT64_MAX && bidx >= gbase && bidx < gbase + R
-            && (bidx - gbase) % LEAF_THREADS == tid) {
-            int qq = int((bidx - gbase) / LEAF_THREADS);
-            #pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (q == qq) {
-                    #pragma unroll
-                    for (int c = 0; c < W; ++c) tw_store_val<T>(ent + NWV + 1 + c * NWV, v[q][c], tag);
-                }
-        }
-        if (int64_t(j) >= gbase && int64_t(j) < gbase + R && (j - gbase) % LEAF_THREADS == tid) {
-            int qj = int((j - gbase) / LEAF_THREADS);
-            #pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (q == qj) {
-                    #pragma unroll
-                    for (int c = 0; c < W; ++c) tw_store_val<T>(buf + int64_t(G) * CW + c * NWV, v[q][c], tag);
-                }
-        }
-        if (bidx == INT64_MAX && tid < W)        // no candidate here: tagged zeros
-            tw_store_val<T>(ent + NWV + 1 + tid * NWV, T(0), tag);
-        if (tid == 0) {
-            tw_store_val<T>(ent, babs, tag);
-            tw_store(ent + NWV, bidx == INT64_MAX ? 0xFFFFFFFFu : uint32_t(bidx), tag);
-        }
-
-        // (3) gather every word of this column: all loads issued at once
-        // (fixed trip count, clamped addresses) and re-polled together until
-        // every tag has arrived (one round trip per poll).
-        {
-            constexpr int NCR = (SL::MAXWORDS + LEAF_THREADS - 1) / LEAF_THREADS;
-            int64_t off[NCR];
-            bool val[NCR];
-            uint32_t got[NCR];
-            #pragma unroll
-            for (int u = 0; u < NCR; ++u) {
-                const int e = tid + u * LEAF_THREADS;
-                val[u] = e < total;
-                off[u] = e < total ? e : total - 1;
-            }
-            tw_gather<NCR>(buf, off, val, tag, got, error, 1);
-            #pragma unroll
-            for (int u = 0; u < NCR; ++u) {
-                const int e = tid + u * LEAF_THREADS;
-                if (e < total) s_words[e] = got[u];
-            }
-        }
-        __syncthreads();
-        if (wave == 0) {
-            T ba = T(-1);
-            int64_t bi = INT64_MAX;
-            int bg = -1;
-            for (int e = lane; e < G; e += 64) {
-                T a = lds_val<T>(&s_words[e * CW]);
-                uint32_t x = s_words[e * CW + NWV];
-                int64_t xi = (x == 0xFFFFFFFFu) ? INT64_MAX : int64_t(x);
-                if (better(a, xi, ba, bi)) { ba = a; bi = xi; bg = e; }
-            }
-            #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                T oa = __shfl_xor(ba, o, 64);
-                int64_t oi = __shfl_xor(bi, o, 64);
-                int og = __shfl_xor(bg, o, 64);
-                if (better(oa, oi, ba, bi)) { ba = oa; bi = oi; bg = og; }
-            }
-            const T rj = lds_val<T>(&s_words[G * CW + (lane < W ? lane : 0) * NWV]);
-            const T ajj = lds_val<T>(&s_words[G * CW + j * NWV]);
-            // Tile_getrf.hh:283-289: keep the diagonal if it is large enough.
-            bool diag = !(bi != INT64_MAX) || bg < 0 || fabs(ajj) >= threshold * ba;
-            int64_t p = diag ? int64_t(j) : bi;
-            if (p < j || p >= m) p = j;            // never trust a row outside the leaf
-            if (lane < W) {
-                s_rowj[lane] = rj;
-                s_piv[lane] = diag ? rj : lds_val<T>(&s_words[(bg < 0 ? 0 : bg) * CW + NWV + 1 + lane * NWV]);
-            }
-            if (lane == 0) s_p = p;
-        }
-        __syncthreads();
-        const int64_t p = s_p;
-
-        // (4) interchange rows j and p among the rows this thread owns
-        #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
-            if (i == j) {
-                #pragma unroll
-                for (int c = 0; c < W; ++c) v[q][c] = s_piv[c];
-            }
-            else if (i == p) {
-                #pragma unroll
-                for (int c = 0; c < W; ++c) v[q][c] = s_rowj[c];
-            }
-        }
-        const T pv = s_piv[j];
-        if (g == 0 && tid == 0) {
-            ipiv[c0 + j] = r0 + p;
-            if (pv == T(0) && *info == 0) *info = c0 + j + 1;
-        }
-        // (5) scale the column below the pivot and update the leaf to the right
-        const bool tiny = fabs(pv) < safe_min;
-        const T inv = T(1) / pv;
-        #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
-            const bool below = i > j && i < m;
-            T l = T(0);
-            #pragma unroll
-            for (int c = 0; c < W; ++c) l = (c == j) ? v[q][c] : l;
-            if (pv != T(0)) l = tiny ? l / pv : l * inv;
-            l = below ? l : T(0);
-            #pragma unroll
-            for (int c = 0; c < W; ++c) {
-                const T u = v[q][c] - l * s_piv[c];
-                v[q][c] = below ? ((c == j) ? l : ((c > j) ? u : v[q][c])) : v[q][c];
-            }
-        }
-        // No barrier here: the next writes of s_wabs, s_words, s_piv and
-        // s_rowj all come after barriers that every wave reaches only once it
-        // has finished this column.
-    }
-
-    #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
-        if (i < m) {
-            T* rowp = A + (r0 + i) * ld + c0;
-            if (VEC && w == W) {
-                #pragma unroll
-                for (int c = 0; c < W; c += VE) {
-                    typedef T vec_t __attribute__((ext_vector_type(VE)));
-                    vec_t x;
-                    #pragma unroll
-                    for (int e = 0; e < VE; ++e) x[e] = v[q][c + e];
-                    *reinterpret_cast<vec_t*>(rowp + c) = x;
-                }
-            }
-            else {
-                #pragma unroll
-                for (int c = 0; c < W; ++c)
-                    if (c < w) rowp[c] = v[q][c];
-            }
-        }
-    }
-}
-
 //------------------------------------------------------------------------------
 // Leaf with rotated register columns.  v[q][c] holds column (j + c) mod W at
 // step j, so the active column is always v[q][0] and every other access uses
@@ -301,8 +66,8 @@ void panel_leaf_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_tota
 // words, about one per thread), decides the pivot, and then wave 0 reads the
 // winner's row, which the winner published in the same round (no second read
 // when the diagonal is kept).
-template <class T, int W, int RPT, int VEC>
-__global__ __launch_bounds__(LEAF_THREADS)
+template <class T, int W, int RPT, int VEC, int NT>
+__global__ __launch_bounds__(NT)
 void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0,
                            int w, int64_t* __restrict__ ipiv, int64_t* __restrict__ info, T threshold,
                            uint32_t tag0, uint64_t* slots, int* error)
@@ -311,20 +76,21 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
     constexpr int NWV = SL::NWV, CW = SL::CW;
     constexpr int HW = NWV + 1;                 // head words of an entry: |a|, row
     constexpr int AMAX = LEAF_GMAX * HW + W * NWV;
-    constexpr int NCR = (AMAX + LEAF_THREADS - 1) / LEAF_THREADS;
+    constexpr int NCR = (AMAX + NT - 1) / NT;
     static_assert((W & (W - 1)) == 0 && W % 4 == 0, "W must be a power of two");
     typedef T vec4_t __attribute__((ext_vector_type(4)));
     const int G = gridDim.x;
     const int g = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t m = m_total - r0;
-    constexpr int64_t R = int64_t(LEAF_THREADS) * RPT;
+    constexpr int64_t R = int64_t(NT) * RPT;
     const int64_t gbase = int64_t(g) * R;
     const int totalA = G * HW + W * NWV;
     const int64_t per = SL::per(G);
 
-    __shared__ T s_wabs[4];
-    __shared__ uint32_t s_widx[4];
+    constexpr int NW = NT / 64;                 // waves per workgroup
+    __shared__ T s_wabs[NW];
+    __shared__ uint32_t s_widx[NW];
     __shared__ uint32_t s_words[AMAX];
     __shared__ __attribute__((aligned(16))) T s_rowj[W];   // row j, rotated
     __shared__ __attribute__((aligned(16))) T s_piv[W];    // pivot row, rotated
@@ -335,7 +101,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
     T v[RPT][W];
     #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+        int64_t i = gbase + tid + int64_t(NT) * q;
         const T* rowp = A + (r0 + (i < m ? i : 0)) * ld + c0;
         if (VEC && w == W) {
             #pragma unroll
@@ -366,7 +132,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
         uint32_t bidx = NONE;
         #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            const int64_t i = gbase + tid + int64_t(NT) * q;
             const T a = fabs(v[q][0]);
             const bool take = i >= j && i < m && a > babs;
             babs = take ? a : babs;
@@ -384,7 +150,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
         __syncthreads();
         babs = s_wabs[0]; bidx = s_widx[0];
         #pragma unroll
-        for (int k = 1; k < 4; ++k) {
+        for (int k = 1; k < NW; ++k) {
             const bool b = better(s_wabs[k], s_widx[k], babs, bidx);
             babs = b ? s_wabs[k] : babs;
             bidx = b ? s_widx[k] : bidx;
@@ -395,7 +161,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
         uint64_t* ent = buf + int64_t(g) * CW;
         #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            const int64_t i = gbase + tid + int64_t(NT) * q;
             if (bidx != NONE && i == int64_t(bidx)) {
                 #pragma unroll
                 for (int c = 0; c < W; ++c) tw_store_val<T>(ent + HW + c * NWV, v[q][c], tag);
@@ -417,7 +183,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
             uint32_t got[NCR];
             #pragma unroll
             for (int u = 0; u < NCR; ++u) {
-                const int e = tid + u * LEAF_THREADS;
+                const int e = tid + u * NT;
                 const int ee = e < totalA ? e : totalA - 1;
                 val[u] = e < totalA;
                 off[u] = ee < G * HW ? int64_t(ee / HW) * CW + ee % HW : int64_t(G) * CW + (ee - G * HW);
@@ -425,7 +191,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
             tw_gather<NCR>(buf, off, val, tag, got, error, 1);
             #pragma unroll
             for (int u = 0; u < NCR; ++u) {
-                const int e = tid + u * LEAF_THREADS;
+                const int e = tid + u * NT;
                 if (e < totalA) s_words[e] = got[u];
             }
         }
@@ -484,7 +250,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
         // (4) interchange rows j and p among the rows this thread owns
         #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            const int64_t i = gbase + tid + int64_t(NT) * q;
             if (i == j) {
                 #pragma unroll
                 for (int c = 0; c < W; c += 4) {
@@ -513,7 +279,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
         const int lim = w - j;
         #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+            const int64_t i = gbase + tid + int64_t(NT) * q;
             if (i > j && i < m) {
                 T l = v[q][0];
                 if (pv != T(0)) l = tiny ? l / pv : l * inv;
@@ -540,7 +306,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
     // after w steps v[q][c] holds column (w + c) mod W
     #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        int64_t i = gbase + tid + int64_t(LEAF_THREADS) * q;
+        int64_t i = gbase + tid + int64_t(NT) * q;
         if (i < m) {
             T* rowp = A + (r0 + i) * ld + c0;
             if (VEC && w == W) {
@@ -564,6 +330,19 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
     }
 }
 
+// Leaf shape (NT threads, RPT rows each): the fewest rows per thread that
+// keep the grid at 64 workgroups or fewer — 256 x 1 up to m = 16384, 512 x 1
+// up to 32768, 512 x 2 beyond (measured per column at m = 65536: 256 x 4
+// 7.8 us, 512 x 2 7.1 us, 1024 x 1 7.5 us, 1024 x 2 22.7 us for FP32 W = 32;
+// fewer, fatter workgroups lose more in their own barriers and updates than
+// they save in gathered heads).
+static void leaf_shape(int64_t m, int& nt, int& rpt)
+{
+    if (m <= 64 * 256) { nt = 256; rpt = 1; }
+    else if (m <= 64 * 512) { nt = 512; rpt = 1; }
+    else { nt = 512; rpt = 2; }
+}
+
 template <class T>
 void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
                 int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
@@ -574,9 +353,9 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
     SA_REQUIRE(w >= 1 && w <= W, -2, "panel_leaf: bad width");
     SA_REQUIRE(m >= w, -2, "panel_leaf: fewer rows than columns");
     SA_REQUIRE(m < int64_t(0xFFFFFFFFu), -2, "panel_leaf: panel too tall");
-    int rpt = 1;
-    while (rpt < LEAF_RPT_MAX && ceildiv(m, int64_t(LEAF_THREADS) * rpt) > 32) rpt *= 2;
-    int64_t G = ceildiv(m, int64_t(LEAF_THREADS) * rpt);
+    int nt = 256, rpt = 1;
+    leaf_shape(m, nt, rpt);
+    int64_t G = ceildiv(m, int64_t(nt) * rpt);
     SA_REQUIRE(G <= LEAF_GMAX, -2, "panel_leaf: panel too tall");
     const int64_t slot_bytes = int64_t(2 * sizeof(uint64_t)) * LeafSlots<T, W>::per(int(G));
     SA_REQUIRE(slot_bytes <= pw.slots_bytes, -2, "panel_leaf: slot area too small");
@@ -584,17 +363,13 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
     constexpr int VE = 16 / int(sizeof(T));
     const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (ld % VE == 0) && (c0 % VE == 0);
     uint64_t* slots = static_cast<uint64_t*>(pw.slots);
-#ifndef SA_LEAF_KERNEL
-#define SA_LEAF_KERNEL panel_leaf_kernel
-#endif
-#define SA_LEAF(R, V) hipLaunchKernelGGL((SA_LEAF_KERNEL<T, W, R, V>), dim3(G), dim3(LEAF_THREADS), 0, s, \
-                                          A, ld, r0, m_total, c0, w, ipiv, info, thr, tag0, slots, pw.error)
-    if (vec) {
-        switch (rpt) { case 1: SA_LEAF(1, 1); break; case 2: SA_LEAF(2, 1); break; default: SA_LEAF(4, 1); break; }
-    }
-    else {
-        switch (rpt) { case 1: SA_LEAF(1, 0); break; case 2: SA_LEAF(2, 0); break; default: SA_LEAF(4, 0); break; }
-    }
+#define SA_LEAF(NTV, R, V) hipLaunchKernelGGL((panel_leaf_rot_kernel<T, W, R, V, NTV>), dim3(G), dim3(NTV), 0, s, \
+                                               A, ld, r0, m_total, c0, w, ipiv, info, thr, tag0, slots, pw.error)
+#define SA_LEAF_V(NTV, R) do { if (vec) SA_LEAF(NTV, R, 1); else SA_LEAF(NTV, R, 0); } while (0)
+    if (nt == 256) SA_LEAF_V(256, 1);
+    else if (rpt == 1) SA_LEAF_V(512, 1);
+    else SA_LEAF_V(512, 2);
+#undef SA_LEAF_V
 #undef SA_LEAF
     SA_CHECK_LAUNCH();
 }
