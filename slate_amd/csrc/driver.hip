@@ -141,9 +141,13 @@ static int64_t choose_nb(const Context& c, int kind, int64_t n, int64_t requeste
     if (requested > 0) return requested;
     for (const auto& e : c.nb_table[kind])
         if (n <= e.first) return e.second;
+    // Measured at n = 65536 (float LU with BF16X3 updates / FP64 LU):
+    // nb 512 1736 / 4400 ms, 768 1604 / 4244, 896 1580 / 4195, 1024 1569 /
+    // 4146 ms (larger K per trailing update, fewer steps).
     if (n <= 2048) return 128;
     if (n <= 16384) return 256;
-    return 512;
+    if (n <= 32768) return 512;
+    return 1024;
 }
 
 // Tags of the panel exchange words advance monotonically over the life of the

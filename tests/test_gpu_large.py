@@ -59,3 +59,33 @@ def test_posv_full_size(gpu):
     r = residual_dev(S0, B, B0)
     say("  info %d scaled residual %.3f (%.1f ms)" % (info, r, slate_amd.last_time_ms()))
     assert info == 0 and r <= 3.0
+
+
+@pytest.mark.parametrize("nb", [768, 1024])
+def test_gesv_large_blocks(gpu, nb):
+    """The block sizes chosen for the largest matrices, on a smaller one."""
+    n = 8192
+    say("gesv n=%d nb=%d" % (n, nb))
+    A = dev_rands(2, n, n)
+    B = dev_rands(3, n, 2)
+    A0, B0 = A.clone(), B.clone()
+    info, ipiv = slate_amd.gesv(A, B, slate_amd.Options(block_size=nb))
+    r = residual_dev(A0, B, B0)
+    say("  info %d scaled residual %.3f (%.1f ms)" % (info, r, slate_amd.last_time_ms()))
+    assert info == 0 and r <= 3.0
+
+
+def test_gesv_mixed_benchmark_size(gpu):
+    """The benchmark's configuration end to end: n = 65536, float LU with
+    three-term split-BF16 updates (block 1024, CU-masked trailing stream),
+    FP64 classic IR, checked by the scaled residual."""
+    n = 65536
+    say("gesv_mixed n=%d bf16x3" % n)
+    A = dev_rands(0, n, n)
+    B = dev_rands(1, n, 1)
+    X = slate_amd.empty_colmajor(n, 1, torch.float64)
+    info, it, _ = slate_amd.gesv_mixed(A, B, X, slate_amd.Options(factor_precision=slate_amd.PREC_BF16X3))
+    ms = slate_amd.last_time_ms()
+    r = residual_dev(A, X, B)
+    say("  info %d iter %d scaled residual %.4f (%.1f ms)" % (info, it, r, ms))
+    assert info == 0 and 0 <= it <= 30 and r <= 3.0
