@@ -6,10 +6,9 @@
 // batched trsm + gemm launch pair per block row (src/work/work_trsmA.cc,
 // reached from src/getrs.cc:94-97).  Here the whole sweep is one launch:
 // block q (BS rows) is handled by workgroup q, which accumulates the products
-// with every already-solved block as soon as a completion counter says it is
-// ready (a "look-back" chain: blocks finish in order, so one monotonic counter
-// is enough), then applies the precomputed inverse of its diagonal block,
-// publishes its piece of the solution and bumps the counter.
+// with every already-solved block as soon as that block's solved piece has
+// been published (a "look-back" chain: blocks finish in order), then applies
+// the precomputed inverse of its diagonal block and publishes its own piece.
 //
 // The diagonal blocks are inverted once per factor by trsv_diag_inverse (the
 // inverted-diagonal-block form of trsm that GPU BLAS libraries use), so the
