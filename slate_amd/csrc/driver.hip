@@ -51,6 +51,7 @@ struct Context {
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
     Buf tvw;                     // trsv look-back exchange words (+ progress word)
+    Buf swaptmp;                 // two-pass row interchange workspace (trailing-update stream)
     uint32_t tv_tag = 0;         // next free tag of tvw
     bool calu = false;           // leaves use tournament pivoting (MethodLU::CALU)
     // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
@@ -480,8 +481,15 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             laswp_plan(p0, p0 + pw, n, ipiv, step_plan(k), s1);
         }
     };
+    // Wide interchanges on the trailing-update stream go through a two-pass
+    // gather/scatter workspace (the lookahead block's, on s1, stay one-pass).
+    const int64_t swap_elems = 2 * nb * n;
+    if (!nopiv) c.swaptmp.ensure(sizeof(T) * size_t(swap_elems));
     auto swap = [&](int64_t a0, int64_t a1, int64_t p0, int64_t p1, int64_t* pk, hipStream_t st) {
-        if (!nopiv) laswp<T>(A, ld, n, a0, a1, p0, p1, ipiv, pk, st, true);
+        if (nopiv) return;
+        const bool wide = st != s1 || !la;
+        laswp<T>(A, ld, n, a0, a1, p0, p1, ipiv, pk, st, true, wide ? c.swaptmp.as<T>() : nullptr,
+                 wide ? swap_elems : 0);
     };
     // panel 0
     factor_panel(0, 0, std::min(nb, n));
@@ -492,10 +500,15 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         const int64_t k0 = k * nb, kb = std::min(nb, n - k0);
         const int64_t c1 = k0 + kb;                   // first column after the panel
         const int64_t r = n - c1;                     // trailing size
-        if (la && c.s2 && s0 == c.s0 && r < c.panel_switch[sizeof(T) == 8 ? 1 : 0]) {
-            SA_HIP(hipEventRecord(c.ev_in, s0));
-            SA_HIP(hipStreamWaitEvent(c.s2, c.ev_in, 0));
-            s0 = c.s2;
+        {
+            const int pi = sizeof(T) == 8 ? 1 : 0;
+            hipStream_t want = c.s0;
+            if (la && c.s2 && r < c.panel_switch[pi]) want = c.s2;
+            if (want != s0) {
+                SA_HIP(hipEventRecord(c.ev_in, s0));
+                SA_HIP(hipStreamWaitEvent(want, c.ev_in, 0));
+                s0 = want;
+            }
         }
         SA_HIP(hipStreamWaitEvent(s0, ev_panel(k), 0));
         // pivots of step k applied to the columns on the left (finished L) on s0
@@ -1593,7 +1606,9 @@ int slate_amd_init(slate_amd_context* out, int device)
         // the FP64 LU is update-bound and keeps every CU (switch 0).
         // SLATE_AMD_PANEL_CUS overrides the reserve (0 = none) and
         // SLATE_AMD_PANEL_SWITCH the trailing size below which both
-        // precisions use it.
+        // precisions use it.  (Measured at n = 65536, nb = 1024: reserve 64
+        // 1543 ms; 32, 48, 96 CUs 1871, 1887, 1705 ms; a smaller reserve of
+        // 16 or 32 CUs for the early, update-bound steps 1752-1833 ms.)
         {
             hipDeviceProp_t prop;
             SA_HIP(hipGetDeviceProperties(&prop, device));
@@ -1627,7 +1642,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
-                       &c.tvw})
+                       &c.tvw, &c.swaptmp})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);

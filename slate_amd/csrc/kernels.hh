@@ -105,9 +105,15 @@ int64_t calu_slot_bytes(int elem_bytes, int64_t m);
 // one-block plan that turns the sequence into a (dst <- src) row list, then a
 // gather/scatter over column chunks.  plan must hold 4 * (k2 - k1) + 1 int64.
 // Pivots outside [i, nrows) are treated as "no interchange".
+// With a workspace (tmp, tmp_elems >= 2 (k2 - k1) (c1 - c0)) a ready plan is
+// applied in two passes instead — gather every source row into tmp, then
+// scatter — each a full-width, 16-byte-vector row copy: the one-pass form
+// stages whole column chunks of all moved rows in LDS, which limits a chunk
+// to a few columns (32-byte row segments) when 2048 rows move.
 template <class T>
 void laswp(T* A, int64_t ld, int64_t nrows, int64_t c0, int64_t c1, int64_t k1, int64_t k2,
-           const int64_t* ipiv, int64_t* plan, hipStream_t s, bool plan_ready = false);
+           const int64_t* ipiv, int64_t* plan, hipStream_t s, bool plan_ready = false,
+           T* tmp = nullptr, int64_t tmp_elems = 0);
 void laswp_plan(int64_t k1, int64_t k2, int64_t nrows, const int64_t* ipiv, int64_t* plan,
                 hipStream_t s);
 // Turn a (dst <- src) plan into its inverse permutation (dst/src swapped).

@@ -330,15 +330,16 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
     }
 }
 
-// Leaf shape (NT threads, RPT rows each): the fewest rows per thread that
-// keep the grid at 64 workgroups or fewer — 256 x 1 up to m = 16384, 512 x 1
-// up to 32768, 512 x 2 beyond (measured per column at m = 65536: 256 x 4
-// 7.8 us, 512 x 2 7.1 us, 1024 x 1 7.5 us, 1024 x 2 22.7 us for FP32 W = 32;
-// fewer, fatter workgroups lose more in their own barriers and updates than
-// they save in gathered heads).
+// Leaf shape (NT threads, RPT rows each).  Measured per column: FP32 W = 32
+// at m = 65536: 256 x 4 7.8 us, 512 x 2 7.1 us, 1024 x 1 7.5 us, 1024 x 2
+// 22.7 us; at m = 16384: 256 x 1 (G = 64) 5.8 us, 512 x 1 (G = 32) 5.1 us;
+// at m = 8192: 256 x 1 (G = 32) 4.8 us.  Fewer, fatter workgroups lose
+// more in their own barriers and updates than they save in gathered heads,
+// and a one-round gather of every candidate row (no second read of the
+// winner's row) measured no faster at G <= 16 (m = 8192, 512 x 1: 5.1 us).
 static void leaf_shape(int64_t m, int& nt, int& rpt)
 {
-    if (m <= 64 * 256) { nt = 256; rpt = 1; }
+    if (m <= 32 * 256) { nt = 256; rpt = 1; }
     else if (m <= 64 * 512) { nt = 512; rpt = 1; }
     else { nt = 512; rpt = 2; }
 }
@@ -575,11 +576,59 @@ void laswp_invert_plan(int64_t* plan, int64_t npiv, hipStream_t s)
     SA_CHECK_LAUNCH();
 }
 
+// Two-pass interchange: pass 0 copies source row plan[2+2e] (columns
+// [c0, c1)) into tmp row e, pass 1 copies tmp row e into destination row
+// plan[1+2e].  grid.y strides over entries, grid.x over 16-byte column groups.
+template <class T, int PASS, int VEC>
+__global__ __launch_bounds__(256)
+void laswp_pass_kernel(T* __restrict__ A, int64_t ld, int64_t c0, int64_t ncol, T* __restrict__ tmp,
+                       const int64_t* __restrict__ plan)
+{
+    constexpr int VE = VEC ? 16 / int(sizeof(T)) : 1;
+    typedef T vec_t __attribute__((ext_vector_type(VE)));
+    const int count = int(plan[0]);
+    const int64_t col = (int64_t(blockIdx.x) * 256 + threadIdx.x) * VE;
+    if (col >= ncol) return;
+    for (int e = blockIdx.y; e < count; e += gridDim.y) {
+        const int64_t row = plan[(PASS == 0 ? 2 : 1) + 2 * e];
+        T* a = A + row * ld + c0 + col;
+        T* t = tmp + int64_t(e) * ncol + col;
+        if (VEC && col + VE <= ncol) {
+            if (PASS == 0) *reinterpret_cast<vec_t*>(t) = *reinterpret_cast<const vec_t*>(a);
+            else *reinterpret_cast<vec_t*>(a) = *reinterpret_cast<const vec_t*>(t);
+        }
+        else {
+            for (int q = 0; q < VE && col + q < ncol; ++q) {
+                if (PASS == 0) t[q] = a[q];
+                else a[q] = t[q];
+            }
+        }
+    }
+}
+
 template <class T>
 void laswp(T* A, int64_t ld, int64_t nrows, int64_t c0, int64_t c1, int64_t k1, int64_t k2,
-           const int64_t* ipiv, int64_t* plan, hipStream_t s, bool plan_ready)
+           const int64_t* ipiv, int64_t* plan, hipStream_t s, bool plan_ready, T* tmp, int64_t tmp_elems)
 {
     if (k2 <= k1) return;
+    if (plan_ready && tmp && c1 > c0 && tmp_elems >= 2 * (k2 - k1) * (c1 - c0)) {
+        constexpr int VE = 16 / int(sizeof(T));
+        const int64_t ncol = c1 - c0;
+        const bool vec = ld % VE == 0 && c0 % VE == 0 && ncol % VE == 0
+                         && reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(tmp) % 16 == 0;
+        const int64_t per = vec ? VE : 1;
+        dim3 grid(unsigned(ceildiv(ncol, 256 * per)), unsigned(std::min<int64_t>(2 * (k2 - k1), 2048)));
+        if (vec) {
+            hipLaunchKernelGGL((laswp_pass_kernel<T, 0, 1>), grid, dim3(256), 0, s, A, ld, c0, ncol, tmp, plan);
+            hipLaunchKernelGGL((laswp_pass_kernel<T, 1, 1>), grid, dim3(256), 0, s, A, ld, c0, ncol, tmp, plan);
+        }
+        else {
+            hipLaunchKernelGGL((laswp_pass_kernel<T, 0, 0>), grid, dim3(256), 0, s, A, ld, c0, ncol, tmp, plan);
+            hipLaunchKernelGGL((laswp_pass_kernel<T, 1, 0>), grid, dim3(256), 0, s, A, ld, c0, ncol, tmp, plan);
+        }
+        SA_CHECK_LAUNCH();
+        return;
+    }
     if (!plan_ready && k2 - k1 <= 64) {
         if (c1 <= c0) return;
         const int np = int(k2 - k1);
@@ -609,8 +658,8 @@ void laswp(T* A, int64_t ld, int64_t nrows, int64_t c0, int64_t c1, int64_t k1, 
 }
 
 template void laswp<double>(double*, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,
-                            const int64_t*, int64_t*, hipStream_t, bool);
+                            const int64_t*, int64_t*, hipStream_t, bool, double*, int64_t);
 template void laswp<float>(float*, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,
-                           const int64_t*, int64_t*, hipStream_t, bool);
+                           const int64_t*, int64_t*, hipStream_t, bool, float*, int64_t);
 
 } // namespace sa

@@ -19,7 +19,7 @@ namespace sa {
 
 //------------------------------------------------------------------------------
 // One thread per right-hand-side column; the triangle lives in LDS and is
-// padded to 32 with an identity so the fully unrolled substitution has no
+// padded to NB with an identity so the fully unrolled substitution has no
 // data-dependent bounds.
 // FULL (nb == TRSM_LEAF) has no row predicates: predicated stores of the
 // unrolled register array make the compiler spill it.
