@@ -302,6 +302,17 @@ void gemm_sp_kernel(int64_t M, int64_t N, int64_t K, const float* __restrict__ L
     }
     v4f ra[2][2], rb[2][2];
     auto load = [&](int64_t k0) {
+        if (k0 + BK <= K) {
+            // whole slice inside K (uniform): plain 16-byte loads, no edge selects
+            #pragma unroll
+            for (int q = 0; q < 2; ++q)
+                #pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    ra[h][q] = *reinterpret_cast<const v4f*>(L + aoff[h] + k0 + lc * 8 + q * 4);
+                    rb[h][q] = *reinterpret_cast<const v4f*>(R + boff[h] + k0 + lc * 8 + q * 4);
+                }
+            return;
+        }
         #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int64_t k = k0 + lc * 8 + q * 4;
