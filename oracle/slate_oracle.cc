@@ -530,7 +530,12 @@ void residual(int64_t n, int64_t nrhs, const double* A, int64_t lda, int sym,
 // internal_util.hh:121-138
 bool ir_converged(const std::vector<double>& nr, const std::vector<double>& nx, double cte)
 {
-    for (size_t i = 0; i < nr.size(); ++i) if (nr[i] > nx[i] * cte) return false;
+    // a non-finite norm is "not converged" (the comparison alone would accept
+    // NaN and inf > inf); the library does the same
+    for (size_t i = 0; i < nr.size(); ++i) {
+        if (!std::isfinite(nr[i]) || !std::isfinite(nx[i])) return false;
+        if (nr[i] > nx[i] * cte) return false;
+    }
     return true;
 }
 

@@ -911,9 +911,17 @@ struct MixedState {
     double* norms;        // [0]: ||A||, [1..]: col norms
 };
 
+// internal_util.hh:121-138 (converged unless ||r_i|| > ||x_i|| * cte), with
+// one difference: a non-finite norm is "not converged".  The reference's
+// comparison is false for NaN and for inf > inf, so a refinement that diverges
+// to overflow would stop as converged with a non-finite solution; here it runs
+// on to itermax and the FP64 fallback.
 static bool converged_host(const std::vector<double>& nr, const std::vector<double>& nx, double cte)
 {
-    for (size_t i = 0; i < nr.size(); ++i) if (nr[i] > nx[i] * cte) return false;   // internal_util.hh:121-138
+    for (size_t i = 0; i < nr.size(); ++i) {
+        if (!std::isfinite(nr[i]) || !std::isfinite(nx[i])) return false;
+        if (nr[i] > nx[i] * cte) return false;
+    }
     return true;
 }
 
