@@ -52,6 +52,7 @@ struct Context {
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
     Buf tvw;                     // trsv look-back exchange words (+ progress word)
     Buf swaptmp;                 // two-pass row interchange workspace (trailing-update stream)
+    Buf swaptmp1;                // the same for the lookahead block (tall blocks only)
     uint32_t tv_tag = 0;         // next free tag of tvw
     bool calu = false;           // leaves use tournament pivoting (MethodLU::CALU)
     // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
@@ -482,14 +483,23 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         }
     };
     // Wide interchanges on the trailing-update stream go through a two-pass
-    // gather/scatter workspace (the lookahead block's, on s1, stay one-pass).
-    const int64_t swap_elems = 2 * nb * n;
-    if (!nopiv) c.swaptmp.ensure(sizeof(T) * size_t(swap_elems));
+    // gather/scatter workspace; the lookahead block's, on s1, stay one-pass
+    // unless the block is too tall for the one-pass LDS staging (nb > 1024
+    // in FP64 / 2048 in float), when they use a workspace of their own.
+    const int64_t swap_elems = 2 * nb * n, swap1_elems = 2 * nb * nb;
+    const bool s1_two_pass = int64_t(4) * 2 * nb * int64_t(sizeof(T)) > 65536;
+    if (!nopiv) {
+        c.swaptmp.ensure(sizeof(T) * size_t(swap_elems));
+        if (s1_two_pass) c.swaptmp1.ensure(sizeof(T) * size_t(swap1_elems));
+    }
     auto swap = [&](int64_t a0, int64_t a1, int64_t p0, int64_t p1, int64_t* pk, hipStream_t st) {
         if (nopiv) return;
-        const bool wide = st != s1 || !la;
-        laswp<T>(A, ld, n, a0, a1, p0, p1, ipiv, pk, st, true, wide ? c.swaptmp.as<T>() : nullptr,
-                 wide ? swap_elems : 0);
+        if (st != s1 || !la)
+            laswp<T>(A, ld, n, a0, a1, p0, p1, ipiv, pk, st, true, c.swaptmp.as<T>(), swap_elems);
+        else if (s1_two_pass)
+            laswp<T>(A, ld, n, a0, a1, p0, p1, ipiv, pk, st, true, c.swaptmp1.as<T>(), swap1_elems);
+        else
+            laswp<T>(A, ld, n, a0, a1, p0, p1, ipiv, pk, st, true);
     };
     // panel 0
     factor_panel(0, 0, std::min(nb, n));
@@ -1650,7 +1660,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
-                       &c.tvw, &c.swaptmp})
+                       &c.tvw, &c.swaptmp, &c.swaptmp1})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);

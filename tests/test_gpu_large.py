@@ -89,3 +89,17 @@ def test_gesv_mixed_benchmark_size(gpu):
     r = residual_dev(A, X, B)
     say("  info %d iter %d scaled residual %.4f (%.1f ms)" % (info, it, r, ms))
     assert info == 0 and 0 <= it <= 30 and r <= 3.0
+
+
+def test_getrf_block_above_one_pass_swap_limit(gpu):
+    """nb = 1536 in FP64: the lookahead block's interchanges exceed the one-pass
+    LDS staging and take the two-pass path."""
+    n = 6144
+    say("getrf fp64 n=%d nb=1536" % n)
+    A = dev_rands(4, n, n)
+    B = dev_rands(5, n, 1)
+    A0, B0 = A.clone(), B.clone()
+    info, ipiv = slate_amd.gesv(A, B, slate_amd.Options(block_size=1536))
+    r = residual_dev(A0, B, B0)
+    say("  info %d scaled residual %.3f" % (info, r))
+    assert info == 0 and r <= 3.0
