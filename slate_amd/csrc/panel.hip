@@ -51,6 +51,19 @@ __device__ inline T lds_val(const uint32_t* w)
 }
 
 template <class T>
+__device__ inline void lds_put_val(uint32_t* w, T v)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        w[0] = uint32_t(u);
+        w[1] = uint32_t(u >> 32);
+    }
+    else {
+        w[0] = __builtin_bit_cast(uint32_t, v);
+    }
+}
+
+template <class T>
 __device__ inline bool better(T a, int64_t ia, T b, int64_t ib)
 {
     return a > b || (a == b && ia < ib);
@@ -91,6 +104,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
     constexpr int NW = NT / 64;                 // waves per workgroup
     __shared__ T s_wabs[NW];
     __shared__ uint32_t s_widx[NW];
+    __shared__ uint32_t s_pub[2][W * NWV];      // rows being published (candidate, row j)
     __shared__ uint32_t s_words[AMAX];
     __shared__ __attribute__((aligned(16))) T s_rowj[W];   // row j, rotated
     __shared__ __attribute__((aligned(16))) T s_piv[W];    // pivot row, rotated
@@ -156,25 +170,35 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
             bidx = b ? s_widx[k] : bidx;
         }
 
-        // (2) publish the head (|a|, row) and the candidate row and, from the
-        // owner of row j, row j (rows in rotated order)
+        // (2) publish the head (|a|, row) first — it is what every workgroup
+        // polls for — then the candidate row and, from the owner of row j,
+        // row j (rotated order).  A row is copied by its owning thread into
+        // LDS and stored by the lanes of that thread's wave, one word each
+        // (one vector store instead of W single-lane stores).
         uint64_t* ent = buf + int64_t(g) * CW;
-        #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int64_t i = gbase + tid + int64_t(NT) * q;
-            if (bidx != NONE && i == int64_t(bidx)) {
-                #pragma unroll
-                for (int c = 0; c < W; ++c) tw_store_val<T>(ent + HW + c * NWV, v[q][c], tag);
-            }
-            if (i == j) {
-                #pragma unroll
-                for (int c = 0; c < W; ++c) tw_store_val<T>(buf + int64_t(G) * CW + c * NWV, v[q][c], tag);
-            }
-        }
         if (tid == 0) {
             tw_store_val<T>(ent, babs, tag);
             tw_store(ent + NWV, bidx, tag);
         }
+        const int64_t cl = (bidx != NONE) ? int64_t(bidx) - gbase : -1;
+        const int64_t jl = int64_t(j) - gbase;
+        const int cand_t = (cl >= 0 && cl < R) ? int(cl % NT) : -1;
+        const int rowj_t = (jl >= 0 && jl < R) ? int(jl % NT) : -1;
+        #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (tid == cand_t && q == int(cl / NT)) {
+                #pragma unroll
+                for (int c = 0; c < W; ++c) lds_put_val<T>(&s_pub[0][c * NWV], v[q][c]);
+            }
+            if (tid == rowj_t && q == int(jl / NT)) {
+                #pragma unroll
+                for (int c = 0; c < W; ++c) lds_put_val<T>(&s_pub[1][c * NWV], v[q][c]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (cand_t >= 0 && wave == cand_t / 64 && lane < W * NWV) tw_store(ent + HW + lane, s_pub[0][lane], tag);
+        if (rowj_t >= 0 && wave == rowj_t / 64 && lane < W * NWV)
+            tw_store(buf + int64_t(G) * CW + lane, s_pub[1][lane], tag);
 
         // (3a) gather the heads of all workgroups and row j
         {
