@@ -1618,20 +1618,23 @@ int slate_amd_init(slate_amd_context* out, int device)
         SA_HIP(hipEventCreateWithFlags(&h->c.ev_in, hipEventDisableTiming));
         SA_HIP(hipEventCreateWithFlags(&h->c.ev_out, hipEventDisableTiming));
         // Panel reserve: the float LU's trailing updates run on s2, whose CU
-        // mask leaves `reserve` CUs (64 of 256 by default) to the panel
+        // mask leaves `reserve` CUs (32 of 256 by default) to the panel
         // stream.  With split-BF16 updates the panel is the critical path and
         // the reserve shortens the n = 65536 factorization by about a quarter;
         // the FP64 LU is update-bound and keeps every CU (switch 0).
         // SLATE_AMD_PANEL_CUS overrides the reserve (0 = none) and
         // SLATE_AMD_PANEL_SWITCH the trailing size below which both
-        // precisions use it.  (Measured at n = 65536, nb = 1024: reserve 64
-        // 1543 ms; 32, 48, 96 CUs 1871, 1887, 1705 ms; a smaller reserve of
-        // 16 or 32 CUs for the early, update-bound steps 1752-1833 ms.)
+        // precisions use it.  Measured at n = 65536, nb = 1024, same box:
+        // reserve 32 / 48 / 64 / 80 CUs: 1485 / 1560 / 1540 / 1611 ms (with
+        // the current leaf; before it was shortened, 64 was best), a smaller
+        // reserve for the early, update-bound steps only: slower.  The
+        // non-monotonic response suggests the reserve works best in whole
+        // XCDs (32 CUs each).
         {
             hipDeviceProp_t prop;
             SA_HIP(hipGetDeviceProperties(&prop, device));
             const int ncu = prop.multiProcessorCount;
-            int reserve = ncu / 4;
+            int reserve = ncu / 8;
             if (const char* e = std::getenv("SLATE_AMD_PANEL_CUS")) reserve = std::atoi(e);
             if (reserve > 0 && reserve < ncu) {
                 std::vector<uint32_t> mask(size_t((ncu + 31) / 32), 0u);
