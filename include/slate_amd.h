@@ -53,7 +53,7 @@ typedef struct {
     int32_t use_fallback;      /* Option::UseFallbackSolver (default 1)          */
     int32_t factor_precision;  /* SLATE_AMD_PREC_* for the mixed drivers         */
     int32_t method_lu;         /* Option::MethodLU: SLATE_AMD_LU_* (default PP)  */
-    int32_t reserved;
+    int32_t depth;             /* Option::Depth: butterfly depth of gesv_rbt (2) */
 } slate_amd_options;
 
 /* slate::MethodLU (include/slate/enums.hh:294-301) for getrf/gesv and the
@@ -127,6 +127,31 @@ int64_t slate_amd_dsposv(slate_amd_context ctx, int uplo, int64_t n, int64_t nrh
 int64_t slate_amd_dsposv_gmres(slate_amd_context ctx, int uplo, int64_t n, int64_t nrhs,
                                double* dA, int64_t lda, const double* dB, int64_t ldb,
                                double* dX, int64_t ldx, int* iter, const slate_amd_options* opts);
+
+/* ---- random butterfly transform solver ---------------------------------- */
+/* slate::gesv_rbt(A, B, X, iter, Options) — slate.hh:558-563,
+ * src/gesv_rbt.cc:94-236.  A := U^T A V (depth opts->depth, seed 42), LU
+ * without pivoting, X = V (LU)^{-1} U^T B, FP64 refinement against a copy of
+ * A with the PP gesv as fallback.  On return A holds the factors of U^T A V,
+ * or the original A when the fallback ran.  *iter: >= 0 refinement steps,
+ * -(itermax+1) no convergence.  The weights come from matgen rands (seeds 42
+ * and 42 + ceil(n/nb)), not LAPACK larnv. */
+int64_t slate_amd_dgesv_rbt(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                            const double* dB, int64_t ldb, double* dX, int64_t ldx, int* iter,
+                            const slate_amd_options* opts);
+/* internal::rbt_generate — src/internal/internal_rbt_generate.cc:26-140:
+ * du[i] = exp(r_i / 20), r_i the matgen rands entry (i, 0) of `seed`.  The
+ * vector is column 0 of the reference's n x depth weight matrix, the only
+ * column its butterflies read. */
+int slate_amd_drbt_generate(slate_amd_context ctx, int64_t n, int64_t seed, double* du);
+/* slate::gerbt(U, A, V) — slate.hh:166-169, src/gerbt.cc:124-217: two-sided
+ * transform A := U^T A V of depth `depth` over nb-row tiles. */
+int slate_amd_dgerbt(slate_amd_context ctx, int64_t n, int depth, int64_t nb, const double* du,
+                     const double* dv, double* dA, int64_t lda);
+/* slate::gerbt(U, B) — slate.hh:171-173, src/gerbt.cc:251-323: B := U^T B
+ * (trans = SLATE_AMD_TRANS) or B := U B (SLATE_AMD_NOTRANS, used with V). */
+int slate_amd_dgerbt_rhs(slate_amd_context ctx, int trans, int64_t n, int64_t nrhs, int depth, int64_t nb,
+                         const double* du, double* dB, int64_t ldb);
 
 /* ---- instrumentation ----------------------------------------------------- */
 /* Times every trailing-update GEMM launch of the factorizations with HIP

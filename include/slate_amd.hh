@@ -62,7 +62,7 @@ enum class MethodLU : char { PartialPiv = 'P', CALU = 'C', NoPiv = 'N' };   // i
 
 enum class Option : char {
     Lookahead, InnerBlocking, MaxPanelThreads, PivotThreshold, Tolerance, MaxIterations,
-    UseFallbackSolver, Target, HoldLocalWorkspace, FactorPrecision, MethodLU,
+    UseFallbackSolver, Target, HoldLocalWorkspace, FactorPrecision, MethodLU, Depth,
 };
 
 class OptionValue {
@@ -325,6 +325,7 @@ inline slate_amd_options to_c(const Options& o, int64_t nb)
         throw NotImplemented("MethodLU");
     c.method_lu = m == MethodLU::NoPiv ? SLATE_AMD_LU_NOPIV
                 : m == MethodLU::CALU ? SLATE_AMD_LU_CALU : SLATE_AMD_LU_PARTIALPIV;
+    c.depth = int32_t(opt_i(o, Option::Depth, 2));
     return c;
 }
 
@@ -555,6 +556,19 @@ int64_t posv_mixed_gmres(HermitianMatrix<double>& A, Matrix<double>& B, Matrix<d
     return detail::mixed_call(A, B, X, opts, [&](slate_amd_options& c, auto& dA, auto& dB, auto& dX) {
         return slate_amd_dsposv_gmres(detail::context(detail::current_device()), uplo, A.n(), B.n(),
                                       dA.ptr(), dA.ld(), dB.ptr(), dB.ld(), dX.ptr(), dX.ld(), &iter, &c);
+    });
+}
+
+// slate::gesv_rbt (include/slate/slate.hh:558-563, src/gesv_rbt.cc:94-236):
+// random butterfly transform, LU without pivoting, FP64 refinement.
+template <typename scalar_t>
+void gesv_rbt(Matrix<scalar_t>& A, Matrix<scalar_t>& B, Matrix<scalar_t>& X, int& iter,
+              const Options& opts = Options())
+{
+    static_assert(std::is_same<scalar_t, double>::value, "gesv_rbt: double only");
+    detail::mixed_call(A, B, X, opts, [&](slate_amd_options& c, auto& dA, auto& dB, auto& dX) {
+        return slate_amd_dgesv_rbt(detail::context(detail::current_device()), A.n(), B.n(), dA.ptr(),
+                                   dA.ld(), dB.ptr(), dB.ld(), dX.ptr(), dX.ld(), &iter, &c);
     });
 }
 

@@ -39,6 +39,16 @@ generate_svd      (this library's generator; the reference's  invariants: singul
                   svd kind, generate_type_svd.hh:90-157,      orthogonality, det = (-1)^n of each
                   uses QR of a Gaussian instead)              n-reflector product, symmetry for sym
 norms/residual    src/norm.cc, src/colNorms.cc                numpy closed forms
+rbt_weights       src/internal/internal_rbt_generate.cc:26-140 closed form exp(rands/20); the
+                                                              LAPACK larnv stream of the reference
+                                                              is parity unpinned (not available)
+gerbt/gerbt_rhs   src/gerbt.cc, src/internal/Tile_gerbt.hh    explicit dense butterfly matrices
+                                                              built from the definition, products
+                                                              of levels, orthogonality with unit
+                                                              weights, U^T A V = (U^T)(A)(V)
+dgesv_rbt         src/gesv_rbt.cc:94-236                      numpy solve, L U = U^T A V for the
+                                                              unpivoted factor, iteration codes,
+                                                              fallback result equals plain gesv
 ================  ==========================================  =====================================
 """
 import ctypes
@@ -93,6 +103,12 @@ def lib():
             "oracle_residual": (None, [c_i64, c_i64, c_p, c_i64, c_int, c_p, c_i64, c_p, c_i64,
                                        c_p, c_i64]),
             "oracle_rotg": (None, [c_p, c_p, c_p, c_p]),
+            "oracle_rbt_weights": (None, [c_i64, c_i64, c_p]),
+            "oracle_rbt_half": (c_i64, [c_i64, c_i64, c_i64, c_i64]),
+            "oracle_gerbt": (None, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_i64]),
+            "oracle_gerbt_rhs": (None, [c_int, c_i64, c_i64, c_i64, c_i64, c_p, c_p, c_i64]),
+            "oracle_dgesv_rbt": (c_i64, [c_i64, c_i64, c_p, c_i64, c_p, c_i64, c_p, c_i64, c_i64,
+                                         c_i64, c_i64, c_i64, c_dbl, c_int, c_p]),
             "oracle_num_threads": (c_int, []),
             "oracle_set_num_threads": (None, [c_int]),
         }
@@ -250,6 +266,52 @@ def posv_mixed_gmres(A, b, nb=256, itermax=30, tol=0.0, use_fallback=True):
     info = lib().oracle_dsposv_gmres(n, _ptr(A), n, _ptr(b), _ptr(x), nb, itermax, tol,
                                      int(use_fallback), ctypes.byref(it))
     return A, x, int(info), it.value
+
+
+def rbt_weights(seed, n):
+    """Butterfly weights exp(r/20), r = rands(seed) column 0 (internal_rbt_generate.cc:45-51)."""
+    u = np.empty(n)
+    lib().oracle_rbt_weights(seed, n, _ptr(u))
+    return u
+
+
+def rbt_half(n, nb, depth, k):
+    return int(lib().oracle_rbt_half(n, nb, depth, k))
+
+
+def gerbt(u, A, v, depth, nb):
+    """A := U^T A V (src/gerbt.cc:124-217); returns a new Fortran array."""
+    A = _fmat(A).copy(order="F")
+    n = A.shape[0]
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    lib().oracle_gerbt(n, nb, depth, _ptr(u), _ptr(v), _ptr(A), max(n, 1))
+    return A
+
+
+def gerbt_rhs(u, B, depth, nb, trans):
+    """B := U^T B (trans) or U B (src/gerbt.cc:251-323); returns a new Fortran array."""
+    B = _fmat(B).copy(order="F")
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n, nrhs = B.shape
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lib().oracle_gerbt_rhs(int(bool(trans)), n, nrhs, nb, depth, _ptr(u), _ptr(B), max(n, 1))
+    return B
+
+
+def gesv_rbt(A, B, nb=256, ib=16, depth=2, itermax=30, tol=0.0, use_fallback=True):
+    """src/gesv_rbt.cc:94-236; returns (A_out, X, info, iter)."""
+    A = _fmat(A).copy(order="F")
+    B = _fmat(B)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n, nrhs = B.shape
+    X = np.zeros((n, nrhs), order="F")
+    it = ctypes.c_int(0)
+    info = lib().oracle_dgesv_rbt(n, nrhs, _ptr(A), n, _ptr(B), n, _ptr(X), n, nb, ib, depth, itermax,
+                                  tol, int(use_fallback), ctypes.byref(it))
+    return A, X, int(info), it.value
 
 
 def sigma(dist, cond, n):

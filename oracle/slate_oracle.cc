@@ -766,6 +766,170 @@ int64_t mixed_gmres(int64_t n, double* A, int64_t lda, int sym, const double* B,
     return info;
 }
 
+//------------------------------------------------------------------------------
+// Random butterfly transforms and the RBT solver.
+//
+// Level k of a depth-d transform pairs index i with i + h for (i mod 2h) < h,
+// h = 2^(d-k-1) * ceil(nt / 2^d) * nb, nt = ceil(n/nb): the tile ranges of
+// src/gerbt.cc:60-100 (gerbt_iterate_2d / _1d) in rows.  The element formulas
+// are those of src/internal/Tile_gerbt.hh:18-96 (two-sided) and :98-230
+// (one-sided), with the same single weight vector per side at every level
+// (column 0 of the reference's n x d weight matrix).  An index whose partner
+// lies beyond n is scaled by its weight only, in both the one- and two-sided
+// forms (the reference's one-sided pass skips whole partnerless tiles).
+int64_t rbt_half(int64_t n, int64_t nb, int64_t d, int64_t k)
+{
+    const int64_t nt = (n + nb - 1) / nb;
+    const int64_t inner = (nt + (int64_t(1) << d) - 1) >> d;
+    return (int64_t(1) << (d - k - 1)) * inner * nb;
+}
+
+// u[i] = exp(r_i / 20) (internal_rbt_generate.cc:45-51), r_i = matgen rands
+// entry (seed, i, 0) in place of LAPACK larnv.
+void rbt_weights(int64_t seed, int64_t n, double* u)
+{
+    for (int64_t i = 0; i < n; ++i) u[i] = std::exp(gen_one(2, seed, i, 0) / 20.0);
+}
+
+// A := U^T A V, levels smallest first (gerbt.cc:60-80, Tile_gerbt.hh:18-96).
+void gerbt2(int64_t n, int64_t nb, int64_t d, const double* u, const double* v, double* A, int64_t lda)
+{
+    const double half = 0.5, s2 = 1.0 / std::sqrt(2.0);
+    for (int64_t k = d - 1; k >= 0; --k) {
+        const int64_t h = rbt_half(n, nb, d, k);
+        for (int64_t j1 = 0; j1 < n; ++j1) {
+            if (j1 % (2 * h) >= h) continue;
+            const int64_t j2 = j1 + h;
+            const bool hc = j2 < n;
+            for (int64_t i1 = 0; i1 < n; ++i1) {
+                if (i1 % (2 * h) >= h) continue;
+                const int64_t i2 = i1 + h;
+                const bool hr = i2 < n;
+                double& a11 = A[i1 + j1*lda];
+                const double u1 = u[i1], v1 = v[j1];
+                if (hr && hc) {
+                    double& a12 = A[i1 + j2*lda];
+                    double& a21 = A[i2 + j1*lda];
+                    double& a22 = A[i2 + j2*lda];
+                    const double s1 = a11 + a12, t1 = a21 + a22, d1 = a11 - a12, e1 = a21 - a22;
+                    a11 = half * u1 * (s1 + t1) * v1;
+                    a12 = half * u1 * (d1 + e1) * v[j2];
+                    a21 = half * u[i2] * (s1 - t1) * v1;
+                    a22 = half * u[i2] * (d1 - e1) * v[j2];
+                }
+                else if (hc) {
+                    double& a12 = A[i1 + j2*lda];
+                    const double x = a11, y = a12;
+                    a11 = s2 * u1 * (x + y) * v1;
+                    a12 = s2 * u1 * (x - y) * v[j2];
+                }
+                else if (hr) {
+                    double& a21 = A[i2 + j1*lda];
+                    const double x = a11, y = a21;
+                    a11 = s2 * u1 * (x + y) * v1;
+                    a21 = s2 * u[i2] * (x - y) * v1;
+                }
+                else {
+                    a11 = u1 * a11 * v1;
+                }
+            }
+        }
+    }
+}
+
+// B := U^T B (trans, levels smallest first) or B := V B (largest first):
+// gerbt.cc:85-100, Tile_gerbt.hh gerbt_left_trans / gerbt_left_notrans.
+void gerbt1(int trans, int64_t n, int64_t nrhs, int64_t nb, int64_t d, const double* u,
+            double* B, int64_t ldb)
+{
+    const double s2 = 1.0 / std::sqrt(2.0);
+    for (int64_t it = 0; it < d; ++it) {
+        const int64_t k = trans ? d - 1 - it : it;
+        const int64_t h = rbt_half(n, nb, d, k);
+        for (int64_t c = 0; c < nrhs; ++c) {
+            double* b = B + c*ldb;
+            for (int64_t i1 = 0; i1 < n; ++i1) {
+                if (i1 % (2 * h) >= h) continue;
+                const int64_t i2 = i1 + h;
+                if (i2 >= n) { b[i1] = u[i1] * b[i1]; continue; }
+                const double x = b[i1], y = b[i2];
+                if (trans) {
+                    b[i1] = s2 * u[i1] * (x + y);
+                    b[i2] = s2 * u[i2] * (x - y);
+                }
+                else {
+                    b[i1] = s2 * (u[i1] * x + u[i2] * y);
+                    b[i2] = s2 * (u[i1] * x - u[i2] * y);
+                }
+            }
+        }
+    }
+}
+
+// src/gesv_rbt.cc:94-236: transform, LU without pivoting (pivot threshold 0
+// keeps every diagonal, Tile_getrf.hh:256-289), solve, refine in FP64 against
+// the original A, PP gesv of the copy as fallback.
+int64_t gesv_rbt(int64_t n, int64_t nrhs, double* A, int64_t lda, const double* B, int64_t ldb,
+                 double* X, int64_t ldx, int64_t nb, int64_t ib, int64_t depth, int64_t itermax,
+                 double tol, int use_fallback, int* iter)
+{
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (tol <= 0) tol = eps * std::sqrt(double(n));
+    std::vector<double> u(n), v(n), Acopy, R(n * nrhs), nx(nrhs), nr(nrhs);
+    std::vector<int64_t> ipiv(n);
+    rbt_weights(42, n, u.data());
+    rbt_weights(42 + (n + nb - 1) / nb, n, v.data());
+    double Anorm = 0;
+    if (itermax > 0) {
+        Acopy.resize(n * n);
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < n; ++i) Acopy[i + j*n] = A[i + j*lda];
+        Anorm = norm_inf_ge(n, n, A, lda);
+    }
+    for (int64_t c = 0; c < nrhs; ++c)
+        for (int64_t i = 0; i < n; ++i) X[i + c*ldx] = B[i + c*ldb];
+    *iter = 0;
+    gerbt2(n, nb, depth, u.data(), v.data(), A, lda);
+    int64_t info = getrf_tiles<double>(n, n, A, lda, nb, ib, 1, 0.0, ipiv.data());
+    if (info == 0) {
+        gerbt1(1, n, nrhs, nb, depth, u.data(), X, ldx);
+        getrs_tiles<double>(n, nrhs, A, lda, ipiv.data(), X, ldx, nb, 0);
+        gerbt1(0, n, nrhs, nb, depth, v.data(), X, ldx);
+    }
+    if (itermax == 0) return info;
+    const double cte = Anorm * tol;
+    bool converged = false;
+    if (info == 0) {
+        residual(n, nrhs, Acopy.data(), n, 0, X, ldx, B, ldb, R.data(), n);
+        col_norms_max(n, nrhs, X, ldx, nx.data());
+        col_norms_max(n, nrhs, R.data(), n, nr.data());
+        if (ir_converged(nr, nx, cte)) converged = true;
+        for (int64_t it = 0; it < itermax && !converged; ++it) {
+            gerbt1(1, n, nrhs, nb, depth, u.data(), R.data(), n);
+            getrs_tiles<double>(n, nrhs, A, lda, ipiv.data(), R.data(), n, nb, 0);
+            gerbt1(0, n, nrhs, nb, depth, v.data(), R.data(), n);
+            for (int64_t c = 0; c < nrhs; ++c)
+                for (int64_t i = 0; i < n; ++i) X[i + c*ldx] += R[i + c*n];
+            residual(n, nrhs, Acopy.data(), n, 0, X, ldx, B, ldb, R.data(), n);
+            col_norms_max(n, nrhs, X, ldx, nx.data());
+            col_norms_max(n, nrhs, R.data(), n, nr.data());
+            if (ir_converged(nr, nx, cte)) { *iter = int(it + 1); converged = true; }
+        }
+    }
+    if (!converged) {
+        *iter = int(-itermax - 1);
+        if (use_fallback) {
+            for (int64_t j = 0; j < n; ++j)
+                for (int64_t i = 0; i < n; ++i) A[i + j*lda] = Acopy[i + j*n];
+            info = getrf_tiles<double>(n, n, Acopy.data(), n, nb, ib, 1, 1.0, ipiv.data());
+            for (int64_t c = 0; c < nrhs; ++c)
+                for (int64_t i = 0; i < n; ++i) X[i + c*ldx] = B[i + c*ldb];
+            if (info == 0) getrs_tiles<double>(n, nrhs, Acopy.data(), n, ipiv.data(), X, ldx, nb, 0);
+        }
+    }
+    return info;
+}
+
 } // namespace
 
 //==============================================================================
@@ -869,5 +1033,22 @@ void oracle_residual(int64_t n, int64_t nrhs, const double* A, int64_t lda, int 
 { residual(n, nrhs, A, lda, sym, X, ldx, B, ldb, R, ldr); }
 
 void oracle_rotg(double* a, double* b, double* c, double* s) { rotg(a, b, c, s); }
+
+void oracle_rbt_weights(int64_t seed, int64_t n, double* u) { rbt_weights(seed, n, u); }
+
+int64_t oracle_rbt_half(int64_t n, int64_t nb, int64_t d, int64_t k) { return rbt_half(n, nb, d, k); }
+
+void oracle_gerbt(int64_t n, int64_t nb, int64_t d, const double* u, const double* v,
+                  double* A, int64_t lda)
+{ gerbt2(n, nb, d, u, v, A, lda); }
+
+void oracle_gerbt_rhs(int trans, int64_t n, int64_t nrhs, int64_t nb, int64_t d, const double* u,
+                      double* B, int64_t ldb)
+{ gerbt1(trans, n, nrhs, nb, d, u, B, ldb); }
+
+int64_t oracle_dgesv_rbt(int64_t n, int64_t nrhs, double* A, int64_t lda, const double* B, int64_t ldb,
+                         double* X, int64_t ldx, int64_t nb, int64_t ib, int64_t depth, int64_t itermax,
+                         double tol, int use_fallback, int* iter)
+{ return gesv_rbt(n, nrhs, A, lda, B, ldb, X, ldx, nb, ib, depth, itermax, tol, use_fallback, iter); }
 
 } // extern "C"

@@ -40,7 +40,8 @@ C_SYMBOLS = [
     "slate_amd_selector_features", "slate_amd_mlp_forward", "slate_amd_mlp_train_step",
     "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb", "slate_amd_dgesv_batched",
     "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_profile_enable",
-    "slate_amd_profile_read",
+    "slate_amd_profile_read", "slate_amd_dgesv_rbt", "slate_amd_drbt_generate", "slate_amd_dgerbt",
+    "slate_amd_dgerbt_rhs",
 ]
 
 
@@ -60,7 +61,7 @@ class Options(ctypes.Structure):
         ("use_fallback", ctypes.c_int32),
         ("factor_precision", ctypes.c_int32),
         ("method_lu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("depth", ctypes.c_int32),
     ]
 
     def __init__(self, **kw):
@@ -106,6 +107,10 @@ def lib():
             "slate_amd_dsgesv_gmres": (i64, [vp, i64, i64, vp, i64, pi64, vp, i64, vp, i64, pint, po]),
             "slate_amd_dsposv": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, vp, i64, pint, po]),
             "slate_amd_dsposv_gmres": (i64, [vp, i32, i64, i64, vp, i64, vp, i64, vp, i64, pint, po]),
+            "slate_amd_dgesv_rbt": (i64, [vp, i64, i64, vp, i64, vp, i64, vp, i64, pint, po]),
+            "slate_amd_drbt_generate": (i32, [vp, i64, i64, vp]),
+            "slate_amd_dgerbt": (i32, [vp, i64, i32, i64, vp, vp, vp, i64]),
+            "slate_amd_dgerbt_rhs": (i32, [vp, i32, i64, i64, i32, i64, vp, vp, i64]),
             "slate_amd_dnorm_inf": (i32, [vp, i32, i64, vp, i64, ctypes.POINTER(dbl)]),
             "slate_amd_generate": (i32, [vp, i32, i64, i64, i64, i64, i64, vp, i64]),
             "slate_amd_generate_svd": (i32, [vp, i32, dbl, i64, i32, i64, vp, i64]),
@@ -305,6 +310,46 @@ def posv_mixed_gmres(A, B, X, uplo=LOWER, opts=None):
     info = _check(lib().slate_amd_dsposv_gmres(context(dev), uplo, A.shape[0], _ncols(B), pa, lda,
                                                pb, ldb, px, ldx, ctypes.byref(it), _opts(opts)))
     return info, it.value
+
+
+#---------------------------------------------------------------- random butterflies
+def gesv_rbt(A, B, X, opts=None):
+    """slate::gesv_rbt (src/gesv_rbt.cc): returns (info, iter); X receives the solution."""
+    pa, lda, dev = _mat(A)
+    pb, ldb, _ = _mat(B)
+    px, ldx, _ = _mat(X)
+    it = ctypes.c_int(0)
+    info = _check(lib().slate_amd_dgesv_rbt(context(dev), A.shape[0], _ncols(B), pa, lda, pb, ldb,
+                                            px, ldx, ctypes.byref(it), _opts(opts)))
+    return info, it.value
+
+
+def rbt_generate(n, seed, device=None):
+    """Butterfly weights exp(r/20), r = matgen rands (seed, i, 0): a float64 CUDA vector."""
+    import torch
+    u = torch.empty(n, dtype=torch.float64, device=device or "cuda")
+    _check(lib().slate_amd_drbt_generate(context(u.device.index or 0), n, seed, u.data_ptr()))
+    return u
+
+
+def gerbt(u, A, v, depth, nb):
+    """slate::gerbt(U, A, V): A := U^T A V in place."""
+    pa, lda, dev = _mat(A)
+    pu, pv = _weights(u, A.shape[0]), _weights(v, A.shape[0])
+    _check(lib().slate_amd_dgerbt(context(dev), A.shape[0], depth, nb, pu, pv, pa, lda))
+
+
+def gerbt_rhs(u, B, depth, nb, trans):
+    """slate::gerbt(U, B): B := U^T B (trans) or U B (not trans) in place."""
+    pb, ldb, dev = _mat(B)
+    _check(lib().slate_amd_dgerbt_rhs(context(dev), TRANS if trans else NOTRANS, B.shape[0], _ncols(B),
+                                      depth, nb, _weights(u, B.shape[0]), pb, ldb))
+
+
+def _weights(u, n):
+    if u.dim() != 1 or u.shape[0] < n or u.stride(0) != 1:
+        raise SlateError("butterfly weights must be a contiguous vector of length >= n")
+    return _mat(u)[0]
 
 
 #---------------------------------------------------------------- utilities

@@ -105,6 +105,7 @@ struct Opts {
     bool fallback = true;
     int prec = SLATE_AMD_PREC_FP32;
     int method = SLATE_AMD_LU_PARTIALPIV;
+    int depth = 2;
 };
 
 // Number of bf16 terms per operand of the float LU's trailing update
@@ -132,6 +133,7 @@ static Opts read_opts(const slate_amd_options* o)
     r.fallback = o->use_fallback != 0;
     r.prec = o->factor_precision;
     r.method = o->method_lu;
+    r.depth = o->depth;
     SA_REQUIRE(r.method == SLATE_AMD_LU_PARTIALPIV || r.method == SLATE_AMD_LU_NOPIV
                || r.method == SLATE_AMD_LU_CALU, -2, "unknown method_lu");
     return r;
@@ -1132,6 +1134,100 @@ static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, 
     return info;
 }
 
+//------------------------------------------------------------------------------
+// gesv_rbt: src/gesv_rbt.cc:94-236.  A := U^T A V with depth-d random
+// butterflies, LU without pivoting, X = V (LU)^{-1} U^T B, then classic
+// refinement in FP64 against a copy of the original A; the PP gesv of the copy
+// is the fallback.  The butterfly weights are exp(r/20) with r the matgen
+// rands stream of seed 42 (U) and 42 + nt (V), nt = ceil(n/nb)
+// (internal_rbt_generate.cc:45-51, :133-134; the reference draws r with
+// LAPACK larnv instead).  A holds the factors of the transformed matrix on
+// return, or the original A when the fallback ran (gesv_rbt.cc:220-225).
+static int64_t gesv_rbt(Context& c, int64_t n, int64_t nrhs, double* A, int64_t lda, const double* B,
+                        int64_t ldb, double* X, int64_t ldx, int* iter, const Opts& o)
+{
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
+    const int64_t itermax = o.itermax;
+    const int depth = o.depth;
+    const int64_t nb = choose_nb(c, SLATE_AMD_TUNE_LU_FP64, n, o.nb);
+    *iter = 0;
+
+    const int64_t we = residual_work_elems(n, std::min<int64_t>(nrhs, 4));
+    c.vec.ensure(sizeof(double) * (2 * size_t(n) + size_t(n) * size_t(nrhs) + size_t(we) + size_t(nrhs) + 8)
+                 + 5 * 256);
+    char* p = static_cast<char*>(c.vec.p);
+    auto take = [&](size_t b) { char* q = p; p += (b + 255) & ~size_t(255); return q; };
+    double* u = reinterpret_cast<double*>(take(sizeof(double) * n));
+    double* v = reinterpret_cast<double*>(take(sizeof(double) * n));
+    double* R = reinterpret_cast<double*>(take(sizeof(double) * n * nrhs));
+    double* work = reinterpret_cast<double*>(take(sizeof(double) * we));
+    double* norms = reinterpret_cast<double*>(take(sizeof(double) * (nrhs + 8)));
+    rbt_weights(42, n, u, c.s0);
+    rbt_weights(42 + ceildiv(n, nb), n, v, c.s0);
+
+    double* Acopy = nullptr;
+    double Anorm = 0;
+    if (itermax > 0) {
+        c.lo.ensure(sizeof(double) * size_t(n) * size_t(n));
+        Acopy = c.lo.as<double>();
+        copy_convert<double, double>(A, lda, Acopy, n, n, n, c.s0);
+        Anorm = norm_host(c, A, n, lda, 0, work, we, norms);
+    }
+    copy_convert<double, double>(B, ldb, X, ldx, n, nrhs, c.s0);
+
+    // factor and solve the transformed system
+    gerbt2<double>(A, lda, n, nb, depth, u, v, c.s0);
+    Opts on = o;
+    on.method = SLATE_AMD_LU_NOPIV;
+    c.ipiv.ensure(sizeof(int64_t) * n);
+    int64_t* dip = c.ipiv.as<int64_t>();
+    int64_t info = getrf_cm<double>(c, A, n, lda, on, dip);
+    if (info == 0) {
+        gerbt1<double>(1, X, ldx, n, nrhs, nb, depth, u, c.s0);
+        getrs_cm<double>(c, 0, A, lda, n, dip, X, ldx, nrhs);
+        gerbt1<double>(0, X, ldx, n, nrhs, nb, depth, v, c.s0);
+    }
+    if (itermax == 0) {
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return info;
+    }
+
+    const double cte = Anorm * tol;
+    std::vector<double> nx, nr;
+    bool converged = false;
+    if (info == 0) {
+        residual(Acopy, n, n, 0, X, ldx, B, ldb, R, n, nrhs, work, we, c.s0);
+        colnorms_host(c, X, n, ldx, nrhs, norms, nx);
+        colnorms_host(c, R, n, n, nrhs, norms, nr);
+        if (converged_host(nr, nx, cte)) converged = true;
+        for (int64_t it = 0; it < itermax && !converged; ++it) {
+            gerbt1<double>(1, R, n, n, nrhs, nb, depth, u, c.s0);
+            getrs_cm<double>(c, 0, A, lda, n, dip, R, n, nrhs);
+            gerbt1<double>(0, R, n, n, nrhs, nb, depth, v, c.s0);
+            axpy(n, nrhs, 1.0, R, n, X, ldx, c.s0);
+            residual(Acopy, n, n, 0, X, ldx, B, ldb, R, n, nrhs, work, we, c.s0);
+            colnorms_host(c, X, n, ldx, nrhs, norms, nx);
+            colnorms_host(c, R, n, n, nrhs, norms, nr);
+            if (converged_host(nr, nx, cte)) { *iter = int(it + 1); converged = true; }
+        }
+    }
+    // A zero pivot of the unpivoted factor (info > 0) leaves X undefined; the
+    // reference would refine a non-finite X, which is "not converged" here.
+    if (!converged) {
+        *iter = int(-itermax - 1);
+        if (o.fallback) {
+            copy_convert<double, double>(Acopy, n, A, lda, n, n, c.s0);
+            info = getrf_cm<double>(c, Acopy, n, n, o, dip);
+            copy_convert<double, double>(B, ldb, X, ldx, n, nrhs, c.s0);
+            if (info == 0) getrs_cm<double>(c, 0, Acopy, n, n, dip, X, ldx, nrhs);
+        }
+    }
+    SA_HIP(hipStreamSynchronize(c.s0));
+    check_kernel_error(c);
+    return info;
+}
+
 // Givens rotation with the reference-BLAS drotg conventions.
 static void rotg(double* a, double* b, double* cc, double* ss)
 {
@@ -1691,7 +1787,7 @@ void slate_amd_options_init(slate_amd_options* o)
     o->use_fallback = 1;
     o->factor_precision = SLATE_AMD_PREC_FP32;
     o->method_lu = SLATE_AMD_LU_PARTIALPIV;
-    o->reserved = 0;
+    o->depth = 2;
 }
 
 double slate_amd_last_time_ms(slate_amd_context ctx) { return ctx ? ctx->c.last_ms : 0.0; }
@@ -1915,6 +2011,59 @@ int64_t slate_amd_dsposv_gmres(slate_amd_context ctx, int uplo, int64_t n, int64
         t.stop();
         return info;
     });
+}
+
+int64_t slate_amd_dgesv_rbt(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
+                            const double* dB, int64_t ldb, double* dX, int64_t ldx, int* iter,
+                            const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda); check_rhs(n, nrhs, ldb); check_rhs(n, nrhs, ldx);
+        SA_REQUIRE(iter != nullptr, -6, "null iter");
+        if (n == 0 || nrhs == 0) { *iter = 0; return 0; }
+        SA_ENTRY_SYNC();
+        sa::Opts o = sa::read_opts(opts);
+        sa::Timer t(c);
+        int64_t info = sa::gesv_rbt(c, n, nrhs, dA, lda, dB, ldb, dX, ldx, iter, o);
+        t.stop();
+        return info;
+    });
+}
+
+int slate_amd_drbt_generate(slate_amd_context ctx, int64_t n, int64_t seed, double* du)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(n >= 0, -2, "bad n");
+        SA_ENTRY_SYNC();
+        sa::rbt_weights(seed, n, du, c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return 0;
+    }));
+}
+
+int slate_amd_dgerbt(slate_amd_context ctx, int64_t n, int depth, int64_t nb, const double* du,
+                     const double* dv, double* dA, int64_t lda)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        SA_ENTRY_SYNC();
+        sa::gerbt2<double>(dA, lda, n, nb, depth, du, dv, c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return 0;
+    }));
+}
+
+int slate_amd_dgerbt_rhs(slate_amd_context ctx, int trans, int64_t n, int64_t nrhs, int depth, int64_t nb,
+                         const double* du, double* dB, int64_t ldb)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_rhs(n, nrhs, ldb);
+        SA_REQUIRE(trans == SLATE_AMD_NOTRANS || trans == SLATE_AMD_TRANS, -2, "bad trans");
+        SA_ENTRY_SYNC();
+        sa::gerbt1<double>(trans, dB, ldb, n, nrhs, nb, depth, du, c.s0);
+        SA_HIP(hipStreamSynchronize(c.s0));
+        return 0;
+    }));
 }
 
 int slate_amd_dgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, double alpha,

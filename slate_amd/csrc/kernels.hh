@@ -207,6 +207,17 @@ template <class S> void add_convert(const S* x, double* y, int64_t n, hipStream_
 void generate(int dist, int64_t seed, int64_t m, int64_t n, int64_t ioff, int64_t joff,
               double* A, int64_t lda, hipStream_t s);
 
+// Random butterfly transforms (rbt.hip): half length of level k, the
+// two-sided A := U^T A V, the one-sided B := U^T B (trans = 1) or B := V B
+// (trans = 0), and the weights u[i] = exp(rands(seed, i, 0) / 20).
+int64_t rbt_half(int64_t n, int64_t nb, int depth, int k);
+template <class T>
+void gerbt2(T* A, int64_t lda, int64_t n, int64_t nb, int depth, const T* u, const T* v, hipStream_t s);
+template <class T>
+void gerbt1(int trans, T* B, int64_t ldb, int64_t n, int64_t nrhs, int64_t nb, int depth, const T* u,
+            hipStream_t s);
+void rbt_weights(int64_t seed, int64_t n, double* u, hipStream_t s);
+
 // Fill an int/unsigned buffer with zeros (graph-safe replacement for memset).
 void zero_u32(unsigned* p, int64_t n, hipStream_t s);
 void zero_f64(double* p, int64_t n, hipStream_t s);

@@ -175,6 +175,12 @@ int main(int argc, char** argv)
         std::printf("gesv_mixed_gmres info=%ld iter=%d scaled_residual=%.3f\n", long(info), iter, r);
         CHECK(info == 0 && iter >= 0, "gesv_mixed_gmres info %ld iter %d", long(info), iter);
         CHECK(r <= 3.0, "gesv_mixed_gmres residual %g", r);
+        iter = -99;
+        slate_amd::gesv_rbt(A, B, X, iter);
+        r = scaled_residual(A0, to_host(dX, size_t(n)), B0, n, 1, false);
+        std::printf("gesv_rbt iter=%d scaled_residual=%.3f\n", iter, r);
+        CHECK(iter >= 0, "gesv_rbt iter %d", iter);
+        CHECK(r <= 3.0, "gesv_rbt residual %g", r);
         (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dX);
 
         std::vector<double> Sh(S), Bh = generate(2, 1, ns, 1), Xh(static_cast<size_t>(ns), 0.0);

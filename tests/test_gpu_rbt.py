@@ -1,0 +1,142 @@
+"""GPU parity of the random butterfly transforms (gerbt, gerbt on a right-hand
+side, the weight generator) and of the RBT solver gesv_rbt against the oracle
+(src/gerbt.cc, src/internal/Tile_gerbt.hh, src/gesv_rbt.cc).
+
+The transforms are elementwise arithmetic with a fixed operation order, so the
+GPU result is compared at a few ulps (tolerance 1e-14 relative).  gesv_rbt's
+iteration counts depend on rounding in the unpivoted LU and are bounded, not
+pinned; the FP64 solution must meet the scaled-residual bound, and the
+fallback must reproduce the plain gesv.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import slate_amd
+from helpers import dev_rands, say, scaled_residual, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(256, 32, 2), (1000, 96, 2), (1000, 96, 3), (777, 64, 1), (45, 7, 2), (20, 32, 2), (4096, 256, 4)]
+
+
+def test_rbt_weights_match_oracle(gpu):
+    say("rbt_generate n=5000")
+    u = slate_amd.rbt_generate(5000, 42)
+    # the device exp and the host libm exp may differ in the last bit
+    np.testing.assert_allclose(u.cpu().numpy(), oracle.rbt_weights(42, 5000), rtol=4.5e-16, atol=0)
+
+
+@pytest.mark.parametrize("n,nb,d", CASES)
+def test_gerbt_matches_oracle(gpu, n, nb, d):
+    say("gerbt n=%d nb=%d depth=%d" % (n, nb, d))
+    rng = np.random.default_rng(n + nb + d)
+    H = rng.standard_normal((n, n))
+    uh, vh = oracle.rbt_weights(42, n), oracle.rbt_weights(43, n)
+    A = to_dev(H)
+    slate_amd.gerbt(to_dev(uh.reshape(-1, 1))[:, 0], A, to_dev(vh.reshape(-1, 1))[:, 0], d, nb)
+    ref = oracle.gerbt(uh, H, vh, d, nb)
+    np.testing.assert_allclose(to_host(A), ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n,nb,d", CASES)
+@pytest.mark.parametrize("trans", [True, False])
+def test_gerbt_rhs_matches_oracle(gpu, n, nb, d, trans):
+    say("gerbt_rhs n=%d nb=%d depth=%d trans=%d" % (n, nb, d, trans))
+    rng = np.random.default_rng(7 * n + d)
+    Bh = rng.standard_normal((n, 5))
+    uh = oracle.rbt_weights(9, n)
+    # leading dimension larger than n
+    Bp = slate_amd.empty_colmajor(n, 5, torch.float64, ld=n + 3)
+    Bp.copy_(to_dev(Bh))
+    slate_amd.gerbt_rhs(to_dev(uh.reshape(-1, 1))[:, 0], Bp, d, nb, trans)
+    ref = oracle.gerbt_rhs(uh, Bh, d, nb, trans)
+    np.testing.assert_allclose(to_host(Bp), ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n,nb,nrhs", [(512, 64, 1), (1000, 96, 3), (2048, 256, 1)])
+def test_gesv_rbt_matches_oracle(gpu, n, nb, nrhs):
+    say("gesv_rbt n=%d nb=%d nrhs=%d" % (n, nb, nrhs))
+    A = dev_rands(0, n, n)
+    B = dev_rands(1, n, nrhs)
+    X = slate_amd.empty_colmajor(n, nrhs, torch.float64)
+    info, it = slate_amd.gesv_rbt(A, B, X, slate_amd.Options(block_size=nb))
+    H, Bh = oracle.rands(0, n, n), oracle.rands(1, n, nrhs)
+    A_o, X_o, info_o, it_o = oracle.gesv_rbt(H, Bh, nb=nb)
+    say("  iter gpu %d oracle %d" % (it, it_o))
+    assert info == info_o == 0
+    assert it >= 0 and it_o >= 0 and abs(it - it_o) <= 2
+    assert scaled_residual(H, to_host(X), Bh) <= 3.0
+    np.testing.assert_allclose(to_host(X), X_o, rtol=1e-9, atol=1e-9 * np.abs(X_o).max())
+    # A holds the unpivoted factors of U^T A V: L U reproduces the transform to
+    # the backward error bound of LU, |L U - T| <= c n eps |L| |U| (unpivoted
+    # LU can grow, so the bound scales with |L| |U|, not with |T|)
+    T = oracle.gerbt(oracle.rbt_weights(42, n), H, oracle.rbt_weights(42 + -(-n // nb), n), 2, nb)
+    for F in (to_host(A), A_o):
+        L, U = np.tril(F, -1) + np.eye(n), np.triu(F)
+        err = np.abs(L @ U - T).max() / (np.abs(L) @ np.abs(U)).max()
+        say("  LU backward error %.2e" % err)
+        assert err <= n * np.finfo(np.float64).eps
+
+
+def test_gesv_rbt_large(gpu):
+    n = 8192
+    say("gesv_rbt n=%d (auto nb)" % n)
+    A = dev_rands(0, n, n)
+    B = dev_rands(1, n, 1)
+    X = slate_amd.empty_colmajor(n, 1, torch.float64)
+    info, it = slate_amd.gesv_rbt(A, B, X)
+    assert info == 0 and 0 <= it <= 30
+    H, Bh = oracle.rands(0, n, n), oracle.rands(1, n, 1)
+    assert scaled_residual(H, to_host(X), Bh) <= 3.0
+
+
+def test_gesv_rbt_no_refinement(gpu):
+    say("gesv_rbt itermax=0 n=700")
+    n = 700
+    A = dev_rands(3, n, n)
+    B = dev_rands(4, n, 1)
+    X = slate_amd.empty_colmajor(n, 1, torch.float64)
+    info, it = slate_amd.gesv_rbt(A, B, X, slate_amd.Options(block_size=64, max_iterations=0))
+    assert info == 0 and it == 0
+    H, Bh = oracle.rands(3, n, n), oracle.rands(4, n, 1)
+    _, X_o, _, _ = oracle.gesv_rbt(H, Bh, nb=64, itermax=0)
+    np.testing.assert_allclose(to_host(X), X_o, rtol=1e-7, atol=1e-7 * np.abs(X_o).max())
+
+
+def test_gesv_rbt_fallback(gpu):
+    say("gesv_rbt fallback n=600")
+    n = 600
+    A = dev_rands(5, n, n)
+    B = dev_rands(6, n, 2)
+    X = slate_amd.empty_colmajor(n, 2, torch.float64)
+    A_in = A.clone()
+    info, it = slate_amd.gesv_rbt(A, B, X, slate_amd.Options(block_size=128, max_iterations=2,
+                                                            tolerance=1e-300))
+    assert info == 0 and it == -3
+    assert torch.equal(A, A_in)                      # the original A is restored
+    A2, B2 = A_in.clone(), B.clone()
+    slate_amd.gesv(A2, B2, slate_amd.Options(block_size=128))
+    assert torch.equal(X, B2)                        # X is the plain PP gesv solution
+
+
+def test_gesv_rbt_depth_zero_is_nopiv(gpu):
+    say("gesv_rbt depth=0 n=300")
+    n = 300
+    H = oracle.rands(8, n, n) + n * np.eye(n)         # diagonally dominant: safe without pivoting
+    A = to_dev(H)
+    B = dev_rands(9, n, 1)
+    X = slate_amd.empty_colmajor(n, 1, torch.float64)
+    info, it = slate_amd.gesv_rbt(A, B, X, slate_amd.Options(block_size=32, depth=0))
+    assert info == 0 and it >= 0
+    A2 = to_dev(H)
+    slate_amd.getrf(A2, slate_amd.Options(block_size=32, method_lu=slate_amd.LU_NOPIV))
+    np.testing.assert_allclose(to_host(A), to_host(A2), rtol=1e-13, atol=1e-13)
+
+
+def test_gesv_rbt_empty(gpu):
+    A = slate_amd.empty_colmajor(0, 0, torch.float64)
+    B = slate_amd.empty_colmajor(0, 1, torch.float64)
+    X = slate_amd.empty_colmajor(0, 1, torch.float64)
+    assert slate_amd.gesv_rbt(A, B, X) == (0, 0)
