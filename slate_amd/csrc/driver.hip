@@ -197,6 +197,12 @@ static void trsm_rec(const T* M, int64_t ld, int cm, int lower, int unit, int64_
     if (nb <= 0 || N <= 0) return;
     constexpr int64_t L = TRSM_LEAF;
     if (nb <= L) { trsm_leaf<T>(M, ld, cm, lower, unit, nb, B, ldb, N, s); return; }
+    // the unit-lower row-major solve of the panel, of L11^{-1} and of getrs
+    // runs as one launch up to trsm_fused_max rows
+    if (cm == 0 && lower && unit && nb <= trsm_fused_max(int(sizeof(T)))) {
+        trsm_lunit_fused<T>(M, ld, nb, B, ldb, N, s);
+        return;
+    }
     int64_t n1 = ((nb / 2 + L - 1) / L) * L;
     if (n1 >= nb) n1 = nb - L;
     int64_t n2 = nb - n1;

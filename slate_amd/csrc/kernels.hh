@@ -126,6 +126,11 @@ constexpr int TRSM_LEAF = 32;
 template <class T>
 void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
                T* B, int64_t ldb, int64_t N, hipStream_t s);
+// Unit-lower row-major triangle (m <= trsm_fused_max) solved against a
+// row-major B (m x N) in one launch: L X = B in place.
+int64_t trsm_fused_max(int elem_bytes);
+template <class T>
+void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s);
 
 // One-launch triangular solve of tri * x = b (one right-hand side, x overwrites
 // b, contiguous) with a look-back chain between blocks of trsv_block() rows.

@@ -109,6 +109,131 @@ template void trsm_leaf<float>(const float*, int64_t, int, int, int, int64_t, fl
                                int64_t, hipStream_t);
 
 //------------------------------------------------------------------------------
+// L X = B in place for a unit lower triangle (tri(i, k) = L[i*ldl + k],
+// m x m) and a row-major B (m x N, ldb), in one launch, m <= trsm_fused_max.
+// Workgroup x owns the columns [8x, 8x + 8), kept in LDS k-contiguous.  The
+// sweep is left-looking over 32-row blocks: every thread owns one (row,
+// column) of the block and forms y = b - L(i, 0:p0) X(0:p0, c) with 16-byte
+// loads of the L row (L1/L2 resident: every workgroup reads the same
+// triangle) and of the X column; then the block's forward substitution runs
+// in registers, each wave holding two columns (one per 32-lane half) and
+// broadcasting each solved entry with v_readlane.  Two barriers per block,
+// no launches inside the sweep: the recursive form needs 2 m/32 - 1 launches.
+template <class T> __device__ __forceinline__ T bcast_half(T v, int k, bool upper);
+template <> __device__ __forceinline__ float bcast_half<float>(float v, int k, bool upper)
+{
+    const int iv = __builtin_bit_cast(int, v);
+    const int a = __builtin_amdgcn_readlane(iv, k);
+    const int b = __builtin_amdgcn_readlane(iv, k + 32);
+    return __builtin_bit_cast(float, upper ? b : a);
+}
+template <> __device__ __forceinline__ double bcast_half<double>(double v, int k, bool upper)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = int(uint32_t(u)), hi = int(uint32_t(u >> 32));
+    const uint32_t alo = uint32_t(__builtin_amdgcn_readlane(lo, k));
+    const uint32_t ahi = uint32_t(__builtin_amdgcn_readlane(hi, k));
+    const uint32_t blo = uint32_t(__builtin_amdgcn_readlane(lo, k + 32));
+    const uint32_t bhi = uint32_t(__builtin_amdgcn_readlane(hi, k + 32));
+    const uint64_t r = upper ? (uint64_t(bhi) << 32 | blo) : (uint64_t(ahi) << 32 | alo);
+    return __builtin_bit_cast(double, r);
+}
+
+template <class T> struct Vec16;
+template <> struct Vec16<float> { typedef float __attribute__((ext_vector_type(4))) type; static constexpr int n = 4; };
+template <> struct Vec16<double> { typedef double __attribute__((ext_vector_type(2))) type; static constexpr int n = 2; };
+
+template <class T>
+__global__ __launch_bounds__(256)
+void trsm_lunit_kernel(const T* __restrict__ L, int64_t ldl, int m, T* __restrict__ B, int64_t ldb,
+                       int64_t N, int ldx, int vec_ok)
+{
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::n;
+    extern __shared__ __align__(16) unsigned char trsm_lds[];
+    T* Xs = reinterpret_cast<T*>(trsm_lds);            // column c at Xs + c*ldx
+    T* Ls = Xs + 8 * ldx;                               // 32 x 33 strictly lower diagonal block
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = lane & 31;
+    const bool upper = lane >= 32;
+    const int cl = 2 * (tid >> 6) + (upper ? 1 : 0);
+    const int64_t j0 = int64_t(blockIdx.x) * 8;
+    {
+        const int c = tid & 7;
+        const bool cv = j0 + c < N;
+        for (int i = tid >> 3; i < m; i += 32)
+            Xs[c * ldx + i] = cv ? B[int64_t(i) * ldb + j0 + c] : T(0);
+    }
+    T* Xc = Xs + cl * ldx;
+    for (int p0 = 0; p0 < m; p0 += 32) {
+        for (int e = tid; e < 32 * 32; e += 256) {
+            const int a = e >> 5, b = e & 31;
+            Ls[a * 33 + b] = (b < a && p0 + a < m) ? L[int64_t(p0 + a) * ldl + p0 + b] : T(0);
+        }
+        __syncthreads();
+        const int i = p0 + r;
+        T y = T(0);
+        if (i < m) {
+            const T* Li = L + int64_t(i) * ldl;
+            T acc[VN];
+            #pragma unroll
+            for (int e = 0; e < VN; ++e) acc[e] = T(0);
+            if (vec_ok) {
+                #pragma unroll 4
+                for (int k = 0; k < p0; k += VN) {
+                    const V l = *reinterpret_cast<const V*>(Li + k);
+                    const V x = *reinterpret_cast<const V*>(Xc + k);
+                    #pragma unroll
+                    for (int e = 0; e < VN; ++e) acc[e] += l[e] * x[e];
+                }
+            }
+            else {
+                for (int k = 0; k < p0; ++k) acc[0] += Li[k] * Xc[k];
+            }
+            T sum = acc[0];
+            #pragma unroll
+            for (int e = 1; e < VN; ++e) sum += acc[e];
+            y = Xc[i] - sum;
+        }
+        T lr[32];
+        #pragma unroll
+        for (int k = 0; k < 32; ++k) lr[k] = Ls[r * 33 + k];
+        #pragma unroll
+        for (int k = 0; k < 31; ++k) {
+            const T xk = bcast_half<T>(y, k, upper);
+            if (r > k) y -= lr[k] * xk;
+        }
+        if (i < m) Xc[i] = y;
+        __syncthreads();
+    }
+    {
+        const int c = tid & 7;
+        if (j0 + c < N)
+            for (int i = tid >> 3; i < m; i += 32) B[int64_t(i) * ldb + j0 + c] = Xs[c * ldx + i];
+    }
+}
+
+int64_t trsm_fused_max(int elem_bytes) { return elem_bytes == 8 ? 768 : 1024; }
+
+template <class T>
+void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s)
+{
+    if (m <= 0 || N <= 0) return;
+    SA_REQUIRE(m <= trsm_fused_max(int(sizeof(T))), -2, "trsm_lunit_fused: triangle too large");
+    SA_REQUIRE(ceildiv(N, 8) < (int64_t(1) << 31), -2, "trsm_lunit_fused: too many columns");
+    constexpr int VN = Vec16<T>::n;
+    const int ldx = int(ceildiv(m, 32) * 32) + VN;
+    const size_t lds = sizeof(T) * (size_t(8) * size_t(ldx) + 32 * 33);
+    const int vec_ok = (reinterpret_cast<uintptr_t>(L) % 16 == 0 && ldl % VN == 0) ? 1 : 0;
+    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(N, 8))), dim3(256), lds, s,
+                       L, ldl, int(m), B, ldb, N, ldx, vec_ok);
+    SA_CHECK_LAUNCH();
+}
+
+template void trsm_lunit_fused<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
+template void trsm_lunit_fused<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);
+
+//------------------------------------------------------------------------------
 // Right-looking upper Cholesky (A = U^T U) of one nb x nb block in LDS.
 // Row-major upper == column-major lower, so this is LAPACK's lower dpotrf on
 // the caller's column-major tile.  Stops at the first non-positive pivot.
