@@ -284,14 +284,17 @@ static void unit_lower_inverse(const T* L, int64_t ld, int64_t nb, T* inv, hipSt
 
 //------------------------------------------------------------------------------
 // Recursive panel LU of rows [c0, n) x columns [c0, c0+w) of a row-major n x n
-// matrix (the panel's top row is the diagonal).  Leaves of width <= W use
-// the cross-workgroup kernel; the splits apply the left half's interchanges
-// to the right half, solve with its unit-lower block, update with the GEMM,
-// recurse, and bring the right half's interchanges back to the left half.
+// matrix (the panel's top row is the diagonal), inside the enclosing panel
+// columns [p0, p0 + pw).  Leaves of width <= W use the cross-workgroup
+// kernel, which also applies its interchanges to the rest of the enclosing
+// panel; the splits solve with the left half's unit-lower block, update with
+// the GEMM and recurse.  (The CALU leaf only swaps its own columns: there the
+// splits apply the left half's interchanges to the right half first and bring
+// the right half's back to the left half at the end.)
 template <class T>
 static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64_t w,
                       int64_t* ipiv, int64_t* info, double thr, int64_t leaf_w,
-                      int64_t* plan, hipStream_t s)
+                      int64_t* plan, hipStream_t s, int64_t p0, int64_t pw)
 {
     if (w <= leaf_w) {
         if (c.calu) {
@@ -299,7 +302,7 @@ static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64
             c.leaf_tag += 16;
         }
         else {
-            panel_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, thr, panel_work(c), c.leaf_tag, s);
+            panel_leaf<T>(A, ld, c0, n, c0, int(w), ipiv, info, thr, panel_work(c), c.leaf_tag, s, p0, pw);
             c.leaf_tag += uint32_t(w);
         }
         return;
@@ -307,8 +310,8 @@ static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64
     int64_t w1 = ((w / 2 + leaf_w - 1) / leaf_w) * leaf_w;
     if (w1 >= w) w1 = w - leaf_w;
     int64_t w2 = w - w1;
-    panel_rec<T>(c, A, ld, n, c0, w1, ipiv, info, thr, leaf_w, plan, s);
-    laswp<T>(A, ld, n, c0 + w1, c0 + w, c0, c0 + w1, ipiv, plan, s);
+    panel_rec<T>(c, A, ld, n, c0, w1, ipiv, info, thr, leaf_w, plan, s, p0, pw);
+    if (c.calu) laswp<T>(A, ld, n, c0 + w1, c0 + w, c0, c0 + w1, ipiv, plan, s);
     trsm_rec<T>(A + c0 * ld + c0, ld, 0, 1, 1, w1, A + c0 * ld + c0 + w1, ld, w2, s);
     GemmArgs g{};
     g.M = n - c0 - w1; g.N = w2; g.K = w1;
@@ -317,8 +320,8 @@ static void panel_rec(Context& c, T* A, int64_t ld, int64_t n, int64_t c0, int64
     g.C = A + (c0 + w1) * ld + c0 + w1; g.ldc = ld;
     g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
     gemm<T>(g, A_KC, B_NC, s);
-    panel_rec<T>(c, A, ld, n, c0 + w1, w2, ipiv, info, thr, leaf_w, plan, s);
-    laswp<T>(A, ld, n, c0, c0 + w1, c0 + w1, c0 + w, ipiv, plan, s);
+    panel_rec<T>(c, A, ld, n, c0 + w1, w2, ipiv, info, thr, leaf_w, plan, s, p0, pw);
+    if (c.calu) laswp<T>(A, ld, n, c0, c0 + w1, c0 + w1, c0 + w, ipiv, plan, s);
 }
 
 // Panel of the no-pivoting LU (src/getrf_nopiv.cc): the kb x kb diagonal
@@ -330,7 +333,7 @@ template <class T>
 static void panel_nopiv(Context& c, T* A, int64_t ld, int64_t n, int64_t k0, int64_t kb, int64_t* ipiv,
                         int64_t* info, int64_t leaf_w, int64_t* plan, T* uinv, T* tmp, hipStream_t s)
 {
-    panel_rec<T>(c, A, ld, k0 + kb, k0, kb, ipiv, info, 0.0, leaf_w, plan, s);
+    panel_rec<T>(c, A, ld, k0 + kb, k0, kb, ipiv, info, 0.0, leaf_w, plan, s, k0, kb);
     const int64_t m = n - k0 - kb;
     if (m <= 0) return;
     T* Akk = A + k0 * ld + k0;
@@ -486,7 +489,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             panel_nopiv<T>(c, A, ld, n, p0, pw, ipiv, info, leaf_w, plan1, npu, npt, s1);
         }
         else {
-            panel_rec<T>(c, A, ld, n, p0, pw, ipiv, info, o.threshold, leaf_w, plan1, s1);
+            panel_rec<T>(c, A, ld, n, p0, pw, ipiv, info, o.threshold, leaf_w, plan1, s1, p0, pw);
             laswp_plan(p0, p0 + pw, n, ipiv, step_plan(k), s1);
         }
     };

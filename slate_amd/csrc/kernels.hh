@@ -85,10 +85,12 @@ struct PanelWork {
 // Column j exchanges words tagged tag0 + j: the caller hands every launch on
 // the same slot area fresh tags (it advances tag0 by w), and slot words start
 // at tag 0, which is never used.
+// With xpw > 0 the leaf also applies its interchanges to the other columns
+// of the enclosing panel [xp0, xp0 + xpw) (rows from r0 = c0 down).
 template <class T>
 void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
                 int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
-                uint32_t tag0, hipStream_t s);
+                uint32_t tag0, hipStream_t s, int64_t xp0 = 0, int64_t xpw = 0);
 int panel_leaf_width(int elem_bytes);   // largest leaf width the kernel supports
 
 // Tournament-pivoting leaf (MethodLU::CALU): same contract as panel_leaf

@@ -70,6 +70,85 @@ __device__ inline bool better(T a, int64_t ia, T b, int64_t ib)
 }
 
 //------------------------------------------------------------------------------
+// The leaf's w interchanges (row j <-> row piv[j], sequentially, rows local to
+// r0) applied to the other columns of the enclosing panel, [xp0, c0) and
+// [c0 + w, xp0 + xpw), so that the recursive panel needs no row-swap launches
+// of its own.  Both ranges are untouched while the leaf runs: the right part
+// is only read by the solves that follow the leaf, the left part (finished
+// columns of L) only needs the interchanges brought back.  Every workgroup
+// builds the same list of at most 2w (dst <- src) row moves: a touched row
+// r receives the original row t_0(t_1(...t_{w-1}(r))), t_j the j-th
+// transposition; then it moves its share of the columns through LDS
+// (gather, barrier, scatter), so sources are read before any is overwritten.
+template <class T, int W, int NT>
+__device__ void leaf_swap_others(T* __restrict__ A, int64_t ld, int64_t r0, int64_t c0, int w, int64_t xp0,
+                                 int64_t xpw, const int* s_pivs, int G, int g)
+{
+    constexpr int CH = 32;                      // columns per chunk
+    __shared__ int s_dst[2 * W], s_src[2 * W];
+    __shared__ int s_nmv;
+    __shared__ T s_stage[2 * W * CH];
+    const int tid = threadIdx.x;
+    if (tid == 0) s_nmv = 0;
+    __syncthreads();                            // s_pivs complete, counter cleared
+    if (tid < 2 * W) {
+        const int t = tid < W ? tid : tid - W;
+        int dst = -1;
+        if (t < w) {
+            if (tid < W) {
+                dst = t;
+            }
+            else {
+                const int pt = s_pivs[t];
+                bool first = pt >= w;
+                for (int u = 0; u < t; ++u) first = first && s_pivs[u] != pt;
+                if (first) dst = pt;
+            }
+        }
+        if (dst >= 0) {
+            int x = dst;
+            for (int jj = w - 1; jj >= 0; --jj) {
+                const int pj = s_pivs[jj];
+                x = (x == jj) ? pj : (x == pj ? jj : x);
+            }
+            if (x != dst) {
+                const int slot = atomicAdd(&s_nmv, 1);
+                s_dst[slot] = dst;
+                s_src[slot] = x;
+            }
+        }
+    }
+    __syncthreads();
+    const int nmv = s_nmv;
+    if (nmv == 0) return;
+    const int64_t nother = xpw - w, left = c0 - xp0;
+    const int64_t share = (nother + G - 1) / G;
+    const int64_t q0 = int64_t(g) * share;
+    const int64_t q1 = q0 + share < nother ? q0 + share : nother;
+    for (int64_t qc = q0; qc < q1; qc += CH) {
+        const int nc = int(q1 - qc < CH ? q1 - qc : CH);
+        for (int e = tid; e < nmv * CH; e += NT) {
+            const int mv = e / CH, cc = e % CH;
+            if (cc < nc) {
+                const int64_t q = qc + cc;
+                const int64_t col = q < left ? xp0 + q : c0 + w + (q - left);
+                s_stage[e] = A[(r0 + s_src[mv]) * ld + col];
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < nmv * CH; e += NT) {
+            const int mv = e / CH, cc = e % CH;
+            if (cc < nc) {
+                const int64_t q = qc + cc;
+                const int64_t col = q < left ? xp0 + q : c0 + w + (q - left);
+                A[(r0 + s_dst[mv]) * ld + col] = s_stage[e];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+//------------------------------------------------------------------------------
 // Leaf with rotated register columns.  v[q][c] holds column (j + c) mod W at
 // step j, so the active column is always v[q][0] and every other access uses
 // compile-time register indices (no per-element column selects); one
@@ -83,7 +162,7 @@ template <class T, int W, int RPT, int VEC, int NT>
 __global__ __launch_bounds__(NT)
 void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0,
                            int w, int64_t* __restrict__ ipiv, int64_t* __restrict__ info, T threshold,
-                           uint32_t tag0, uint64_t* slots, int* error)
+                           uint32_t tag0, uint64_t* slots, int* error, int64_t xp0, int64_t xpw)
 {
     using SL = LeafSlots<T, W>;
     constexpr int NWV = SL::NWV, CW = SL::CW;
@@ -110,6 +189,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
     __shared__ __attribute__((aligned(16))) T s_piv[W];    // pivot row, rotated
     __shared__ __attribute__((aligned(16))) T s_prot[W];   // pivot row, rotated, 0 outside columns j+1 .. w-1
     __shared__ int64_t s_p;
+    __shared__ int s_pivs[W];                   // local pivot row of each column
 
     constexpr int VE = 16 / int(sizeof(T));
     T v[RPT][W];
@@ -270,6 +350,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
         }
         __syncthreads();
         const int64_t p = s_p;
+        if (tid == 0) s_pivs[j] = int(p);
 
         // (4) interchange rows j and p among the rows this thread owns
         #pragma unroll
@@ -352,6 +433,7 @@ void panel_leaf_rot_kernel(T* __restrict__ A, int64_t ld, int64_t r0, int64_t m_
             }
         }
     }
+    if (xpw > w) leaf_swap_others<T, W, NT>(A, ld, r0, c0, w, xp0, xpw, s_pivs, G, g);
 }
 
 // Leaf shape (NT threads, RPT rows each).  Measured per column: FP32 W = 32
@@ -371,7 +453,7 @@ static void leaf_shape(int64_t m, int& nt, int& rpt)
 template <class T>
 void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
                 int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
-                uint32_t tag0, hipStream_t s)
+                uint32_t tag0, hipStream_t s, int64_t xp0, int64_t xpw)
 {
     constexpr int W = LeafW<T>::W;
     const int64_t m = m_total - r0;
@@ -382,6 +464,8 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
     leaf_shape(m, nt, rpt);
     int64_t G = ceildiv(m, int64_t(nt) * rpt);
     SA_REQUIRE(G <= LEAF_GMAX, -2, "panel_leaf: panel too tall");
+    SA_REQUIRE(xpw == 0 || (r0 == c0 && xp0 <= c0 && c0 + w <= xp0 + xpw), -2,
+               "panel_leaf: leaf outside the enclosing panel");
     const int64_t slot_bytes = int64_t(2 * sizeof(uint64_t)) * LeafSlots<T, W>::per(int(G));
     SA_REQUIRE(slot_bytes <= pw.slots_bytes, -2, "panel_leaf: slot area too small");
     T thr = T(threshold);
@@ -389,7 +473,8 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
     const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (ld % VE == 0) && (c0 % VE == 0);
     uint64_t* slots = static_cast<uint64_t*>(pw.slots);
 #define SA_LEAF(NTV, R, V) hipLaunchKernelGGL((panel_leaf_rot_kernel<T, W, R, V, NTV>), dim3(G), dim3(NTV), 0, s, \
-                                               A, ld, r0, m_total, c0, w, ipiv, info, thr, tag0, slots, pw.error)
+                                               A, ld, r0, m_total, c0, w, ipiv, info, thr, tag0, slots, pw.error, \
+                                               xp0, xpw)
 #define SA_LEAF_V(NTV, R) do { if (vec) SA_LEAF(NTV, R, 1); else SA_LEAF(NTV, R, 0); } while (0)
     if (nt == 256) SA_LEAF_V(256, 1);
     else if (rpt == 1) SA_LEAF_V(512, 1);
@@ -400,9 +485,9 @@ void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w
 }
 
 template void panel_leaf<double>(double*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t*,
-                                 double, const PanelWork&, uint32_t, hipStream_t);
+                                 double, const PanelWork&, uint32_t, hipStream_t, int64_t, int64_t);
 template void panel_leaf<float>(float*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t*,
-                                double, const PanelWork&, uint32_t, hipStream_t);
+                                double, const PanelWork&, uint32_t, hipStream_t, int64_t, int64_t);
 
 //------------------------------------------------------------------------------
 // Row interchanges.  The sequential LAPACK-style swap list (i, ipiv[i]),
