@@ -499,26 +499,38 @@ template void panel_leaf<float>(float*, int64_t, int64_t, int64_t, int64_t, int,
 // them, which keeps every row access coalesced in the row-major layout.
 static constexpr int PLAN_MAX = 2048;
 
+// Rows are kept relative to k1 as 32-bit ints in LDS, padded to a multiple
+// of 4 with identity transpositions, so both per-thread sweeps (the duplicate
+// test and the backward walk of src_of) read four entries per broadcast LDS
+// load and do only register compares in between.
 __global__ __launch_bounds__(1024)
 void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __restrict__ ipiv,
                        int64_t* __restrict__ plan)
 {
-    __shared__ int64_t piv[PLAN_MAX];
+    __shared__ __attribute__((aligned(16))) int piv[PLAN_MAX];
     __shared__ int keep[PLAN_MAX];
     __shared__ int scan[PLAN_MAX];
+    typedef int int4_t __attribute__((ext_vector_type(4)));
     const int tid = threadIdx.x, nt = blockDim.x;
     const int np = int(k2 - k1);
-    for (int i = tid; i < np; i += nt) {
-        int64_t p = ipiv[k1 + i];
-        piv[i] = (p >= k1 + i && p < nrows) ? p : k1 + i;   // out-of-range -> no swap
+    const int np4 = (np + 3) & ~3;
+    for (int i = tid; i < np4; i += nt) {
+        int v = i;                                          // padding / out-of-range: no swap
+        if (i < np) {
+            const int64_t p = ipiv[k1 + i];
+            if (p >= k1 + i && p < nrows) v = int(p - k1);
+        }
+        piv[i] = v;
     }
     __syncthreads();
     for (int i = tid; i < np; i += nt) {
-        int64_t p = piv[i];
-        int k = 1;
-        // independent LDS reads, no loop-carried exit: pipelines well
-        for (int i2 = 0; i2 < i; ++i2) k &= (piv[i2] != p);
-        k &= (p >= k2);
+        const int p = piv[i];
+        int k = p >= np;                                    // a row below the swapped block
+        for (int i2 = 0; i2 < (i & ~3); i2 += 4) {
+            const int4_t q = *reinterpret_cast<const int4_t*>(&piv[i2]);
+            k &= (q[0] != p) & (q[1] != p) & (q[2] != p) & (q[3] != p);
+        }
+        for (int i2 = i & ~3; i2 < i; ++i2) k &= (piv[i2] != p);
         keep[i] = k;
         scan[i] = k;
     }
@@ -535,26 +547,27 @@ void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __r
     }
     const int nkeep = np > 0 ? scan[np - 1] : 0;
     const int count = np + nkeep;
-    auto src_of = [&](int64_t r) {
-        int64_t x = r;
-        for (int i = np - 1; i >= 0; --i) {
-            int64_t a = k1 + i, b = piv[i];
-            if (x == a) x = b;
-            else if (x == b) x = a;
+    // source of relative row x: t_0(t_1(...t_{np-1}(x))), t_i = (i piv[i])
+    auto src_of = [&](int x) {
+        for (int i = np4 - 4; i >= 0; i -= 4) {
+            const int4_t q = *reinterpret_cast<const int4_t*>(&piv[i]);
+            #pragma unroll
+            for (int u = 3; u >= 0; --u) {
+                const int a = i + u, b = q[u];
+                x = (x == a) ? b : ((x == b) ? a : x);
+            }
         }
-        return x;
+        return k1 + int64_t(x);
     };
     for (int e = tid; e < np; e += nt) {
-        int64_t r = k1 + e;
-        plan[1 + 2 * e] = r;
-        plan[2 + 2 * e] = src_of(r);
+        plan[1 + 2 * e] = k1 + e;
+        plan[2 + 2 * e] = src_of(e);
     }
     for (int i = tid; i < np; i += nt)
         if (keep[i]) {
-            int e = np + scan[i] - 1;
-            int64_t r = piv[i];
-            plan[1 + 2 * e] = r;
-            plan[2 + 2 * e] = src_of(r);
+            const int e = np + scan[i] - 1;
+            plan[1 + 2 * e] = k1 + int64_t(piv[i]);
+            plan[2 + 2 * e] = src_of(piv[i]);
         }
     if (tid == 0) plan[0] = count;
 }
@@ -663,6 +676,7 @@ void laswp_plan(int64_t k1, int64_t k2, int64_t nrows, const int64_t* ipiv, int6
                 hipStream_t s)
 {
     SA_REQUIRE(k2 - k1 <= PLAN_MAX, -2, "laswp: too many pivots in one call");
+    SA_REQUIRE(nrows - k1 < (int64_t(1) << 31), -2, "laswp: rows beyond 32-bit plan range");
     if (k2 <= k1) return;
     hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(1024), 0, s, k1, k2, nrows, ipiv, plan);
     SA_CHECK_LAUNCH();

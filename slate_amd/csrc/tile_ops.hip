@@ -144,12 +144,25 @@ template <> struct Vec16<float> { typedef float __attribute__((ext_vector_type(4
 template <> struct Vec16<double> { typedef double __attribute__((ext_vector_type(2))) type; static constexpr int n = 2; };
 
 template <class T>
+struct TrsmProb {
+    const T* L; int64_t ldl; T* B; int64_t ldb; int64_t N; int m; int ldx; int vec_ok;
+};
+
+// blockIdx.y selects one of two independent problems (the two diagonal
+// blocks of a blocked inverse run in one launch).
+template <class T>
 __global__ __launch_bounds__(256)
-void trsm_lunit_kernel(const T* __restrict__ L, int64_t ldl, int m, T* __restrict__ B, int64_t ldb,
-                       int64_t N, int ldx, int vec_ok)
+void trsm_lunit_kernel(TrsmProb<T> q0, TrsmProb<T> q1)
 {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::n;
+    const TrsmProb<T> P = blockIdx.y ? q1 : q0;
+    const int64_t j0 = int64_t(blockIdx.x) * 8;
+    if (j0 >= P.N) return;                              // uniform per workgroup, before any barrier
+    const T* __restrict__ L = P.L;
+    T* __restrict__ B = P.B;
+    const int64_t ldl = P.ldl, ldb = P.ldb, N = P.N;
+    const int m = P.m, ldx = P.ldx;
     extern __shared__ __align__(16) unsigned char trsm_lds[];
     T* Xs = reinterpret_cast<T*>(trsm_lds);            // column c at Xs + c*ldx
     T* Ls = Xs + 8 * ldx;                               // 32 x 33 strictly lower diagonal block
@@ -157,7 +170,6 @@ void trsm_lunit_kernel(const T* __restrict__ L, int64_t ldl, int m, T* __restric
     const int r = lane & 31;
     const bool upper = lane >= 32;
     const int cl = 2 * (tid >> 6) + (upper ? 1 : 0);
-    const int64_t j0 = int64_t(blockIdx.x) * 8;
     {
         const int c = tid & 7;
         const bool cv = j0 + c < N;
@@ -178,7 +190,7 @@ void trsm_lunit_kernel(const T* __restrict__ L, int64_t ldl, int m, T* __restric
             T acc[VN];
             #pragma unroll
             for (int e = 0; e < VN; ++e) acc[e] = T(0);
-            if (vec_ok) {
+            if (P.vec_ok) {
                 #pragma unroll 4
                 for (int k = 0; k < p0; k += VN) {
                     const V l = *reinterpret_cast<const V*>(Li + k);
@@ -216,22 +228,48 @@ void trsm_lunit_kernel(const T* __restrict__ L, int64_t ldl, int m, T* __restric
 int64_t trsm_fused_max(int elem_bytes) { return elem_bytes == 8 ? 768 : 1024; }
 
 template <class T>
-void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s)
+static TrsmProb<T> trsm_prob(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, size_t& lds)
 {
-    if (m <= 0 || N <= 0) return;
     SA_REQUIRE(m <= trsm_fused_max(int(sizeof(T))), -2, "trsm_lunit_fused: triangle too large");
     SA_REQUIRE(ceildiv(N, 8) < (int64_t(1) << 31), -2, "trsm_lunit_fused: too many columns");
     constexpr int VN = Vec16<T>::n;
-    const int ldx = int(ceildiv(m, 32) * 32) + VN;
-    const size_t lds = sizeof(T) * (size_t(8) * size_t(ldx) + 32 * 33);
-    const int vec_ok = (reinterpret_cast<uintptr_t>(L) % 16 == 0 && ldl % VN == 0) ? 1 : 0;
-    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(N, 8))), dim3(256), lds, s,
-                       L, ldl, int(m), B, ldb, N, ldx, vec_ok);
+    TrsmProb<T> q;
+    q.L = L; q.ldl = ldl; q.B = B; q.ldb = ldb; q.N = N; q.m = int(m);
+    q.ldx = int(ceildiv(m, 32) * 32) + VN;
+    q.vec_ok = (reinterpret_cast<uintptr_t>(L) % 16 == 0 && ldl % VN == 0) ? 1 : 0;
+    lds = std::max(lds, sizeof(T) * (size_t(8) * size_t(q.ldx) + 32 * 33));
+    return q;
+}
+
+template <class T>
+void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s)
+{
+    if (m <= 0 || N <= 0) return;
+    size_t lds = 0;
+    const TrsmProb<T> q = trsm_prob(L, ldl, m, B, ldb, N, lds);
+    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(N, 8)), 1), dim3(256), lds, s, q, q);
+    SA_CHECK_LAUNCH();
+}
+
+template <class T>
+void trsm_lunit_fused2(const T* L0, int64_t ldl0, int64_t m0, T* B0, int64_t ldb0, int64_t N0,
+                       const T* L1, int64_t ldl1, int64_t m1, T* B1, int64_t ldb1, int64_t N1, hipStream_t s)
+{
+    SA_REQUIRE(m0 > 0 && m1 > 0 && N0 > 0 && N1 > 0, -2, "trsm_lunit_fused2: empty problem");
+    size_t lds = 0;
+    const TrsmProb<T> q0 = trsm_prob(L0, ldl0, m0, B0, ldb0, N0, lds);
+    const TrsmProb<T> q1 = trsm_prob(L1, ldl1, m1, B1, ldb1, N1, lds);
+    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(std::max(N0, N1), 8)), 2), dim3(256), lds, s,
+                       q0, q1);
     SA_CHECK_LAUNCH();
 }
 
 template void trsm_lunit_fused<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
 template void trsm_lunit_fused<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);
+template void trsm_lunit_fused2<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t,
+                                        const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
+template void trsm_lunit_fused2<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t,
+                                       const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);
 
 //------------------------------------------------------------------------------
 // Right-looking upper Cholesky (A = U^T U) of one nb x nb block in LDS.
