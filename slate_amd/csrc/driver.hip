@@ -53,7 +53,6 @@ struct Context {
     Buf tvw;                     // trsv look-back exchange words (+ progress word)
     Buf swaptmp;                 // two-pass row interchange workspace (trailing-update stream)
     Buf swaptmp1;                // the same for the lookahead block (tall blocks only)
-    Buf itmp;                    // blocked L11^{-1}: the L21 X11 product
     uint32_t tv_tag = 0;         // next free tag of tvw
     bool calu = false;           // leaves use tournament pivoting (MethodLU::CALU)
     // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
@@ -276,12 +275,13 @@ static void apply_inv(const T* inv, int64_t nb, T* B, int64_t ldb, int64_t N, T*
 }
 
 // inv := L^{-1} for the unit-lower nb x nb diagonal block of a row-major matrix.
-// From nb = 256 on, by 2 x 2 blocks: both diagonal blocks of the inverse in
-// one launch (each sweep half as long), then X21 = -X22 (L21 X11) by two GEMMs
-// through c.itmp.  The inverse of a unit-lower matrix is exactly zero above
-// the diagonal, so the GEMMs can take the full blocks.
+// From nb = 256 on, in two half-height sweeps instead of one full one: X11 =
+// L11^{-1} (top-left block), then the bottom rows [X21 X22] solve
+// L22 [X21 X22] = [-L21 X11  I] with the same substitution kernel.  Every
+// block comes from substitution, as in the recursive solve (multiplying by
+// an explicit X22 instead loses accuracy when L22 is ill-conditioned).
 template <class T>
-static void unit_lower_inverse(Context& c, const T* L, int64_t ld, int64_t nb, T* inv, hipStream_t s)
+static void unit_lower_inverse(const T* L, int64_t ld, int64_t nb, T* inv, hipStream_t s)
 {
     set_identity<T>(inv, nb, nb, s);
     const int64_t h = ceildiv(nb / 2, 32) * 32, h2 = nb - h;
@@ -289,18 +289,13 @@ static void unit_lower_inverse(Context& c, const T* L, int64_t ld, int64_t nb, T
         trsm_rec<T>(L, ld, 0, 1, 1, nb, inv, nb, nb, s);
         return;
     }
-    trsm_lunit_fused2<T>(L, ld, h, inv, nb, h, L + h * ld + h, ld, h2, inv + h * nb + h, nb, h2, s);
-    c.itmp.ensure(sizeof(T) * size_t(h2) * size_t(h));
-    T* W = c.itmp.as<T>();
+    trsm_lunit_fused<T>(L, ld, h, inv, nb, h, s);                       // X11
     GemmArgs g{};
     g.M = h2; g.N = h; g.K = h;
-    g.A = L + h * ld; g.lda = ld; g.B = inv; g.ldb = nb; g.C = W; g.ldc = h;
-    g.alpha = 1.0; g.beta = 0.0; g.uplo = 0;
-    gemm<T>(g, A_KC, B_NC, s);                                     // W = L21 X11
-    g.M = h2; g.N = h; g.K = h2;
-    g.A = inv + h * nb + h; g.lda = nb; g.B = W; g.ldb = h; g.C = inv + h * nb; g.ldc = nb;
-    g.alpha = -1.0; g.beta = 0.0;
-    gemm<T>(g, A_KC, B_NC, s);                                     // X21 = -X22 W
+    g.A = L + h * ld; g.lda = ld; g.B = inv; g.ldb = nb; g.C = inv + h * nb; g.ldc = nb;
+    g.alpha = -1.0; g.beta = 0.0; g.uplo = 0;
+    gemm<T>(g, A_KC, B_NC, s);                                          // -L21 X11
+    trsm_lunit_fused<T>(L + h * ld + h, ld, h2, inv + h * nb, nb, nb, s);  // [X21 X22]
 }
 
 //------------------------------------------------------------------------------
@@ -535,7 +530,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     };
     // panel 0
     factor_panel(0, 0, std::min(nb, n));
-    unit_lower_inverse<T>(c, A, ld, std::min(nb, n), step_inv(0), s1);
+    unit_lower_inverse<T>(A, ld, std::min(nb, n), step_inv(0), s1);
     split_panel(0, 0, std::min(nb, n), s1);
     SA_HIP(hipEventRecord(ev_panel(0), s1));
     for (int64_t k = 0; k < nsteps; ++k) {
@@ -565,7 +560,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             apply_inv<T>(step_inv(k), kb, A + k0 * ld + c1, ld, nb1, tmp1, s1);
             update(k, k0, kb, c1, nb1, true, s1, 0);
             factor_panel(k + 1, c1, nb1);
-            unit_lower_inverse<T>(c, A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
+            unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             split_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
             // s0: step k on the columns beyond k+1
@@ -585,7 +580,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
                 SA_HIP(hipStreamWaitEvent(s1, ev_trail(k + 1), 0));
             }
             factor_panel(k + 1, c1, nb1);
-            unit_lower_inverse<T>(c, A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
+            unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             split_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
         }
@@ -1789,7 +1784,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
-                       &c.tvw, &c.swaptmp, &c.swaptmp1, &c.itmp})
+                       &c.tvw, &c.swaptmp, &c.swaptmp1})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);

@@ -133,10 +133,6 @@ void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
 int64_t trsm_fused_max(int elem_bytes);
 template <class T>
 void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s);
-// Two independent solves of that kind in one launch.
-template <class T>
-void trsm_lunit_fused2(const T* L0, int64_t ldl0, int64_t m0, T* B0, int64_t ldb0, int64_t N0,
-                       const T* L1, int64_t ldl1, int64_t m1, T* B1, int64_t ldb1, int64_t N1, hipStream_t s);
 
 // One-launch triangular solve of tri * x = b (one right-hand side, x overwrites
 // b, contiguous) with a look-back chain between blocks of trsv_block() rows.

@@ -148,15 +148,12 @@ struct TrsmProb {
     const T* L; int64_t ldl; T* B; int64_t ldb; int64_t N; int m; int ldx; int vec_ok;
 };
 
-// blockIdx.y selects one of two independent problems (the two diagonal
-// blocks of a blocked inverse run in one launch).
 template <class T>
 __global__ __launch_bounds__(256)
-void trsm_lunit_kernel(TrsmProb<T> q0, TrsmProb<T> q1)
+void trsm_lunit_kernel(TrsmProb<T> P)
 {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::n;
-    const TrsmProb<T> P = blockIdx.y ? q1 : q0;
     const int64_t j0 = int64_t(blockIdx.x) * 8;
     if (j0 >= P.N) return;                              // uniform per workgroup, before any barrier
     const T* __restrict__ L = P.L;
@@ -247,30 +244,12 @@ void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int
     if (m <= 0 || N <= 0) return;
     size_t lds = 0;
     const TrsmProb<T> q = trsm_prob(L, ldl, m, B, ldb, N, lds);
-    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(N, 8)), 1), dim3(256), lds, s, q, q);
-    SA_CHECK_LAUNCH();
-}
-
-template <class T>
-void trsm_lunit_fused2(const T* L0, int64_t ldl0, int64_t m0, T* B0, int64_t ldb0, int64_t N0,
-                       const T* L1, int64_t ldl1, int64_t m1, T* B1, int64_t ldb1, int64_t N1, hipStream_t s)
-{
-    SA_REQUIRE(m0 > 0 && m1 > 0 && N0 > 0 && N1 > 0, -2, "trsm_lunit_fused2: empty problem");
-    size_t lds = 0;
-    const TrsmProb<T> q0 = trsm_prob(L0, ldl0, m0, B0, ldb0, N0, lds);
-    const TrsmProb<T> q1 = trsm_prob(L1, ldl1, m1, B1, ldb1, N1, lds);
-    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(std::max(N0, N1), 8)), 2), dim3(256), lds, s,
-                       q0, q1);
+    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(N, 8))), dim3(256), lds, s, q);
     SA_CHECK_LAUNCH();
 }
 
 template void trsm_lunit_fused<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
 template void trsm_lunit_fused<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);
-template void trsm_lunit_fused2<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t,
-                                        const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
-template void trsm_lunit_fused2<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t,
-                                       const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);
-
 //------------------------------------------------------------------------------
 // Right-looking upper Cholesky (A = U^T U) of one nb x nb block in LDS.
 // Row-major upper == column-major lower, so this is LAPACK's lower dpotrf on
