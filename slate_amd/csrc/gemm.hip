@@ -360,6 +360,137 @@ void gemm_f64_kernel(GemmArgs p)
             }
 }
 
+//------------------------------------------------------------------------------
+// FP32 product with A K-contiguous and B N-contiguous (the float LU's
+// L11^{-1} applications, the panel's node updates, the solves' off-diagonal
+// blocks, and the trailing update of the FP32-MFMA factor): the FP64 kernel
+// above with float operands and v_mfma_f32_16x16x4f32.  Slices are 32 deep
+// (the same 128-byte rows), lane l's eight k-steps use k = 8 (l/16) + s, so
+// its A and B fragments for a slice are 8 consecutive floats (two
+// ds_read_b128 each); LDS rows are 36 floats (144 bytes) as in the FP64
+// kernel.  The accumulator layout differs from FP64's: register r of lane l
+// holds D[4 (l/16) + r][l%16].
+template <int ROLE>
+__global__ __launch_bounds__(256, 2)
+void gemm_f32_kernel(GemmArgs p)
+{
+    constexpr int BM = 128, BN = 128, BK = 32, LDP = 36;
+    __shared__ float As[2][BM * LDP];
+    __shared__ float Bt[2][BN * LDP];
+
+    const int64_t M = p.M, N = p.N, K = p.K;
+    const int64_t tiles_m = ceildiv(M, BM), tiles_n = ceildiv(N, BN);
+    const int64_t ntiles = tiles_m * tiles_n;
+    int64_t t = xcd_remap(blockIdx.x, ntiles);
+    constexpr int64_t GM = 8;
+    int64_t group = t / (GM * tiles_n);
+    int64_t first_m = group * GM;
+    int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
+    int64_t within = t - group * GM * tiles_n;
+    const int64_t m0 = (first_m + within % gm) * BM;
+    const int64_t n0 = (within / gm) * BN;
+    if (p.uplo == 1 && n0 + BN <= m0) return;
+
+    const float* __restrict__ A = static_cast<const float*>(p.A);
+    const float* __restrict__ B = static_cast<const float*>(p.B);
+    float* __restrict__ C = static_cast<float*>(p.C);
+    const int64_t lda = p.lda, ldb = p.ldb, ldc = p.ldc;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int fr = lane & 15, fk = lane >> 4;
+    // staging: A row ar, k ak .. ak+15; B k row bk, columns bc .. bc+15
+    const int ar = tid >> 1, ak = (tid & 1) * 16;
+    const int bk = tid & 31, bc = (tid >> 5) * 16;
+    const bool full_mn = m0 + BM <= M && n0 + BN <= N;
+    const bool vec_ok = (lda % 4 == 0) && (ldb % 4 == 0) && (reinterpret_cast<uintptr_t>(A) % 16 == 0)
+                        && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+    v4f ra[4], rb[4];
+    auto load = [&](int64_t k0) {
+        if (full_mn && vec_ok && k0 + BK <= K) {
+            const float* ap = A + (m0 + ar) * lda + k0 + ak;
+            const float* bp = B + (k0 + bk) * ldb + n0 + bc;
+            #pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                ra[q] = *reinterpret_cast<const v4f*>(ap + 4 * q);
+                rb[q] = *reinterpret_cast<const v4f*>(bp + 4 * q);
+            }
+        }
+        else {
+            const int64_t gmr = m0 + ar, gkb = k0 + bk;
+            const int64_t cm = gmr < M ? gmr : M - 1, ckb = gkb < K ? gkb : K - 1;
+            #pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t gk = k0 + ak + e, gn = n0 + bc + e;
+                const float x = A[cm * lda + (gk < K ? gk : K - 1)];
+                const float y = B[ckb * ldb + (gn < N ? gn : N - 1)];
+                ra[e >> 2][e & 3] = (gmr < M && gk < K) ? x : 0.0f;
+                rb[e >> 2][e & 3] = (gkb < K && gn < N) ? y : 0.0f;
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) *reinterpret_cast<v4f*>(&As[buf][ar * LDP + ak + 4 * q]) = ra[q];
+        #pragma unroll
+        for (int e = 0; e < 16; ++e) Bt[buf][(bc + e) * LDP + bk] = rb[e >> 2][e & 3];
+    };
+
+    v4f acc[4][4];
+    #pragma unroll
+    for (int i = 0; i < 4; ++i)
+        #pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t ktiles = ceildiv(K, BK);
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int64_t kt = 0; kt < ktiles; ++kt) {
+        const int cur = int(kt & 1);
+        const bool more = kt + 1 < ktiles;
+        if (more) load((kt + 1) * BK);
+        #pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            v4f af[4], bf[4];
+            #pragma unroll
+            for (int i = 0; i < 4; ++i)
+                af[i] = *reinterpret_cast<const v4f*>(&As[cur][(wm * 64 + i * 16 + fr) * LDP + 8 * fk + 4 * h]);
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                bf[j] = *reinterpret_cast<const v4f*>(&Bt[cur][(wn * 64 + j * 16 + fr) * LDP + 8 * fk + 4 * h]);
+            #pragma unroll
+            for (int s_ = 0; s_ < 4; ++s_)
+                #pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    #pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][s_], bf[j][s_], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            store(cur ^ 1);
+            __syncthreads();
+        }
+    }
+
+    const float alpha = float(p.alpha), beta = float(p.beta);
+    #pragma unroll
+    for (int i = 0; i < 4; ++i)
+        #pragma unroll
+        for (int j = 0; j < 4; ++j)
+            #pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gm_ = m0 + wm * 64 + i * 16 + 4 * fk + r;
+                const int64_t gn_ = n0 + wn * 64 + j * 16 + fr;
+                if (gm_ < M && gn_ < N && (p.uplo == 0 || gn_ >= gm_)) {
+                    float* c = C + gm_ * ldc + gn_;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * *c;
+                    *c = v;
+                }
+            }
+}
+
 template <class T, int ALAY, int BLAY, int ROLE = 0>
 static void launch(const GemmArgs& a, hipStream_t s)
 {
@@ -371,6 +502,9 @@ static void launch(const GemmArgs& a, hipStream_t s)
     SA_REQUIRE(nt < (int64_t(1) << 31), -2, "gemm: grid too large");
     if constexpr (std::is_same<T, double>::value && ALAY == A_KC && BLAY == B_NC) {
         hipLaunchKernelGGL((gemm_f64_kernel<ROLE>), dim3(unsigned(nt)), dim3(256), 0, s, a);
+    }
+    else if constexpr (std::is_same<T, float>::value && ALAY == A_KC && BLAY == B_NC) {
+        hipLaunchKernelGGL((gemm_f32_kernel<ROLE>), dim3(unsigned(nt)), dim3(256), 0, s, a);
     }
     else {
         hipLaunchKernelGGL((gemm_kernel<T, ALAY, BLAY, BM, BN, BK, 2, 2, 2, ROLE>), dim3(unsigned(nt)), dim3(256),
