@@ -500,9 +500,9 @@ template void panel_leaf<float>(float*, int64_t, int64_t, int64_t, int64_t, int,
 static constexpr int PLAN_MAX = 2048;
 
 // Rows are kept relative to k1 as 32-bit ints in LDS, padded to a multiple
-// of 4 with identity transpositions, so both per-thread sweeps (the duplicate
-// test and the backward walk of src_of) read four entries per broadcast LDS
-// load and do only register compares in between.
+// of 16 with identity transpositions, so both per-thread sweeps (the
+// duplicate test and the backward walk of src_of) read sixteen entries per
+// group of four broadcast LDS loads and do only register compares in between.
 __global__ __launch_bounds__(1024)
 void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __restrict__ ipiv,
                        int64_t* __restrict__ plan)
@@ -513,7 +513,7 @@ void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __r
     typedef int int4_t __attribute__((ext_vector_type(4)));
     const int tid = threadIdx.x, nt = blockDim.x;
     const int np = int(k2 - k1);
-    const int np4 = (np + 3) & ~3;
+    const int np4 = (np + 15) & ~15;                     // padded to whole 16-entry groups
     for (int i = tid; i < np4; i += nt) {
         int v = i;                                          // padding / out-of-range: no swap
         if (i < np) {
@@ -526,11 +526,14 @@ void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __r
     for (int i = tid; i < np; i += nt) {
         const int p = piv[i];
         int k = p >= np;                                    // a row below the swapped block
-        for (int i2 = 0; i2 < (i & ~3); i2 += 4) {
-            const int4_t q = *reinterpret_cast<const int4_t*>(&piv[i2]);
-            k &= (q[0] != p) & (q[1] != p) & (q[2] != p) & (q[3] != p);
+        for (int i2 = 0; i2 < (i & ~15); i2 += 16) {
+            int4_t q[4];
+            #pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = *reinterpret_cast<const int4_t*>(&piv[i2 + 4 * u]);
+            #pragma unroll
+            for (int u = 0; u < 4; ++u) k &= (q[u][0] != p) & (q[u][1] != p) & (q[u][2] != p) & (q[u][3] != p);
         }
-        for (int i2 = i & ~3; i2 < i; ++i2) k &= (piv[i2] != p);
+        for (int i2 = i & ~15; i2 < i; ++i2) k &= (piv[i2] != p);
         keep[i] = k;
         scan[i] = k;
     }
@@ -549,11 +552,13 @@ void laswp_plan_kernel(int64_t k1, int64_t k2, int64_t nrows, const int64_t* __r
     const int count = np + nkeep;
     // source of relative row x: t_0(t_1(...t_{np-1}(x))), t_i = (i piv[i])
     auto src_of = [&](int x) {
-        for (int i = np4 - 4; i >= 0; i -= 4) {
-            const int4_t q = *reinterpret_cast<const int4_t*>(&piv[i]);
+        for (int i = np4 - 16; i >= 0; i -= 16) {
+            int4_t q[4];
             #pragma unroll
-            for (int u = 3; u >= 0; --u) {
-                const int a = i + u, b = q[u];
+            for (int u = 0; u < 4; ++u) q[u] = *reinterpret_cast<const int4_t*>(&piv[i + 4 * u]);
+            #pragma unroll
+            for (int u = 15; u >= 0; --u) {
+                const int a = i + u, b = q[u >> 2][u & 3];
                 x = (x == a) ? b : ((x == b) ? a : x);
             }
         }
