@@ -150,3 +150,24 @@ def test_sgetrf_pivots_equal_oracle(gpu):
     np.testing.assert_array_equal(ipiv, ip_o)
     eps32 = float(np.finfo(np.float32).eps)
     assert np.abs(to_host(A) - LU_o).max() <= 4 * n * eps32 * np.abs(LU_o).max()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_getrf_odd_leading_dimension(gpu, dtype):
+    # lda = n + 1: every kernel on the path (panel leaf, its interchanges of the
+    # other panel columns, the one-launch triangular solves, the GEMMs) takes
+    # its unaligned, scalar-access branch
+    # float at n = 400 (as the aligned float test): larger float cases can
+    # meet near-ties whose pivot depends on the summation order
+    n, nb = (700, 128) if dtype == torch.float64 else (400, 64)
+    say("getrf n=%d lda=%d nb=%d %s" % (n, n + 1, nb, dtype))
+    H = oracle.rands(41, n, n)
+    A = slate_amd.empty_colmajor(n, n, dtype, ld=n + 1)
+    A.copy_(to_dev(H, dtype))
+    info, ipiv = slate_amd.getrf(A, slate_amd.Options(block_size=nb))
+    np_dtype = np.float64 if dtype == torch.float64 else np.float32
+    LU_o, ip_o, info_o = oracle.getrf(H.astype(np_dtype), nb=nb, ib=16, dtype=np_dtype)
+    assert info == info_o == 0
+    np.testing.assert_array_equal(ipiv, ip_o)
+    eps = float(np.finfo(np_dtype).eps)
+    assert np.abs(to_host(A) - LU_o).max() <= 4 * n * eps * np.abs(LU_o).max()
