@@ -53,6 +53,7 @@ struct Context {
     Buf tvw;                     // trsv look-back exchange words (+ progress word)
     Buf swaptmp;                 // two-pass row interchange workspace (trailing-update stream)
     Buf swaptmp1;                // the same for the lookahead block (tall blocks only)
+    Buf ttri;                    // row-major copy of a column-major-view triangle (Cholesky solves)
     uint32_t tv_tag = 0;         // next free tag of tvw
     bool calu = false;           // leaves use tournament pivoting (MethodLU::CALU)
     // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
@@ -197,10 +198,11 @@ static void trsm_rec(const T* M, int64_t ld, int cm, int lower, int unit, int64_
     if (nb <= 0 || N <= 0) return;
     constexpr int64_t L = TRSM_LEAF;
     if (nb <= L) { trsm_leaf<T>(M, ld, cm, lower, unit, nb, B, ldb, N, s); return; }
-    // the unit-lower row-major solve of the panel, of L11^{-1} and of getrs
-    // runs as one launch up to trsm_fused_max rows
-    if (cm == 0 && lower && unit && nb <= trsm_fused_max(int(sizeof(T)))) {
-        trsm_lunit_fused<T>(M, ld, nb, B, ldb, N, s);
+    // the lower row-major solves (the panel's, L11^{-1}, getrs' L sweep, the
+    // Cholesky blocks after their transposed copy) run as one launch up to
+    // trsm_fused_max rows
+    if (cm == 0 && lower && nb <= trsm_fused_max(int(sizeof(T)))) {
+        trsm_lower_fused<T>(M, ld, nb, B, ldb, N, unit, s);
         return;
     }
     int64_t n1 = ((nb / 2 + L - 1) / L) * L;
@@ -232,6 +234,22 @@ static void trsm_rec(const T* M, int64_t ld, int cm, int lower, int unit, int64_
     }
 }
 
+// Lower triangle of a column-major view (cm = 1: tri(i, k) = M[i + k*ld], the
+// transposed row-major upper factor of the Cholesky): copied row-major into
+// c.ttri (nb^2 elements) so that the one-launch lower solve applies; false
+// when the triangle is outside that solve's range.
+template <class T>
+static bool trsm_lower_cm_fused(Context& c, const T* M, int64_t ld, int unit, int64_t nb, T* B, int64_t ldb,
+                                int64_t N, hipStream_t s)
+{
+    if (nb <= TRSM_LEAF || nb > trsm_fused_max(int(sizeof(T))) || N <= 0) return false;
+    c.ttri.ensure(sizeof(T) * size_t(nb) * size_t(nb));
+    T* Lr = c.ttri.as<T>();
+    transpose_convert<T, T>(M, ld, Lr, nb, nb, nb, s);               // Lr[i*nb + k] = M[i + k*ld]
+    trsm_lower_fused<T>(Lr, nb, nb, B, ldb, N, unit, s);
+    return true;
+}
+
 // Wide triangular solve B := tri^{-1} B for an nb x nb triangle and an
 // nb x N row-major B with N >> nb: the triangle is inverted once (the same
 // recursive solve applied to an identity, nb x nb) and the product with B is
@@ -243,13 +261,18 @@ template <class T>
 static void trsm_wide(Context& c, const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
                       T* B, int64_t ldb, int64_t N, hipStream_t s)
 {
-    if (N < 4 * nb || nb <= TRSM_LEAF) { trsm_rec<T>(M, ld, cm, lower, unit, nb, B, ldb, N, s); return; }
+    if (N < 4 * nb || nb <= TRSM_LEAF) {
+        if (!(cm == 1 && lower && trsm_lower_cm_fused<T>(c, M, ld, unit, nb, B, ldb, N, s)))
+            trsm_rec<T>(M, ld, cm, lower, unit, nb, B, ldb, N, s);
+        return;
+    }
     c.tinv.ensure(sizeof(T) * size_t(nb) * size_t(nb));
     c.ttmp.ensure(sizeof(T) * size_t(nb) * size_t(N));
     T* inv = c.tinv.as<T>();
     T* tmp = c.ttmp.as<T>();
     set_identity<T>(inv, nb, nb, s);
-    trsm_rec<T>(M, ld, cm, lower, unit, nb, inv, nb, nb, s);
+    if (!(cm == 1 && lower && trsm_lower_cm_fused<T>(c, M, ld, unit, nb, inv, nb, nb, s)))
+        trsm_rec<T>(M, ld, cm, lower, unit, nb, inv, nb, nb, s);
     GemmArgs g{};
     g.M = nb; g.N = N; g.K = nb;
     g.A = inv; g.lda = nb; g.B = B; g.ldb = ldb; g.C = tmp; g.ldc = N;
@@ -609,7 +632,8 @@ static void potrf_diag_rec(Context& c, T* M, int64_t ld, int64_t nb, int64_t col
     if (n1 >= nb) n1 = nb - 64;
     int64_t n2 = nb - n1;
     potrf_diag_rec<T>(c, M, ld, n1, col0, info, s);
-    trsm_rec<T>(M, ld, 1, 1, 0, n1, M + n1, ld, n2, s);
+    if (!trsm_lower_cm_fused<T>(c, M, ld, 0, n1, M + n1, ld, n2, s))
+        trsm_rec<T>(M, ld, 1, 1, 0, n1, M + n1, ld, n2, s);
     GemmArgs g{};
     g.M = n2; g.N = n2; g.K = n1;
     g.A = M + n1; g.lda = ld; g.B = M + n1; g.ldb = ld;
@@ -649,12 +673,18 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
                 continue;
             }
         }
+        // C -= U12^T U12 (upper tiles only): U12^T is copied K-contiguous
+        // (r x kb, c.ttmp is free again after trsm_wide) so the update runs on
+        // the 16-byte-staged MFMA kernel instead of the generic one
+        c.ttmp.ensure(sizeof(T) * size_t(r) * size_t(kb));
+        T* ut = c.ttmp.as<T>();
+        transpose_convert<T, T>(Akk + kb, ld, ut, kb, r, kb, s);        // ut[i*kb + k] = U12[k][i]
         GemmArgs g{};
         g.M = r; g.N = r; g.K = kb;
-        g.A = Akk + kb; g.lda = ld; g.B = Akk + kb; g.ldb = ld;
+        g.A = ut; g.lda = kb; g.B = Akk + kb; g.ldb = ld;
         g.C = A + (k0 + kb) * ld + k0 + kb; g.ldc = ld;
         g.alpha = -1.0; g.beta = 1.0; g.uplo = 1;
-        gemm<T>(g, A_MC, B_NC, s);
+        gemm<T>(g, A_KC, B_NC, s);
     }
 }
 
@@ -1784,7 +1814,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
-                       &c.tvw, &c.swaptmp, &c.swaptmp1})
+                       &c.tvw, &c.swaptmp, &c.swaptmp1, &c.ttri})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);

@@ -133,6 +133,9 @@ void trsm_leaf(const T* M, int64_t ld, int cm, int lower, int unit, int64_t nb,
 int64_t trsm_fused_max(int elem_bytes);
 template <class T>
 void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s);
+// The same for a lower triangle with its own diagonal when unit == 0.
+template <class T>
+void trsm_lower_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, int unit, hipStream_t s);
 
 // One-launch triangular solve of tri * x = b (one right-hand side, x overwrites
 // b, contiguous) with a look-back chain between blocks of trsv_block() rows.

@@ -109,8 +109,9 @@ template void trsm_leaf<float>(const float*, int64_t, int, int, int, int64_t, fl
                                int64_t, hipStream_t);
 
 //------------------------------------------------------------------------------
-// L X = B in place for a unit lower triangle (tri(i, k) = L[i*ldl + k],
-// m x m) and a row-major B (m x N, ldb), in one launch, m <= trsm_fused_max.
+// L X = B in place for a lower triangle (tri(i, k) = L[i*ldl + k], m x m;
+// unit diagonal or not) and a row-major B (m x N, ldb), in one launch,
+// m <= trsm_fused_max.
 // Workgroup x owns the columns [8x, 8x + 8), kept in LDS k-contiguous.  The
 // sweep is left-looking over 32-row blocks: every thread owns one (row,
 // column) of the block and forms y = b - L(i, 0:p0) X(0:p0, c) with 16-byte
@@ -148,7 +149,7 @@ struct TrsmProb {
     const T* L; int64_t ldl; T* B; int64_t ldb; int64_t N; int m; int ldx; int vec_ok;
 };
 
-template <class T>
+template <class T, int UNIT>
 __global__ __launch_bounds__(256)
 void trsm_lunit_kernel(TrsmProb<T> P)
 {
@@ -207,11 +208,14 @@ void trsm_lunit_kernel(TrsmProb<T> P)
         T lr[32];
         #pragma unroll
         for (int k = 0; k < 32; ++k) lr[k] = Ls[r * 33 + k];
+        const T diag = (!UNIT && i < m) ? L[int64_t(i) * ldl + i] : T(1);
         #pragma unroll
         for (int k = 0; k < 31; ++k) {
+            if (!UNIT && r == k) y = y / diag;
             const T xk = bcast_half<T>(y, k, upper);
             if (r > k) y -= lr[k] * xk;
         }
+        if (!UNIT && r == 31) y = y / diag;
         if (i < m) Xc[i] = y;
         __syncthreads();
     }
@@ -239,15 +243,24 @@ static TrsmProb<T> trsm_prob(const T* L, int64_t ldl, int64_t m, T* B, int64_t l
 }
 
 template <class T>
-void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s)
+void trsm_lower_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, int unit, hipStream_t s)
 {
     if (m <= 0 || N <= 0) return;
     size_t lds = 0;
     const TrsmProb<T> q = trsm_prob(L, ldl, m, B, ldb, N, lds);
-    hipLaunchKernelGGL((trsm_lunit_kernel<T>), dim3(unsigned(ceildiv(N, 8))), dim3(256), lds, s, q);
+    if (unit) hipLaunchKernelGGL((trsm_lunit_kernel<T, 1>), dim3(unsigned(ceildiv(N, 8))), dim3(256), lds, s, q);
+    else hipLaunchKernelGGL((trsm_lunit_kernel<T, 0>), dim3(unsigned(ceildiv(N, 8))), dim3(256), lds, s, q);
     SA_CHECK_LAUNCH();
 }
 
+template <class T>
+void trsm_lunit_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int64_t N, hipStream_t s)
+{
+    trsm_lower_fused<T>(L, ldl, m, B, ldb, N, 1, s);
+}
+
+template void trsm_lower_fused<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, int, hipStream_t);
+template void trsm_lower_fused<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t, int, hipStream_t);
 template void trsm_lunit_fused<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
 template void trsm_lunit_fused<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);
 //------------------------------------------------------------------------------
