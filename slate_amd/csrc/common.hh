@@ -162,4 +162,21 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t nblocks)
     return bid;
 }
 
+// Tile t of the upper triangle (tn >= tm) of a T x T tile grid, enumerated
+// row by row: row tm starts at off(tm) = tm (2T - tm + 1) / 2.  Launching only
+// these tiles keeps every workgroup (and every XCD's band) busy in the
+// triangular update, where a full grid would idle below the diagonal.
+__device__ inline void upper_tile(int64_t t, int64_t T, int64_t& tm, int64_t& tn)
+{
+    const double b = double(2 * T + 1);
+    int64_t r = int64_t((b - sqrt(b * b - 8.0 * double(t))) * 0.5);
+    auto off = [T](int64_t x) { return x * (2 * T - x + 1) / 2; };
+    if (r < 0) r = 0;
+    if (r > T - 1) r = T - 1;
+    while (r > 0 && off(r) > t) --r;
+    while (r + 1 < T && off(r + 1) <= t) ++r;
+    tm = r;
+    tn = r + (t - off(r));
+}
+
 } // namespace sa

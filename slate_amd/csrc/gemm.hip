@@ -245,15 +245,26 @@ void gemm_f64_kernel(GemmArgs p)
 
     const int64_t M = p.M, N = p.N, K = p.K;
     const int64_t tiles_m = ceildiv(M, BM), tiles_n = ceildiv(N, BN);
-    const int64_t ntiles = tiles_m * tiles_n;
+    // uplo with a square tile grid: only the upper-triangle tiles are launched
+    const bool tri = p.uplo == 1 && tiles_m == tiles_n;
+    const int64_t ntiles = tri ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
     int64_t t = xcd_remap(blockIdx.x, ntiles);
-    constexpr int64_t GM = 8;
-    int64_t group = t / (GM * tiles_n);
-    int64_t first_m = group * GM;
-    int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
-    int64_t within = t - group * GM * tiles_n;
-    const int64_t m0 = (first_m + within % gm) * BM;
-    const int64_t n0 = (within / gm) * BN;
+    int64_t m0, n0;
+    if (tri) {
+        int64_t tm, tn;
+        upper_tile(t, tiles_m, tm, tn);
+        m0 = tm * BM;
+        n0 = tn * BN;
+    }
+    else {
+        constexpr int64_t GM = 8;
+        int64_t group = t / (GM * tiles_n);
+        int64_t first_m = group * GM;
+        int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
+        int64_t within = t - group * GM * tiles_n;
+        m0 = (first_m + within % gm) * BM;
+        n0 = (within / gm) * BN;
+    }
     if (p.uplo == 1 && n0 + BN <= m0) return;
 
     const double* __restrict__ A = static_cast<const double*>(p.A);
@@ -380,15 +391,26 @@ void gemm_f32_kernel(GemmArgs p)
 
     const int64_t M = p.M, N = p.N, K = p.K;
     const int64_t tiles_m = ceildiv(M, BM), tiles_n = ceildiv(N, BN);
-    const int64_t ntiles = tiles_m * tiles_n;
+    // uplo with a square tile grid: only the upper-triangle tiles are launched
+    const bool tri = p.uplo == 1 && tiles_m == tiles_n;
+    const int64_t ntiles = tri ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
     int64_t t = xcd_remap(blockIdx.x, ntiles);
-    constexpr int64_t GM = 8;
-    int64_t group = t / (GM * tiles_n);
-    int64_t first_m = group * GM;
-    int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
-    int64_t within = t - group * GM * tiles_n;
-    const int64_t m0 = (first_m + within % gm) * BM;
-    const int64_t n0 = (within / gm) * BN;
+    int64_t m0, n0;
+    if (tri) {
+        int64_t tm, tn;
+        upper_tile(t, tiles_m, tm, tn);
+        m0 = tm * BM;
+        n0 = tn * BN;
+    }
+    else {
+        constexpr int64_t GM = 8;
+        int64_t group = t / (GM * tiles_n);
+        int64_t first_m = group * GM;
+        int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
+        int64_t within = t - group * GM * tiles_n;
+        m0 = (first_m + within % gm) * BM;
+        n0 = (within / gm) * BN;
+    }
     if (p.uplo == 1 && n0 + BN <= m0) return;
 
     const float* __restrict__ A = static_cast<const float*>(p.A);
@@ -499,6 +521,11 @@ static void launch(const GemmArgs& a, hipStream_t s)
     // the fastest of the tilings measured for both FP64 and FP32.
     constexpr int BM = 128, BN = 128, BK = (sizeof(T) == 8) ? 16 : 32;
     int64_t nt = ceildiv(a.M, BM) * ceildiv(a.N, BN);
+    // the 16-byte-staged kernels launch only the upper-triangle tiles of a
+    // square triangular update (same tile size in both directions)
+    if constexpr (ALAY == A_KC && BLAY == B_NC)
+        if (a.uplo == 1 && ceildiv(a.M, BM) == ceildiv(a.N, BN))
+            nt = ceildiv(a.M, BM) * (ceildiv(a.M, BM) + 1) / 2;
     SA_REQUIRE(nt < (int64_t(1) << 31), -2, "gemm: grid too large");
     if constexpr (std::is_same<T, double>::value && ALAY == A_KC && BLAY == B_NC) {
         hipLaunchKernelGGL((gemm_f64_kernel<ROLE>), dim3(unsigned(nt)), dim3(256), 0, s, a);
