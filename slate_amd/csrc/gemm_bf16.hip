@@ -277,15 +277,26 @@ void gemm_sp_kernel(int64_t M, int64_t N, int64_t K, const float* __restrict__ L
     __shared__ u32x4 Bs[NS][BN * 4];
 
     const int64_t tiles_m = ceildiv(M, BM), tiles_n = ceildiv(N, BN);
-    const int64_t ntiles = tiles_m * tiles_n;
+    // uplo with a square tile grid: only the upper-triangle tiles are launched
+    const bool tri = uplo && tiles_m == tiles_n;
+    const int64_t ntiles = tri ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
     int64_t t = xcd_remap(blockIdx.x, ntiles);
-    constexpr int64_t GM = 8;
-    int64_t group = t / (GM * tiles_n);
-    int64_t first_m = group * GM;
-    int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
-    int64_t within = t - group * GM * tiles_n;
-    const int64_t m0 = (first_m + within % gm) * BM;
-    const int64_t n0 = (within / gm) * BN;
+    int64_t m0, n0;
+    if (tri) {
+        int64_t tm, tn;
+        upper_tile(t, tiles_m, tm, tn);
+        m0 = tm * BM;
+        n0 = tn * BN;
+    }
+    else {
+        constexpr int64_t GM = 8;
+        int64_t group = t / (GM * tiles_n);
+        int64_t first_m = group * GM;
+        int64_t gm = (tiles_m - first_m < GM) ? (tiles_m - first_m) : GM;
+        int64_t within = t - group * GM * tiles_n;
+        m0 = (first_m + within % gm) * BM;
+        n0 = (within / gm) * BN;
+    }
     if (uplo && n0 + BN <= m0) return;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -436,6 +447,7 @@ void gemm_s32(int ns, int64_t M, int64_t N, int64_t K, const float* L, int64_t l
                && reinterpret_cast<uintptr_t>(R) % 16 == 0, -2, "gemm_s32: operands must be 16-byte aligned");
     SA_REQUIRE(K >= 4, -2, "gemm_s32: K must be at least 4");
     int64_t nt = ceildiv(M, 128) * ceildiv(N, 128);
+    if (uplo && ceildiv(M, 128) == ceildiv(N, 128)) nt = ceildiv(M, 128) * (ceildiv(M, 128) + 1) / 2;
     SA_REQUIRE(nt < (int64_t(1) << 31), -2, "gemm_s32: grid too large");
     // split at staging, three workgroups per CU (48 KB of LDS each at NS = 3)
 #define SA_S32(NSV, RV) hipLaunchKernelGGL((gemm_sp_kernel<NSV, RV, 3>), dim3(unsigned(nt)), dim3(256), 0, s, \
