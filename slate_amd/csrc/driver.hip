@@ -54,6 +54,7 @@ struct Context {
     Buf swaptmp;                 // two-pass row interchange workspace (trailing-update stream)
     Buf swaptmp1;                // the same for the lookahead block (tall blocks only)
     Buf ttri;                    // row-major copy of a column-major-view triangle (Cholesky solves)
+    Buf cut0, cut1;              // Cholesky: K-contiguous U12^T for the trailing (s0) and block-row (s1) updates
     uint32_t tv_tag = 0;         // next free tag of tvw
     bool calu = false;           // leaves use tournament pivoting (MethodLU::CALU)
     // tuned block sizes per kind (slate_amd_tune_set_nb): (largest n, nb), ascending
@@ -650,42 +651,105 @@ template <class T>
 static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int64_t* info)
 {
     const int64_t nb = choose_nb(c, SLATE_AMD_TUNE_CHOL, n, o.nb);
-    hipStream_t s = c.s0;
     // Float factor with split-BF16 updates: U12 (kb x r) is split and
     // transposed once per step into r x Kp planes, which are both operands
     // of the triangular update C -= U12^T U12.
     const int ns = std::is_same<T, float>::value ? split_terms(o.prec) : 0;
     const int64_t Kp = ceildiv(nb, 32) * 32, plane = n * Kp;
     if (ns > 0) c.bfu0.ensure(std::max(sizeof(__bf16) * size_t(ns), sizeof(float)) * size_t(plane));
-    for (int64_t k0 = 0; k0 < n; k0 += nb) {
-        int64_t kb = std::min(nb, n - k0);
+    const int64_t nsteps = ceildiv(n, nb);
+    // Lookahead as in the LU (src/potrf.cc:84-194 with Option::Lookahead): s1
+    // applies step k to the next block row and factors it (diagonal block and
+    // row-panel solve), while s0 applies step k to the rows beyond it.  The
+    // panel's workspaces (tinv, ttmp, ttri) belong to s1; s0 uses cut0.
+    const bool la = o.lookahead > 0 && nsteps > 2;
+    hipStream_t s0 = c.s0, s1 = la ? c.s1 : c.s0;
+    if (int64_t(c.evs.size()) < 2 * nsteps + 4) {
+        for (int64_t i = int64_t(c.evs.size()); i < 2 * nsteps + 4; ++i) {
+            hipEvent_t e;
+            SA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            c.evs.push_back(e);
+        }
+    }
+    auto ev_panel = [&](int64_t k) { return c.evs[2 + 2 * k]; };
+    auto ev_trail = [&](int64_t k) { return c.evs[3 + 2 * k]; };
+    if (la) {
+        SA_HIP(hipEventRecord(c.evs[0], s0));
+        SA_HIP(hipStreamWaitEvent(s1, c.evs[0], 0));
+    }
+    // diagonal block k0 (kb) and its row-panel solve U12 := U11^{-T} A12
+    auto panel = [&](int64_t k0, int64_t kb, hipStream_t st) {
         T* Akk = A + k0 * ld + k0;
-        potrf_diag_rec<T>(c, Akk, ld, kb, k0, info, s);
-        int64_t r = n - k0 - kb;
-        if (r <= 0) break;
-        trsm_wide<T>(c, Akk, ld, 1, 1, 0, kb, Akk + kb, ld, r, s);
+        potrf_diag_rec<T>(c, Akk, ld, kb, k0, info, st);
+        const int64_t r = n - k0 - kb;
+        if (r > 0) trsm_wide<T>(c, Akk, ld, 1, 1, 0, kb, Akk + kb, ld, r, st);
+    };
+    // rows [r0, n) x columns [r0 + ..., n) of the trailing upper triangle
+    // -= U12(:, r0:)^T U12(:, r0:) for the step whose row panel is at k0
+    auto herk = [&](int64_t k0, int64_t kb, int64_t r0, Buf& wk, hipStream_t st) {
+        const int64_t r = n - r0;
+        if (r <= 0) return;
+        const T* U = A + k0 * ld + r0;                 // kb x r, row-major
+        T* C = A + r0 * ld + r0;
         if constexpr (std::is_same<T, float>::value) {
-            if (ns > 0) {
-                // both operands are U12^T (r x kb): one transposed copy
+            if (ns > 0 && &wk == &c.bfu0) {
                 float* ut = c.bfu0.as<float>();
-                transpose_f32(Akk + kb, ld, kb, r, ut, Kp, s);
-                gemm_s32(ns, r, r, kb, ut, Kp, ut, Kp, A + (k0 + kb) * ld + k0 + kb, ld, -1.0f, s, 0, 1);
-                continue;
+                transpose_f32(U, ld, kb, r, ut, Kp, st);
+                gemm_s32(ns, r, r, kb, ut, Kp, ut, Kp, C, ld, -1.0f, st, 0, 1);
+                return;
             }
         }
-        // C -= U12^T U12 (upper tiles only): U12^T is copied K-contiguous
-        // (r x kb, c.ttmp is free again after trsm_wide) so the update runs on
-        // the 16-byte-staged MFMA kernel instead of the generic one
-        c.ttmp.ensure(sizeof(T) * size_t(r) * size_t(kb));
-        T* ut = c.ttmp.as<T>();
-        transpose_convert<T, T>(Akk + kb, ld, ut, kb, r, kb, s);        // ut[i*kb + k] = U12[k][i]
+        // U^T copied K-contiguous so the update runs on the 16-byte-staged kernel
+        wk.ensure(sizeof(T) * size_t(r) * size_t(kb));
+        T* ut = wk.as<T>();
+        transpose_convert<T, T>(U, ld, ut, kb, r, kb, st);               // ut[i*kb + k] = U[k][i]
         GemmArgs g{};
         g.M = r; g.N = r; g.K = kb;
-        g.A = ut; g.lda = kb; g.B = Akk + kb; g.ldb = ld;
-        g.C = A + (k0 + kb) * ld + k0 + kb; g.ldc = ld;
+        g.A = ut; g.lda = kb; g.B = U; g.ldb = ld; g.C = C; g.ldc = ld;
         g.alpha = -1.0; g.beta = 1.0; g.uplo = 1;
-        gemm<T>(g, A_KC, B_NC, s);
+        gemm<T>(g, A_KC, B_NC, st);
+    };
+    // block row [c1, c2) -= U12(:, c1:c2)^T U12(:, c1:n), upper part only
+    auto block_row = [&](int64_t k0, int64_t kb, int64_t c1, int64_t c2, hipStream_t st) {
+        const int64_t w = c2 - c1, r = n - c1;
+        const T* U = A + k0 * ld + c1;
+        c.cut1.ensure(sizeof(T) * size_t(w) * size_t(kb));
+        T* ut = c.cut1.as<T>();
+        transpose_convert<T, T>(U, ld, ut, kb, w, kb, st);
+        GemmArgs g{};
+        g.M = w; g.N = r; g.K = kb;
+        g.A = ut; g.lda = kb; g.B = U; g.ldb = ld; g.C = A + c1 * ld + c1; g.ldc = ld;
+        g.alpha = -1.0; g.beta = 1.0; g.uplo = 1;
+        gemm<T>(g, A_KC, B_NC, st);
+    };
+    Buf& wk0 = ns > 0 ? c.bfu0 : c.cut0;
+    if (!la) {
+        for (int64_t k0 = 0; k0 < n; k0 += nb) {
+            const int64_t kb = std::min(nb, n - k0);
+            panel(k0, kb, s0);
+            herk(k0, kb, k0 + kb, wk0, s0);
+        }
+        return;
     }
+    panel(0, std::min(nb, n), s1);
+    SA_HIP(hipEventRecord(ev_panel(0), s1));
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const int64_t k0 = k * nb, kb = std::min(nb, n - k0), c1 = k0 + kb;
+        const int64_t r = n - c1;
+        if (r <= 0) break;
+        const int64_t nb1 = std::min(nb, r), c2 = c1 + nb1;
+        // s1: step k on block row k+1, then panel k+1
+        SA_HIP(hipStreamWaitEvent(s1, k > 0 ? ev_trail(k) : ev_panel(0), 0));
+        block_row(k0, kb, c1, c2, s1);
+        panel(c1, nb1, s1);
+        SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
+        // s0: step k on the rows beyond block row k+1
+        SA_HIP(hipStreamWaitEvent(s0, ev_panel(k), 0));
+        herk(k0, kb, c2, wk0, s0);
+        SA_HIP(hipEventRecord(ev_trail(k + 1), s0));
+    }
+    SA_HIP(hipEventRecord(c.evs[1], s1));
+    SA_HIP(hipStreamWaitEvent(s0, c.evs[1], 0));
 }
 
 // Solve with LU factors in row-major (cm = 0) or column-major (cm = 1) layout;
@@ -1814,7 +1878,7 @@ int slate_amd_finalize(slate_amd_context ctx)
     (void)hipDeviceSynchronize();
     for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
-                       &c.tvw, &c.swaptmp, &c.swaptmp1, &c.ttri})
+                       &c.tvw, &c.swaptmp, &c.swaptmp1, &c.ttri, &c.cut0, &c.cut1})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);
