@@ -529,7 +529,13 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             panel_nopiv<T>(c, A, ld, n, p0, pw, ipiv, info, leaf_w, plan1, npu, npt, s1);
         }
         else {
-            panel_rec<T>(c, A, ld, n, p0, pw, ipiv, info, o.threshold, leaf_w, plan1, s1, p0, pw);
+            // FP64 panels taller than 32768 rows keep 16-column leaves: the
+            // 32-column register form needs 2 rows per thread there and drops
+            // to one workgroup per CU (measured 3564 -> 3980 ms at n = 65536),
+            // while below it the wider leaves save launches (643 -> 625 ms at
+            // n = 32768)
+            const int64_t lw = (sizeof(T) == 8 && !c.calu && n - p0 > 32768 && leaf_w > 16 && o.ib <= 0) ? 16 : leaf_w;
+            panel_rec<T>(c, A, ld, n, p0, pw, ipiv, info, o.threshold, lw, plan1, s1, p0, pw);
             laswp_plan(p0, p0 + pw, n, ipiv, step_plan(k), s1);
         }
     };

@@ -24,7 +24,7 @@ namespace sa {
 static constexpr int LEAF_GMAX = 128;     // workgroups per leaf launch
 
 template <class T> struct LeafW;
-template <> struct LeafW<double> { static constexpr int W = 16; };
+template <> struct LeafW<double> { static constexpr int W = 32; };
 template <> struct LeafW<float> { static constexpr int W = 32; };
 
 int panel_leaf_width(int elem_bytes) { return elem_bytes == 8 ? LeafW<double>::W : LeafW<float>::W; }
@@ -450,12 +450,32 @@ static void leaf_shape(int64_t m, int& nt, int& rpt)
     else { nt = 512; rpt = 2; }
 }
 
+template <class T, int W>
+static void panel_leaf_w(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
+                         int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
+                         uint32_t tag0, hipStream_t s, int64_t xp0, int64_t xpw);
+
 template <class T>
 void panel_leaf(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
                 int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
                 uint32_t tag0, hipStream_t s, int64_t xp0, int64_t xpw)
 {
-    constexpr int W = LeafW<T>::W;
+    // FP64 leaves of at most 16 columns run the 16-wide register form (its
+    // 2-row shape keeps two workgroups per CU on tall panels)
+    if constexpr (sizeof(T) == 8) {
+        if (w <= 16) {
+            panel_leaf_w<T, 16>(A, ld, r0, m_total, c0, w, ipiv, info, threshold, pw, tag0, s, xp0, xpw);
+            return;
+        }
+    }
+    panel_leaf_w<T, LeafW<T>::W>(A, ld, r0, m_total, c0, w, ipiv, info, threshold, pw, tag0, s, xp0, xpw);
+}
+
+template <class T, int W>
+static void panel_leaf_w(T* A, int64_t ld, int64_t r0, int64_t m_total, int64_t c0, int w,
+                         int64_t* ipiv, int64_t* info, double threshold, const PanelWork& pw,
+                         uint32_t tag0, hipStream_t s, int64_t xp0, int64_t xpw)
+{
     const int64_t m = m_total - r0;
     SA_REQUIRE(w >= 1 && w <= W, -2, "panel_leaf: bad width");
     SA_REQUIRE(m >= w, -2, "panel_leaf: fewer rows than columns");
