@@ -123,7 +123,7 @@ template <class T>
 __device__ inline void tv_put(uint64_t* p, T v, uint32_t tag) { tw_store_val<T>(p, v, tag); }
 
 template <class T, int BS, int LOWER, int CM>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const T* __restrict__ Dinv,
                           T* __restrict__ X, uint64_t* __restrict__ xw, uint32_t tag,
                           uint64_t* __restrict__ progress, int* error)
@@ -138,7 +138,11 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
     const int b = LOWER ? q : nblk - 1 - q;
     const int64_t r0 = int64_t(b) * BS;
     const int rows = int((n - r0 < BS) ? n - r0 : BS);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // Wave-uniform, so a tile's row (row-major) or column (column-major)
+    // addresses are scalar; that keeps the kernel at two waves per SIMD, so
+    // every block of an n = 65536 solve is resident from the start.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     __shared__ uint32_t sxw[2][BS * NWV];       // x words of the dependency being applied
     __shared__ T sr[BS];
@@ -146,7 +150,7 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
     __shared__ T sred[4][RPW][65];
 
     // Prefetch this block's inverse: thread (it, ih) holds inv(it, ih*KP .. ih*KP+KP-1).
-    const int it = tid % BS, ih = tid / BS;
+    const int it = tid % BS, ih = (wave * 64) / BS;   // BS is a multiple of 64
     T dinv[KP];
     {
         const T* dp = Dinv + int64_t(b) * BS * BS;
@@ -193,6 +197,7 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
 
     const uint64_t t_start = wall_clock64();
     bool failed = false;
+    unsigned known = 0;                         // thread 0: chain progress last seen
     if (ndep > 0) load_tile(0);
     #pragma unroll 1
     for (int t = 0; t < ndep; ++t) {
@@ -200,13 +205,15 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
         const int64_t c0 = int64_t(j) * BS;
         const int cols = int((n - c0 < BS) ? n - c0 : BS);
         const int buf = t & 1;
-        // pace: sleep while the chain is more than two blocks away from t
+        // pace: sleep while the chain is more than two blocks away from t;
+        // progress is re-read only while the last value seen is that far
         if (tid == 0 && !failed) {
-            while (true) {
+            while (known + 2 < unsigned(t + 1)) {
                 const uint64_t pw = tw_load(progress);
                 const unsigned done = (uint32_t(pw >> 32) == tag) ? uint32_t(pw) : 0u;
-                if (done + 2 >= unsigned(t + 1)) break;
-                const unsigned dist = unsigned(t + 1) - done;
+                known = done > known ? done : known;
+                if (known + 2 >= unsigned(t + 1)) break;
+                const unsigned dist = unsigned(t + 1) - known;
                 const unsigned naps = dist > 16 ? 8 : dist / 2;
                 for (unsigned z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(32);
                 if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0
