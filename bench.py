@@ -212,8 +212,8 @@ def main():
     flops = 2.0 / 3.0 * n ** 3
     value = flops / (ms_per_step * 1e-3) / 1e12
     res_mixed = scaled_residual_dev(A, X[:, 0], B[:, 0])
-    log("bench: gesv_mixed %.1f ms/step (device %.1f ms), %.2f TFLOP/s, iters %s, scaled residual %.3f"
-        % (ms_per_step, dev_ms / args.steps, value, iters, res_mixed))
+    log("bench: %s %.1f ms/step (device %.1f ms), %.2f TFLOP/s, iters %s, scaled residual %.3f"
+        % ("gesv_mixed_gmres" if gmres else "gesv_mixed", ms_per_step, dev_ms / args.steps, value, iters, res_mixed))
 
     fp64 = None
     if not args.no_fp64:
