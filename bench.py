@@ -37,15 +37,6 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def scaled_residual_dev(A, x, b):
-    """||b - A x||_1 / (n ||A||_1 ||x||_1 eps) on the GPU (checker only)."""
-    import torch
-    n = A.shape[0]
-    r = b - A @ x
-    a1 = A.abs().sum(dim=0).max()
-    return float(r.abs().sum() / (n * a1 * x.abs().sum()) / torch.finfo(torch.float64).eps)
-
-
 def measured_traffic(kernel_key, factor_precision):
     """HBM bytes per launch of the trailing-update kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
@@ -67,24 +58,84 @@ def measured_traffic(kernel_key, factor_precision):
         return None, None
 
 
-def cpu_baseline(n_cpu, gmres):
-    """Oracle (CPU restatement of SLATE's HostTask gesv_mixed_gmres, or
-    gesv_mixed with classic IR) on a bounded sample."""
+def cpu_baseline(n_cpu, gmres, gpu_c1=None):
+    """Oracle (CPU restatement of SLATE's HostTask algorithms, tile BLAS
+    through scipy's bundled OpenBLAS when present) on bounded samples:
+
+    * the headline workload's form (float tile LU + FP64 refinement) at
+      n = n_cpu, timed once: cpu_baseline.value;
+    * the plain FP64 gesv at n = 2000, nb = 256 (the reference's CPU-runnable
+      case), median of 3, with its pivots compared with the GPU's (gpu_c1)."""
+    import numpy as np
     import oracle
     threads = oracle.lib().oracle_num_threads()
+    blas = oracle.openblas_path()
+    oracle.set_blas(blas)
+    nthreads = max(1, threads // 2)                 # panel threads max(cores/2, 1), getrf.cc:41-43
+    # C1: n = 2000, nb = 256, ib = 32 (tester default), FP64 gesv
+    n1 = 2000
+    A1 = oracle.rands(0, n1, n1)
+    b1 = oracle.rands(1, n1, 1)
+    ts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        LU, ipiv, X1, info1 = oracle.gesv(A1, b1, nb=256, ib=32, nthreads=nthreads)
+        ts.append(time.perf_counter() - t)
+    t1 = float(np.median(ts))
+    c1 = {"n": n1, "nb": 256, "ms": round(1e3 * t1, 2),
+          "tflops": round((2.0 / 3.0) * n1 ** 3 / t1 / 1e12, 5), "info": int(info1),
+          "scaled_residual": float("%.3e" % oracle.scaled_residual(A1, X1, b1))}
+    if gpu_c1 is not None:
+        c1["pivots_equal_gpu"] = bool(np.array_equal(np.asarray(ipiv), np.asarray(gpu_c1)))
     A = oracle.rands(0, n_cpu, n_cpu)
     b = oracle.rands(1, n_cpu, 1)
     solve = oracle.gesv_mixed_gmres if gmres else oracle.gesv_mixed
     t = time.perf_counter()
-    _, _, X, info, it = solve(A, b, nb=256, ib=16, nthreads=max(1, threads // 2))
+    _, _, X, info, it = solve(A, b, nb=256, ib=16, nthreads=nthreads)
     dt = time.perf_counter() - t
     tf = (2.0 / 3.0) * n_cpu ** 3 / dt / 1e12
+    oracle.set_blas(None)
     return {
         "value": round(tf, 5), "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
-        "sample": "oracle %s n=%d nrhs=1 (float tile LU nb=256 ib=16 + FP64 %s, %d iters), "
-                  "matgen rands seed 0/1, %.1f s" % ("gesv_mixed_gmres" if gmres else "gesv_mixed", n_cpu,
-                                                     "GMRES-IR" if gmres else "IR", it, dt),
+        "blas": os.path.basename(blas) if blas else "oracle loops",
+        "sample": "oracle %s n=%d nrhs=1 (float tile LU nb=256 ib=16, %d panel threads, tile BLAS %s, "
+                  "+ FP64 %s, %d iters), matgen rands seed 0/1, %.1f s"
+                  % ("gesv_mixed_gmres" if gmres else "gesv_mixed", n_cpu, nthreads,
+                     "OpenBLAS" if blas else "loops", "GMRES-IR" if gmres else "IR", it, dt),
+        "c1_plain_fp64_gesv": c1,
     }
+
+
+def c1_gpu_pivots():
+    """Pivots of the GPU FP64 getrf at n = 2000, nb = 256 (matgen rands seed 0)."""
+    import torch
+    import slate_amd as S
+    A = S.empty_colmajor(2000, 2000, torch.float64)
+    S.generate(A, S.DIST_RANDS, 0)
+    _, ipiv = S.getrf(A, S.Options(block_size=256))
+    return ipiv
+
+
+def errors_dev(A, x, b, x_ref):
+    """Normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf),
+    the tester's scaled residual ||b - A x||_1 / (n ||A||_1 ||x||_1 eps)
+    (test/test_gesv.cc:330-377) and the forward error against x_ref,
+    ||x - x_ref||_inf / ||x_ref||_inf; on the GPU (checker only)."""
+    import torch
+    n = A.shape[0]
+    r = b - A @ x
+    Aabs = A.abs()
+    ainf = Aabs.sum(dim=1).max()
+    a1 = Aabs.sum(dim=0).max()
+    del Aabs
+    eps = torch.finfo(torch.float64).eps
+    out = {
+        "backward_error": float(r.abs().max() / (ainf * x.abs().max() + b.abs().max())),
+        "scaled_residual": float(r.abs().sum() / (n * a1 * x.abs().sum()) / eps),
+    }
+    if x_ref is not None:
+        out["forward_error_vs_fp64"] = float((x - x_ref).abs().max() / x_ref.abs().max())
+    return {k: float("%.3e" % v) for k, v in out.items()}
 
 
 def bench_batched(args):
@@ -151,6 +202,7 @@ def main():
                     help="batched: many uneven systems in one launch instead of the headline solve")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
+    ap.add_argument("--fp64-runs", type=int, default=3, help="timed plain FP64 solves (after one warmup)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one plain elementwise copy (y = 2 x, float32) of this many MiB "
@@ -211,30 +263,48 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     flops = 2.0 / 3.0 * n ** 3
     value = flops / (ms_per_step * 1e-3) / 1e12
-    res_mixed = scaled_residual_dev(A, X[:, 0], B[:, 0])
-    log("bench: %s %.1f ms/step (device %.1f ms), %.2f TFLOP/s, iters %s, scaled residual %.3f"
-        % ("gesv_mixed_gmres" if gmres else "gesv_mixed", ms_per_step, dev_ms / args.steps, value, iters, res_mixed))
+    log("bench: %s %.1f ms/step (device %.1f ms), %.2f TFLOP/s, iters %s"
+        % ("gesv_mixed_gmres" if gmres else "gesv_mixed", ms_per_step, dev_ms / args.steps, value, iters))
 
     fp64 = None
+    x64 = None
     if not args.no_fp64:
-        A64 = A.clone()
-        B64 = B.clone()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        info64, _ = S.gesv(A64, B64, opts)
-        torch.cuda.synchronize()
-        t64 = time.perf_counter() - t
-        res64 = scaled_residual_dev(A, B64[:, 0], B[:, 0])
-        fp64 = {"ms": round(1e3 * t64, 2), "tflops": round(flops / t64 / 1e12, 3),
-                "scaled_residual": round(res64, 4), "info": info64}
-        log("bench: plain FP64 gesv %.1f ms, %.2f TFLOP/s, scaled residual %.3f" % (1e3 * t64, fp64["tflops"], res64))
+        # plain FP64 gesv on the same input: 1 warmup + fp64_runs timed, median
+        A64 = torch.empty_like(A)
+        B64 = torch.empty_like(B)
+        opts64 = S.Options(block_size=args.nb)
+        ts = []
+        for rep in range(1 + args.fp64_runs):
+            A64.copy_(A)
+            B64.copy_(B)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            info64, _ = S.gesv(A64, B64, opts64)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            assert info64 == 0, "plain FP64 gesv: info %d" % info64
+            if rep > 0:
+                ts.append(dt)
+            log("bench: plain FP64 gesv %s %.1f ms" % ("warmup" if rep == 0 else "run %d" % rep, 1e3 * dt))
+        t64 = sorted(ts)[len(ts) // 2]
+        x64 = B64[:, 0].clone()
+        err64 = errors_dev(A, x64, B[:, 0], None)
+        assert err64["scaled_residual"] <= 3.0, "plain FP64 gesv: scaled residual %g" % err64["scaled_residual"]
+        fp64 = {"ms_median": round(1e3 * t64, 2), "ms_runs": [round(1e3 * v, 2) for v in ts],
+                "tflops": round(flops / t64 / 1e12, 3), "info": info64, **err64}
+        log("bench: plain FP64 gesv median %.1f ms, %.2f TFLOP/s, scaled residual %.3e"
+            % (1e3 * t64, fp64["tflops"], err64["scaled_residual"]))
         del A64, B64
         torch.cuda.empty_cache()
+    err_mixed = errors_dev(A, X[:, 0], B[:, 0], x64)
+    log("bench: mixed solution %s" % err_mixed)
+    assert err_mixed["scaled_residual"] <= 3.0, "mixed solve: scaled residual %g" % err_mixed["scaled_residual"]
 
-    # Dominant kernel: the trailing update.  Its MFMA work is the algorithmic
-    # 2mnk per launch times the bf16 products per update of the split form.
-    mfma_flops = gemm_flops * (products if products else 1)
-    achieved = mfma_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    # Dominant kernel: the trailing update.  roofline.achieved counts the
+    # algorithmic 2mnk flops per launch; the MFMA issue rate (the bf16 products
+    # of the split form) is reported separately.
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    mfma_products = products if products else 1
     if products:
         peak, kname = PEAK["bf16"], ("gemm_sp_kernel<%d,1,3> trailing update (float operands split into bf16 "
                                      "terms at LDS staging, v_mfma_f32_16x16x32_bf16, %d products per update)"
@@ -249,10 +319,12 @@ def main():
         "kernel": kname,
         "launches": int(launches), "avg_launch_ms": round(gemm_ms / max(launches, 1), 4),
         "algorithmic_flops_per_launch": round(gemm_flops / max(launches, 1), 1),
+        "mfma_issue_tflops": round(achieved * mfma_products, 3),
+        "mfma_issue_frac": round(achieved * mfma_products / peak, 4),
     }
     if traffic_src:
         roofline["traffic_source"] = traffic_src
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, gmres)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, gmres, c1_gpu_pivots())
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
@@ -266,9 +338,9 @@ def main():
                            "GMRES-IR" if gmres else "classic IR", n),
             "n": n, "nrhs": 1, "nb": args.nb or "auto", "factor_precision": args.factor_precision,
             "refinement": "GMRES-IR" if gmres else "classic IR", "iters": iters,
-            "scaled_residual": round(res_mixed, 4),
+            **err_mixed,
             "plain_fp64_gesv": fp64,
-            "speedup_vs_plain_fp64": round(fp64["ms"] / ms_per_step, 3) if fp64 else None,
+            "speedup_vs_plain_fp64": round(fp64["ms_median"] / ms_per_step, 3) if fp64 else None,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

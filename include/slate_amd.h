@@ -28,13 +28,21 @@ typedef struct slate_amd_context_t* slate_amd_context;
 /* Factorization precision of the mixed-precision drivers.  The BF16 forms keep
  * a float factor whose trailing updates run on bf16 MFMA with fp32
  * accumulation, each float operand split into 1, 2 or 3 bf16 terms (about 8,
- * 16 or 24 significant bits per product; BF16X3 is float-accurate). */
+ * 16 or 24 significant bits per product; BF16X3 is float-accurate).  The FP8
+ * forms run them on fp8 MFMA: operands quantized to OCP e4m3 with a
+ * power-of-two scale per tile of 128 rows (the largest 2^s with
+ * max|x| 2^s <= 448), one plane (about 4 significant bits) or two (the
+ * remainder in a second plane, about 8 bits, 3 products); block size a
+ * multiple of 128.  Only SLATE's double -> float exists in the reference
+ * (src/gesv_mixed.cc:293-316); the others are this library's. */
 enum {
     SLATE_AMD_PREC_FP32 = 0,   /* float factor (SLATE's scalar_lo = float)       */
     SLATE_AMD_PREC_BF16 = 1,   /* float storage, bf16 MFMA updates, fp32 accum   */
     SLATE_AMD_PREC_BF16X2 = 2, /* two-term bf16 split: 3 products per update     */
     SLATE_AMD_PREC_FP64 = 3,   /* no refinement: plain FP64 factor and solve     */
-    SLATE_AMD_PREC_BF16X3 = 4  /* three-term bf16 split: 6 products per update   */
+    SLATE_AMD_PREC_BF16X3 = 4, /* three-term bf16 split: 6 products per update   */
+    SLATE_AMD_PREC_FP8 = 5,    /* float storage, e4m3 MFMA updates, tile scales  */
+    SLATE_AMD_PREC_FP8X2 = 6   /* e4m3 with a remainder plane: 3 products        */
 };
 
 enum { SLATE_AMD_LOWER = 0, SLATE_AMD_UPPER = 1 };   /* slate::Uplo                */
@@ -183,6 +191,12 @@ int slate_amd_sgemm(slate_amd_context ctx, int64_t m, int64_t n, int64_t k, floa
  * accuracy tests of that kernel. */
 int slate_amd_sgemm_bf16(slate_amd_context ctx, int terms, int64_t m, int64_t n, int64_t k, float alpha,
                          const float* dA, int64_t lda, const float* dB, int64_t ldb, float* dC, int64_t ldc);
+/* The same on the fp8 MFMA kernel behind the FP8 factor precisions: A
+ * quantized to e4m3 per tile of 128 rows, B per tile of 128 columns (each
+ * over all of k) with power-of-two scales, terms = 1 or 2 planes, fp32
+ * accumulation.  No counterpart in the reference; for the accuracy tests. */
+int slate_amd_sgemm_fp8(slate_amd_context ctx, int terms, int64_t m, int64_t n, int64_t k, float alpha,
+                        const float* dA, int64_t lda, const float* dB, int64_t ldb, float* dC, int64_t ldc);
 
 /* slate::norm(Norm::Inf, A) — src/norm.cc:24-240; sym != 0: symmetric, lower stored. */
 int slate_amd_dnorm_inf(slate_amd_context ctx, int sym, int64_t n, const double* dA, int64_t lda,

@@ -56,7 +56,11 @@ enum class Op : char { NoTrans = 'N', Trans = 'T', ConjTrans = 'C' };
 
 // Factorization precision of the mixed drivers; SLATE picks it with the
 // scalar_lo template argument, the BF16 choice has no SLATE counterpart.
-enum class FactorPrecision : int { FP32 = SLATE_AMD_PREC_FP32, BF16 = SLATE_AMD_PREC_BF16 };
+enum class FactorPrecision : int {
+    FP32 = SLATE_AMD_PREC_FP32, BF16 = SLATE_AMD_PREC_BF16, BF16X2 = SLATE_AMD_PREC_BF16X2,
+    BF16X3 = SLATE_AMD_PREC_BF16X3, FP8 = SLATE_AMD_PREC_FP8, FP8X2 = SLATE_AMD_PREC_FP8X2,
+    FP64 = SLATE_AMD_PREC_FP64
+};
 
 enum class MethodLU : char { PartialPiv = 'P', CALU = 'C', NoPiv = 'N' };   // include/slate/enums.hh:294-301
 

@@ -109,6 +109,16 @@ def lib():
             "oracle_gerbt_rhs": (None, [c_int, c_i64, c_i64, c_i64, c_i64, c_p, c_p, c_i64]),
             "oracle_dgesv_rbt": (c_i64, [c_i64, c_i64, c_p, c_i64, c_p, c_i64, c_p, c_i64, c_i64,
                                          c_i64, c_i64, c_i64, c_dbl, c_int, c_p]),
+            "oracle_set_blas": (c_int, [ctypes.c_char_p]),
+            "oracle_set_update_precision": (c_int, [c_int, c_int, c_i64]),
+            "oracle_sgemm_update": (None, [c_i64, c_i64, c_i64, c_p, c_i64, c_p, c_i64, c_p, c_i64]),
+            "oracle_e4m3_decode": (c_dbl, [c_int]),
+            "oracle_e4m3_round": (c_dbl, [c_dbl]),
+            "oracle_e4m3_encode": (c_int, [c_dbl]),
+            "oracle_e4m3_scale_exp": (c_int, [c_dbl]),
+            "oracle_bf16_rne": (ctypes.c_float, [ctypes.c_float]),
+            "oracle_gmres_first_cycle": (c_i64, [c_i64, c_p, c_i64, c_int, c_p, c_p, c_i64, c_i64, c_int,
+                                                 c_p, c_p, c_p, c_p]),
             "oracle_num_threads": (c_int, []),
             "oracle_set_num_threads": (None, [c_int]),
         }
@@ -448,3 +458,92 @@ def scaled_residual(A, X, B):
 def ipiv_from_lapack(ipiv_lapack):
     """LAPACK 1-based int32 ipiv -> int64 array (same convention as the oracle)."""
     return np.asarray(ipiv_lapack, dtype=np.int64) + 0
+
+
+# ---- low-precision update emulation, FP8 e4m3, tile BLAS ---------------------
+UPD_FLOAT, UPD_BF16, UPD_FP8 = 0, 1, 2
+
+
+class update_precision:
+    """Context manager: the float factor's trailing updates use split-BF16
+    (kind UPD_BF16, terms 1-3) or per-tile scaled FP8 e4m3 (UPD_FP8, terms
+    1-2, tiles of `qrows` rows) operands, as the GPU factor precisions do."""
+
+    def __init__(self, kind, terms, qrows=128):
+        self.args = (int(kind), int(terms), int(qrows))
+
+    def __enter__(self):
+        if lib().oracle_set_update_precision(*self.args) != 0:
+            raise ValueError("bad update precision %r" % (self.args,))
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_update_precision(0, 0, 128)
+
+
+def sgemm_update(A, B, C):
+    """C - A B (float32) under the current update precision; returns a new array."""
+    A = _fmat(A, np.float32)
+    B = _fmat(B, np.float32)
+    C = _fmat(C, np.float32).copy(order="F")
+    m, k = A.shape
+    n = B.shape[1]
+    lib().oracle_sgemm_update(m, n, k, _ptr(A), m, _ptr(B), k, _ptr(C), m)
+    return C
+
+
+def e4m3_decode(b):
+    return lib().oracle_e4m3_decode(int(b))
+
+
+def e4m3_round(x):
+    return lib().oracle_e4m3_round(float(x))
+
+
+def e4m3_encode(x):
+    return lib().oracle_e4m3_encode(float(x))
+
+
+def e4m3_scale_exp(amax):
+    return lib().oracle_e4m3_scale_exp(float(amax))
+
+
+def bf16_rne(x):
+    return lib().oracle_bf16_rne(float(x))
+
+
+def openblas_path():
+    """scipy's bundled LP64 OpenBLAS (exports scipy_dgemm_ ...), or None."""
+    import glob
+    try:
+        import scipy
+    except ImportError:
+        return None
+    d = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+    hits = sorted(glob.glob(os.path.join(d, "libscipy_openblas-*.so")))
+    return hits[0] if hits else None
+
+
+def set_blas(path):
+    """Tile gemm/trsm through the OpenBLAS at `path` (None: plain loops)."""
+    rc = lib().oracle_set_blas(path.encode() if path else None)
+    if rc != 0:
+        raise RuntimeError("oracle_set_blas(%r) failed" % (path,))
+
+
+def gmres_first_cycle(A, b, nb=256, itermax=30, sym=False, cgs_passes=2):
+    """First restart cycle of the oracle's gesv_mixed_gmres (posv_mixed_gmres
+    with sym): returns (V n x (j+1), W n x j, Hbar (j+1) x j, |S| history).
+    cgs_passes=1 plants a one-pass classical Gram-Schmidt."""
+    A = _fmat(A)
+    n = A.shape[0]
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.float64).reshape(n))
+    X = np.zeros(n)
+    R1 = min(30, itermax, min(nb, n) - 1) + 1
+    V = np.zeros((n, R1), order="F")
+    W = np.zeros((n, R1), order="F")
+    H = np.zeros((R1, R1), order="F")
+    S = np.zeros(R1)
+    j = lib().oracle_gmres_first_cycle(n, _ptr(A), n, int(sym), _ptr(b), _ptr(X), nb, itermax,
+                                       int(cgs_passes), _ptr(V), _ptr(W), _ptr(H), _ptr(S))
+    return V[:, :j + 1], W[:, :j], H[:j + 1, :j], S[:j]

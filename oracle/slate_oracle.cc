@@ -19,6 +19,7 @@
 #include <cstring>
 #include <limits>
 #include <vector>
+#include <dlfcn.h>
 #include <omp.h>
 
 namespace {
@@ -64,12 +65,207 @@ inline double gen_one(int dist, int64_t seed, int64_t i, int64_t j)
 template <class T> inline T cabs1(T x) { return std::fabs(x); }   // util.hh:125-140 (real)
 
 //------------------------------------------------------------------------------
+// Optional tile BLAS.  SLATE's HostTask tiles call a vendor BLAS
+// (src/internal/internal_gemm.cc, Tile_blas.hh); for the timed CPU baseline the
+// oracle can route its tile gemm/trsm through an OpenBLAS loaded at run time
+// (oracle_set_blas; Fortran LP64 interface, symbols scipy_<name>_ as exported
+// by scipy's bundled OpenBLAS), one BLAS thread per OpenMP task as in SLATE.
+// Without it the plain loops below run.  Results agree to rounding only.
+typedef void (*gemm_d_t)(const char*, const char*, const int*, const int*, const int*, const double*,
+                         const double*, const int*, const double*, const int*, const double*, double*,
+                         const int*);
+typedef void (*gemm_s_t)(const char*, const char*, const int*, const int*, const int*, const float*,
+                         const float*, const int*, const float*, const int*, const float*, float*,
+                         const int*);
+typedef void (*trsm_d_t)(const char*, const char*, const char*, const char*, const int*, const int*,
+                         const double*, const double*, const int*, double*, const int*);
+typedef void (*trsm_s_t)(const char*, const char*, const char*, const char*, const int*, const int*,
+                         const float*, const float*, const int*, float*, const int*);
+struct Blas {
+    gemm_d_t dgemm = nullptr;
+    gemm_s_t sgemm = nullptr;
+    trsm_d_t dtrsm = nullptr;
+    trsm_s_t strsm = nullptr;
+};
+Blas g_blas;
+
+template <class T> inline bool blas_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha,
+                                         const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
+                                         T* C, int64_t ldc)
+{
+    const int M = int(m), N = int(n), K = int(k), la = int(lda), lb = int(ldb), lc = int(ldc);
+    if constexpr (sizeof(T) == 8) {
+        if (!g_blas.dgemm) return false;
+        g_blas.dgemm(&ta, &tb, &M, &N, &K, &alpha, A, &la, B, &lb, &beta, C, &lc);
+    }
+    else {
+        if (!g_blas.sgemm) return false;
+        g_blas.sgemm(&ta, &tb, &M, &N, &K, &alpha, A, &la, B, &lb, &beta, C, &lc);
+    }
+    return true;
+}
+
+//------------------------------------------------------------------------------
+// Low-precision trailing updates of the float factor (this library's factor
+// precisions beyond SLATE's float; the reference has double -> float only,
+// src/gesv_mixed.cc:293-316).  With g_q.kind != 0 the float factor's trailing
+// update C -= A B (LU gemm, Cholesky herk/gemm) multiplies rounded operands,
+// products exact in float, accumulated in double and subtracted in float:
+//  * kind 1, terms t = 1..3: each float x split into bf16 terms
+//    x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1) (round to
+//    nearest even); the products x_i y_j with i + j < t are kept.
+//  * kind 2, terms t = 1..2: FP8 e4m3 (OCP E4M3FN) with a power-of-two scale
+//    per tile.  A tile is a block of `qrows` rows (of A) or columns (of B)
+//    over the whole inner dimension k of the update, counted from the first
+//    row/column of the update; its scale 2^s is the largest with
+//    max|x| 2^s <= 448.  q0 = e4m3(x 2^s); with t = 2 the remainder plane
+//    q1 = e4m3((x 2^s - q0) 2^4); the product is
+//    2^-(sa+sb) (qa0 qb0 + 2^-4 (qa0 qb1 + qa1 qb0)).
+struct UpdQ { int kind = 0; int terms = 0; int64_t qrows = 128; };
+UpdQ g_q;
+
+inline float bf16_rne(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+// OCP FP8 E4M3FN: bias 7, no infinities, S.1111.111 = NaN, max 448 =
+// 1.75 * 2^8, min normal 2^-6, subnormals m * 2^-9.
+inline double e4m3_decode(uint8_t b)
+{
+    const int sgn = b >> 7, e = (b >> 3) & 15, m = b & 7;
+    if (e == 15 && m == 7) return std::numeric_limits<double>::quiet_NaN();
+    const double v = e == 0 ? std::ldexp(double(m), -9) : std::ldexp(1.0 + m / 8.0, e - 7);
+    return sgn ? -v : v;
+}
+
+// Round to the nearest e4m3 value, ties to even, saturating at +-448 (the
+// scaled operands never exceed it).  The quantum of the binade of |x| is
+// 2^(e-3), 2^-9 below the normal range; rint rounds half to even.
+inline double e4m3_round(double x)
+{
+    if (!(std::fabs(x) > 0)) return x == x ? 0.0 * x : x;
+    int e;
+    std::frexp(std::fabs(x), &e);          // |x| in [2^(e-1), 2^e)
+    const int q = std::max(e - 1, -6) - 3;
+    double r = std::ldexp(std::nearbyint(std::ldexp(x, -q)), q);
+    if (r > 448) r = 448;
+    if (r < -448) r = -448;
+    return r;
+}
+
+// Encoded byte of e4m3_round(x) (x finite).
+inline uint8_t e4m3_encode(double x)
+{
+    const double r = e4m3_round(x);
+    const uint8_t sgn = std::signbit(r) ? 0x80 : 0;
+    const double a = std::fabs(r);
+    if (a == 0) return sgn;
+    int e;
+    const double f = std::frexp(a, &e);    // a = f 2^e, f in [0.5, 1)
+    int be = e - 1 + 7;
+    if (be <= 0) return uint8_t(sgn | int(std::ldexp(a, 9)));          // subnormal: m 2^-9
+    const int m = int(std::ldexp(f, 4)) - 8;                             // (1 + m/8) 2^(e-1)
+    return uint8_t(sgn | (be << 3) | m);
+}
+
+// Largest s with amax 2^s <= 448 (0 for an all-zero tile).
+inline int e4m3_scale_exp(double amax)
+{
+    if (!(amax > 0)) return 0;
+    int e;
+    const double f = std::frexp(amax, &e); // amax = f 2^e, 448 = 0.875 2^9
+    return f <= 0.875 ? 9 - e : 8 - e;
+}
+
+// Quantized planes of an m x k operand (element (i, p) at X[i + p*ldx] when
+// rows_are_m, else X[p + i*ldx]); tiles of g_q.qrows along m starting at row
+// `tile0` of the operand.  out[t][i + p*m] = term t in the unscaled units,
+// scale[i] = 2^-s of row i's tile (1 for the bf16 split).
+template <class T>
+void quantize_operand(int64_t m, int64_t k, const T* X, int64_t ldx, bool rows_are_m,
+                      std::vector<double>* out, std::vector<double>& scale)
+{
+    auto at = [&](int64_t i, int64_t p) -> double { return rows_are_m ? X[i + p*ldx] : X[p + i*ldx]; };
+    const int t = g_q.terms;
+    for (int q = 0; q < t; ++q) out[q].assign(m * k, 0.0);
+    scale.assign(m, 1.0);
+    if (g_q.kind == 1) {
+        for (int64_t p = 0; p < k; ++p)
+            for (int64_t i = 0; i < m; ++i) {
+                float r = float(at(i, p));
+                for (int q = 0; q < t; ++q) {
+                    const float h = bf16_rne(r);
+                    out[q][i + p*m] = h;
+                    r = r - h;
+                }
+            }
+        return;
+    }
+    const int64_t qr = g_q.qrows;
+    for (int64_t i0 = 0; i0 < m; i0 += qr) {
+        const int64_t i1 = std::min(m, i0 + qr);
+        double amax = 0;
+        for (int64_t p = 0; p < k; ++p)
+            for (int64_t i = i0; i < i1; ++i) amax = std::max(amax, std::fabs(at(i, p)));
+        const int s = e4m3_scale_exp(amax);
+        for (int64_t i = i0; i < i1; ++i) scale[i] = std::ldexp(1.0, -s);
+        for (int64_t p = 0; p < k; ++p)
+            for (int64_t i = i0; i < i1; ++i) {
+                const double xs = std::ldexp(double(float(at(i, p))), s);
+                const double q0 = e4m3_round(xs);
+                out[0][i + p*m] = q0;
+                if (t > 1) out[1][i + p*m] = e4m3_round((xs - q0) * 16.0);
+            }
+    }
+}
+
+// C[m x n] -= A B on quantized operands; A m x k (column-major, lda), B given
+// as op(B) = k x n from Bsrc: b_is_kn ? Bsrc[p + j*ldb] : Bsrc[j + p*ldb].
+// Only C(i, j) with (!lower_only || i >= j) are updated.
+template <class T>
+void gemm_quantized(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const T* Bsrc, int64_t ldb,
+                    bool b_is_kn, T* C, int64_t ldc, bool lower_only = false)
+{
+    std::vector<double> qa[3], qb[3], sa, sb;
+    quantize_operand<T>(m, k, A, lda, true, qa, sa);
+    // B as an n x k operand: element (j, p)
+    quantize_operand<T>(n, k, Bsrc, ldb, !b_is_kn, qb, sb);
+    const int t = g_q.terms;
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = lower_only ? j : 0; i < m; ++i) {
+            double acc = 0;
+            for (int64_t p = 0; p < k; ++p) {
+                if (g_q.kind == 1) {
+                    for (int a = 0; a < t; ++a)
+                        for (int b = 0; a + b < t; ++b) acc += qa[a][i + p*m] * qb[b][j + p*n];
+                }
+                else {
+                    double v = qa[0][i + p*m] * qb[0][j + p*n];
+                    if (t > 1) v += (qa[0][i + p*m] * qb[1][j + p*n] + qa[1][i + p*m] * qb[0][j + p*n]) / 16.0;
+                    acc += v;
+                }
+            }
+            acc *= sa[i] * sb[j];
+            C[i + j*ldc] = T(C[i + j*ldc] - T(acc));
+        }
+}
+
+//------------------------------------------------------------------------------
 // Plain loop kernels on column-major blocks.
 // C[m x n] += alpha * A[m x k] * B[k x n]   (op = NoTrans/NoTrans)
 template <class T>
 void gemm_nn(int64_t m, int64_t n, int64_t k, T alpha,
              const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc)
 {
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    if (blas_gemm<T>('N', 'N', m, n, k, alpha, A, lda, B, ldb, T(1), C, ldc)) return;
     for (int64_t j = 0; j < n; ++j)
         for (int64_t p = 0; p < k; ++p) {
             T b = alpha * B[p + j*ldb];
@@ -85,6 +281,8 @@ template <class T>
 void gemm_nt(int64_t m, int64_t n, int64_t k, T alpha,
              const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc)
 {
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    if (blas_gemm<T>('N', 'T', m, n, k, alpha, A, lda, B, ldb, T(1), C, ldc)) return;
     for (int64_t j = 0; j < n; ++j)
         for (int64_t p = 0; p < k; ++p) {
             T b = alpha * B[j + p*ldb];
@@ -99,6 +297,17 @@ void gemm_nt(int64_t m, int64_t n, int64_t k, T alpha,
 template <class T>
 void trsm_left_lower(int64_t m, int64_t n, bool unit, const T* L, int64_t ldl, T* B, int64_t ldb)
 {
+    if (m > 0 && n > 0) {
+        const char sd = 'L', ul = 'L', tr = 'N', dg = unit ? 'U' : 'N';
+        const int M = int(m), N = int(n), l1 = int(ldl), l2 = int(ldb);
+        const T one = 1;
+        if constexpr (sizeof(T) == 8) {
+            if (g_blas.dtrsm) { g_blas.dtrsm(&sd, &ul, &tr, &dg, &M, &N, &one, L, &l1, B, &l2); return; }
+        }
+        else {
+            if (g_blas.strsm) { g_blas.strsm(&sd, &ul, &tr, &dg, &M, &N, &one, L, &l1, B, &l2); return; }
+        }
+    }
     for (int64_t j = 0; j < n; ++j) {
         T* b = B + j*ldb;
         for (int64_t p = 0; p < m; ++p) {
@@ -335,9 +544,13 @@ int64_t getrf_tiles(int64_t m, int64_t n, T* A, int64_t lda, int64_t nb, int64_t
         for (int64_t t = 0; t < cnt; ++t) {
             int64_t i = k + 1 + t % (mt - k - 1);
             int64_t j = k + 1 + t / (mt - k - 1);
-            gemm_nn<T>(g.tmb(i), g.tnb(j), g.tnb(k), T(-1),
-                       A + i*nb + k*nb*lda, lda, A + k*nb + j*nb*lda, lda,
-                       A + i*nb + j*nb*lda, lda);
+            if (sizeof(T) == 4 && g_q.kind != 0)
+                gemm_quantized<T>(g.tmb(i), g.tnb(j), g.tnb(k), A + i*nb + k*nb*lda, lda,
+                                  A + k*nb + j*nb*lda, lda, true, A + i*nb + j*nb*lda, lda);
+            else
+                gemm_nn<T>(g.tmb(i), g.tnb(j), g.tnb(k), T(-1),
+                           A + i*nb + k*nb*lda, lda, A + k*nb + j*nb*lda, lda,
+                           A + i*nb + j*nb*lda, lda);
         }
         kk += g.tnb(k);
     }
@@ -437,7 +650,12 @@ int64_t potrf_tiles(int64_t n, T* A, int64_t lda, int64_t nb)
             int64_t i = k + 1 + t % r, j = k + 1 + t / r;
             if (i < j) continue;
             T* Cij = A + i*nb + j*nb*lda;
-            if (i == j) {
+            if (sizeof(T) == 4 && g_q.kind != 0) {
+                // herk (lower triangle of the diagonal tile) or gemm on quantized operands
+                gemm_quantized<T>(g.tmb(i), g.tnb(j), g.tnb(k), A + i*nb + k*nb*lda, lda,
+                                  A + j*nb + k*nb*lda, lda, false, Cij, lda, i == j);
+            }
+            else if (i == j) {
                 // herk, lower triangle only
                 std::vector<T> tmp(g.tmb(i) * g.tmb(i), T(0));
                 gemm_nt<T>(g.tmb(i), g.tmb(i), g.tnb(k), T(1), A + i*nb + k*nb*lda, lda,
@@ -651,6 +869,25 @@ inline void rot(double* x, double* y, double c, double s)
     *x = t;
 }
 
+// Inspection of the first restart cycle of mixed_gmres (tests only): with
+// g_gd.active set, the cycle's basis V (n x (restart+1)), the Hessenberg
+// matrix before the Givens rotations (Hbar, (restart+1) x restart, column j =
+// the CGS2 coefficients of step j and the norm of the new vector), |S(j+1)|
+// after each step and the basis W = M^{-1} V are copied out, and the driver
+// stops after that cycle.  cgs_passes = 1 plants a one-pass classical
+// Gram-Schmidt (the reference reorthogonalises, gesv_mixed_gmres.cc:310-327),
+// so that a test can show the orthogonality check detects it.
+struct GmresDebug {
+    bool active = false;
+    int cgs_passes = 2;
+    double* V = nullptr;
+    double* W = nullptr;
+    double* Hbar = nullptr;
+    double* Sabs = nullptr;
+    int64_t steps = 0;
+};
+GmresDebug g_gd;
+
 // GMRES-IR, src/gesv_mixed_gmres.cc:111-410 (single RHS), and the same with a
 // Cholesky preconditioner for posv_mixed_gmres.  The preconditioner applies
 // the low-precision factor to a float-rounded vector (:282-286), the Arnoldi
@@ -713,28 +950,40 @@ int64_t mixed_gmres(int64_t n, double* A, int64_t lda, int sym, const double* B,
                     const double* Vp = V.data() + p*n;
                     for (int64_t i = 0; i < n; ++i) Vj1[i] -= Vp[i] * Hj[p];
                 }
-                for (int64_t p = 0; p <= j; ++p) {
-                    double s = 0; const double* Vp = V.data() + p*n;
-                    for (int64_t i = 0; i < n; ++i) s += Vp[i] * Vj1[i];
-                    z[p] = s;
+                if (!(g_gd.active && g_gd.cgs_passes == 1)) {
+                    for (int64_t p = 0; p <= j; ++p) {
+                        double s = 0; const double* Vp = V.data() + p*n;
+                        for (int64_t i = 0; i < n; ++i) s += Vp[i] * Vj1[i];
+                        z[p] = s;
+                    }
+                    for (int64_t p = 0; p <= j; ++p) {
+                        const double* Vp = V.data() + p*n;
+                        for (int64_t i = 0; i < n; ++i) Vj1[i] -= Vp[i] * z[p];
+                    }
+                    for (int64_t p = 0; p <= j; ++p) Hj[p] += z[p];
                 }
-                for (int64_t p = 0; p <= j; ++p) {
-                    const double* Vp = V.data() + p*n;
-                    for (int64_t i = 0; i < n; ++i) Vj1[i] -= Vp[i] * z[p];
-                }
-                for (int64_t p = 0; p <= j; ++p) Hj[p] += z[p];
                 double nrm = 0;
                 for (int64_t i = 0; i < n; ++i) nrm += Vj1[i] * Vj1[i];
                 nrm = std::sqrt(nrm);
                 const double sc1 = 1.0 / nrm;
                 for (int64_t i = 0; i < n; ++i) Vj1[i] *= sc1;
                 Hj[j+1] = nrm;
+                if (g_gd.active && g_gd.Hbar)
+                    for (int64_t p = 0; p <= j + 1; ++p) g_gd.Hbar[p + j*R1] = Hj[p];
                 for (int64_t i = 0; i < j; ++i) rot(&Hj[i], &Hj[i+1], ga[i], gb[i]);
                 double hjj = Hj[j], hj1 = Hj[j+1];
                 rotg(&hjj, &hj1, &ga[j], &gb[j]);
                 rot(&Hj[j], &Hj[j+1], ga[j], gb[j]);
                 rot(&S[j], &S[j+1], ga[j], gb[j]);
                 arn[0] = std::fabs(S[j+1]);
+                if (g_gd.active && g_gd.Sabs) g_gd.Sabs[j] = arn[0];
+            }
+            if (g_gd.active) {
+                g_gd.steps = j;
+                if (g_gd.V) std::copy(V.begin(), V.begin() + n * (j + 1), g_gd.V);
+                if (g_gd.W) std::copy(W.begin() + n, W.begin() + n * (j + 1), g_gd.W);
+                converged = true;
+                break;
             }
             // X += W(:,1:j) * (H(0:j-1,0:j-1)^{-1} S(0:j-1)) (:360-374)
             std::vector<double> y(S.begin(), S.begin() + j);
@@ -934,6 +1183,75 @@ int64_t gesv_rbt(int64_t n, int64_t nrhs, double* A, int64_t lda, const double* 
 
 //==============================================================================
 extern "C" {
+
+// Route the tile gemm/trsm through the OpenBLAS at `path` (scipy's bundled
+// LP64 build, symbols scipy_dgemm_ ...), one BLAS thread per OpenMP task.
+// path = NULL unloads.  Returns 0, or -1 if a symbol is missing.
+int oracle_set_blas(const char* path)
+{
+    g_blas = Blas{};
+    if (!path) return 0;
+    void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1;
+    g_blas.dgemm = reinterpret_cast<gemm_d_t>(dlsym(h, "scipy_dgemm_"));
+    g_blas.sgemm = reinterpret_cast<gemm_s_t>(dlsym(h, "scipy_sgemm_"));
+    g_blas.dtrsm = reinterpret_cast<trsm_d_t>(dlsym(h, "scipy_dtrsm_"));
+    g_blas.strsm = reinterpret_cast<trsm_s_t>(dlsym(h, "scipy_strsm_"));
+    auto set_threads = reinterpret_cast<void (*)(int)>(dlsym(h, "scipy_openblas_set_num_threads"));
+    if (set_threads) set_threads(1);
+    if (!g_blas.dgemm || !g_blas.sgemm || !g_blas.dtrsm || !g_blas.strsm || !set_threads) {
+        g_blas = Blas{};
+        return -1;
+    }
+    return 0;
+}
+
+// Low-precision trailing updates of the float factor: kind 0 float, 1 split
+// bf16 (terms 1..3), 2 FP8 e4m3 with per-tile power-of-two scales (terms
+// 1..2, tiles of qrows rows/columns).
+int oracle_set_update_precision(int kind, int terms, int64_t qrows)
+{
+    if (kind < 0 || kind > 2) return -1;
+    if (kind == 1 && (terms < 1 || terms > 3)) return -1;
+    if (kind == 2 && (terms < 1 || terms > 2)) return -1;
+    if (qrows < 1) return -1;
+    g_q.kind = kind; g_q.terms = kind ? terms : 0; g_q.qrows = qrows;
+    return 0;
+}
+
+// C (m x n, float, column-major) -= A B with the current update precision
+// (A m x k, B k x n, both float column-major); kind 0 = plain float gemm.
+void oracle_sgemm_update(int64_t m, int64_t n, int64_t k, const float* A, int64_t lda, const float* B,
+                         int64_t ldb, float* C, int64_t ldc)
+{
+    if (g_q.kind != 0) gemm_quantized<float>(m, n, k, A, lda, B, ldb, true, C, ldc);
+    else gemm_nn<float>(m, n, k, -1.0f, A, lda, B, ldb, C, ldc);
+}
+
+double oracle_e4m3_decode(int b) { return e4m3_decode(uint8_t(b)); }
+double oracle_e4m3_round(double x) { return e4m3_round(x); }
+int oracle_e4m3_encode(double x) { return e4m3_encode(x); }
+int oracle_e4m3_scale_exp(double amax) { return e4m3_scale_exp(amax); }
+float oracle_bf16_rne(float x) { return bf16_rne(x); }
+
+// The oracle's gesv_mixed_gmres (sym 0) / posv_mixed_gmres (sym 1, lower)
+// stopped after its first restart cycle; see GmresDebug.  Returns the number
+// of Arnoldi steps j; V gets n x (j+1), W n x j, Hbar (restart+1) x restart,
+// Sabs j values.
+int64_t oracle_gmres_first_cycle(int64_t n, double* A, int64_t lda, int sym, const double* B, double* X,
+                                 int64_t nb, int64_t itermax, int cgs_passes, double* V, double* W,
+                                 double* Hbar, double* Sabs)
+{
+    g_gd = GmresDebug{};
+    g_gd.active = true; g_gd.cgs_passes = cgs_passes;
+    g_gd.V = V; g_gd.W = W; g_gd.Hbar = Hbar; g_gd.Sabs = Sabs;
+    std::vector<int64_t> ipiv(n);
+    int it = 0;
+    mixed_gmres(n, A, lda, sym, B, X, ipiv.data(), nb, 16, 1, 1.0, itermax, 0.0, 0, &it);
+    const int64_t steps = g_gd.steps;
+    g_gd = GmresDebug{};
+    return steps;
+}
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }
 void oracle_set_num_threads(int t) { omp_set_num_threads(t); }

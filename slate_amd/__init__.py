@@ -13,6 +13,7 @@ zero pivot / non-SPD minor), and for the mixed drivers also ``iter``.  Errors
 raise ``SlateError``; there is no CPU fallback — if the library is missing or
 no GPU is present, calls fail loudly.
 """
+import atexit
 import ctypes
 import os
 
@@ -21,7 +22,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libslate_amd.so")
 
-PREC_FP32, PREC_BF16, PREC_BF16X2, PREC_FP64, PREC_BF16X3 = 0, 1, 2, 3, 4
+PREC_FP32, PREC_BF16, PREC_BF16X2, PREC_FP64, PREC_BF16X3, PREC_FP8, PREC_FP8X2 = 0, 1, 2, 3, 4, 5, 6
 LOWER, UPPER = 0, 1
 NOTRANS, TRANS = 0, 1
 DIST_UNIFORM, DIST_RANDS, DIST_RANDN = 1, 2, 3
@@ -39,7 +40,7 @@ C_SYMBOLS = [
     "slate_amd_dgecondest", "slate_amd_dpocondest", "slate_amd_mlp_num_params",
     "slate_amd_selector_features", "slate_amd_mlp_forward", "slate_amd_mlp_train_step",
     "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb", "slate_amd_dgesv_batched",
-    "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_profile_enable",
+    "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_sgemm_fp8", "slate_amd_profile_enable",
     "slate_amd_profile_read", "slate_amd_dgesv_rbt", "slate_amd_drbt_generate", "slate_amd_dgerbt",
     "slate_amd_dgerbt_rhs",
 ]
@@ -131,6 +132,8 @@ def lib():
                                       ctypes.c_float, vp, i64]),
             "slate_amd_sgemm_bf16": (i32, [vp, i32, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
                                            vp, i64]),
+            "slate_amd_sgemm_fp8": (i32, [vp, i32, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
+                                          vp, i64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -151,6 +154,16 @@ def context(device=0):
         from . import autotune
         autotune.load(device=device)       # measured block sizes, when tuned on this machine
     return _CTX[device]
+
+
+@atexit.register
+def release_contexts():
+    """slate_amd_finalize for every context this process created: waits for
+    the device, then frees the workspaces, events and streams (including the
+    CU-masked update stream) while the HIP runtime is still loaded."""
+    while _CTX:
+        _, h = _CTX.popitem()
+        lib().slate_amd_finalize(h)
 
 
 def last_error():
@@ -175,7 +188,11 @@ def _mat(A, dtypes=("float64",)):
     if str(A.dtype).replace("torch.", "") not in dtypes:
         raise SlateError("unsupported dtype %s" % A.dtype)
     if A.dim() == 1:
+        if A.shape[0] > 1 and A.stride(0) != 1:
+            raise SlateError("vector must be contiguous (stride(0) == 1)")
         return A.data_ptr(), max(1, A.shape[0]), A.device.index or 0
+    if A.dim() != 2:
+        raise SlateError("expected a matrix or a vector")
     if A.stride(0) != 1 and A.shape[0] > 1:
         raise SlateError("matrix must be column-major (stride(0) == 1)")
     lda = A.stride(1) if A.shape[1] > 1 else max(1, A.shape[0])
@@ -184,6 +201,18 @@ def _mat(A, dtypes=("float64",)):
 
 def _ncols(B):
     return 1 if B.dim() == 1 else B.shape[1]
+
+
+def _rows(A, *mats):
+    """A must be square (n x n) and every right-hand side / solution must
+    have exactly n rows (a shorter view with a large stride(1) would pass the
+    C ABI's ld check and be written past its end)."""
+    n = A.shape[0]
+    if A.dim() != 2 or A.shape[1] != n:
+        raise SlateError("expected a square matrix, got %s" % (tuple(A.shape),))
+    for M in mats:
+        if M.shape[0] != n:
+            raise SlateError("expected %d rows, got %d" % (n, M.shape[0]))
 
 
 def colmajor(t):
@@ -226,6 +255,7 @@ def getrs(LU, ipiv, B, trans=False, opts=None):
     """slate::getrs: B := op(A)^{-1} B using the factors of getrf."""
     pa, lda, dev = _mat(LU, ("float64", "float32"))
     pb, ldb, _ = _mat(B, ("float64", "float32"))
+    _rows(LU, B)
     n = LU.shape[0]
     ip = np.ascontiguousarray(ipiv, dtype=np.int64)
     f = lib().slate_amd_dgetrs if LU.dtype.itemsize == 8 else lib().slate_amd_sgetrs
@@ -237,6 +267,7 @@ def gesv(A, B, opts=None):
     """slate::gesv: factor A and overwrite B with the solution; returns (info, ipiv)."""
     pa, lda, dev = _mat(A)
     pb, ldb, _ = _mat(B)
+    _rows(A, B)
     n = A.shape[0]
     ipiv = _ipiv_out(n)
     info = _check(lib().slate_amd_dgesv(context(dev), n, _ncols(B), pa, lda, _p64(ipiv), pb, ldb,
@@ -255,6 +286,7 @@ def potrf(A, uplo=LOWER, opts=None):
 def potrs(L, B, uplo=LOWER, opts=None):
     pa, lda, dev = _mat(L)
     pb, ldb, _ = _mat(B)
+    _rows(L, B)
     _check(lib().slate_amd_dpotrs(context(dev), uplo, L.shape[0], _ncols(B), pa, lda, pb, ldb,
                                   _opts(opts)))
 
@@ -262,6 +294,7 @@ def potrs(L, B, uplo=LOWER, opts=None):
 def posv(A, B, uplo=LOWER, opts=None):
     pa, lda, dev = _mat(A)
     pb, ldb, _ = _mat(B)
+    _rows(A, B)
     return _check(lib().slate_amd_dposv(context(dev), uplo, A.shape[0], _ncols(B), pa, lda, pb, ldb,
                                         _opts(opts)))
 
@@ -272,6 +305,7 @@ def gesv_mixed(A, B, X, opts=None):
     pa, lda, dev = _mat(A)
     pb, ldb, _ = _mat(B)
     px, ldx, _ = _mat(X)
+    _rows(A, B, X)
     n = A.shape[0]
     ipiv = _ipiv_out(n)
     it = ctypes.c_int(0)
@@ -284,6 +318,7 @@ def gesv_mixed_gmres(A, B, X, opts=None):
     pa, lda, dev = _mat(A)
     pb, ldb, _ = _mat(B)
     px, ldx, _ = _mat(X)
+    _rows(A, B, X)
     n = A.shape[0]
     ipiv = _ipiv_out(n)
     it = ctypes.c_int(0)
@@ -296,6 +331,7 @@ def posv_mixed(A, B, X, uplo=LOWER, opts=None):
     pa, lda, dev = _mat(A)
     pb, ldb, _ = _mat(B)
     px, ldx, _ = _mat(X)
+    _rows(A, B, X)
     it = ctypes.c_int(0)
     info = _check(lib().slate_amd_dsposv(context(dev), uplo, A.shape[0], _ncols(B), pa, lda, pb, ldb,
                                          px, ldx, ctypes.byref(it), _opts(opts)))
@@ -306,6 +342,7 @@ def posv_mixed_gmres(A, B, X, uplo=LOWER, opts=None):
     pa, lda, dev = _mat(A)
     pb, ldb, _ = _mat(B)
     px, ldx, _ = _mat(X)
+    _rows(A, B, X)
     it = ctypes.c_int(0)
     info = _check(lib().slate_amd_dsposv_gmres(context(dev), uplo, A.shape[0], _ncols(B), pa, lda,
                                                pb, ldb, px, ldx, ctypes.byref(it), _opts(opts)))
@@ -318,6 +355,7 @@ def gesv_rbt(A, B, X, opts=None):
     pa, lda, dev = _mat(A)
     pb, ldb, _ = _mat(B)
     px, ldx, _ = _mat(X)
+    _rows(A, B, X)
     it = ctypes.c_int(0)
     info = _check(lib().slate_amd_dgesv_rbt(context(dev), A.shape[0], _ncols(B), pa, lda, pb, ldb,
                                             px, ldx, ctypes.byref(it), _opts(opts)))
@@ -370,6 +408,17 @@ def gemm_bf16(terms, alpha, A, B, C):
     pc, ldc, _ = _mat(C, ("float32",))
     _check(lib().slate_amd_sgemm_bf16(context(dev), int(terms), C.shape[0], _ncols(C), A.shape[1],
                                       alpha, pa, lda, pb, ldb, pc, ldc))
+
+
+def gemm_fp8(terms, alpha, A, B, C):
+    """C += alpha A B (float32, column-major) on the e4m3 MFMA kernel: A
+    quantized per 128-row tile, B per 128-column tile (power-of-two scales),
+    `terms` (1 or 2) planes per operand."""
+    pa, lda, dev = _mat(A, ("float32",))
+    pb, ldb, _ = _mat(B, ("float32",))
+    pc, ldc, _ = _mat(C, ("float32",))
+    _check(lib().slate_amd_sgemm_fp8(context(dev), int(terms), C.shape[0], _ncols(C), A.shape[1],
+                                     alpha, pa, lda, pb, ldb, pc, ldc))
 
 
 def profile_enable(on=True, device=0):
@@ -438,6 +487,7 @@ def gesv_auto(A, B, X, params=None, opts=None):
     pa, lda, dev = _mat(A, ("float64",))
     pb, _, _ = _mat(B, ("float64",))
     px, _, _ = _mat(X, ("float64",))
+    _rows(A, B, X)
     n = A.shape[0]
     ipiv = _ipiv_out(n)
     it, ch = ctypes.c_int(0), ctypes.c_int(-1)

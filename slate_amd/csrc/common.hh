@@ -153,6 +153,13 @@ __device__ inline T wave_sum(T v)
     return v;
 }
 
+template <class T>
+__device__ inline T wave_max(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // Map a linear block id onto tiles so that the round-robin XCD dispatch
 // (block b runs on XCD b % 8) hands each XCD a contiguous chunk of tiles.
 __device__ inline int64_t xcd_remap(int64_t bid, int64_t nblocks)

@@ -45,6 +45,7 @@ struct Context {
     hipEvent_t t_start = nullptr, t_stop = nullptr;
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
     Buf bfl, bfu0, bfu1, bfg;    // split-BF16 operand planes (LU update, gemm entry)
+    Buf f8l, f8u0, f8u1, f8s;    // FP8 operand planes (L panel slots, U blocks) and tile scales
     Buf npu, npt;                // no-pivoting panel: U11^{-1} and staging
     Buf bmeta, bwork;            // batched solves: per-system metadata, per-workgroup workspace
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
@@ -122,6 +123,17 @@ static int split_terms(int prec)
     }
 }
 
+// Number of e4m3 planes per operand of the float factor's trailing update
+// (0 = not an FP8 form).
+static int fp8_terms(int prec)
+{
+    switch (prec) {
+        case SLATE_AMD_PREC_FP8: return 1;
+        case SLATE_AMD_PREC_FP8X2: return 2;
+        default: return 0;
+    }
+}
+
 static Opts read_opts(const slate_amd_options* o)
 {
     Opts r;
@@ -138,6 +150,7 @@ static Opts read_opts(const slate_amd_options* o)
     r.depth = o->depth;
     SA_REQUIRE(r.method == SLATE_AMD_LU_PARTIALPIV || r.method == SLATE_AMD_LU_NOPIV
                || r.method == SLATE_AMD_LU_CALU, -2, "unknown method_lu");
+    SA_REQUIRE(r.prec >= SLATE_AMD_PREC_FP32 && r.prec <= SLATE_AMD_PREC_FP8X2, -2, "unknown factor_precision");
     return r;
 }
 
@@ -398,8 +411,11 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
                      int64_t* info)
 {
     const int kind = sizeof(T) == 8 ? SLATE_AMD_TUNE_LU_FP64
-                   : (split_terms(o.prec) > 0 ? SLATE_AMD_TUNE_LU_BF16 : SLATE_AMD_TUNE_LU_FP32);
+                   : (split_terms(o.prec) > 0 || fp8_terms(o.prec) > 0 ? SLATE_AMD_TUNE_LU_BF16
+                                                                      : SLATE_AMD_TUNE_LU_FP32);
     const int64_t nb = choose_nb(c, kind, n, o.nb);
+    const int f8 = std::is_same<T, float>::value ? fp8_terms(o.prec) : 0;
+    SA_REQUIRE(f8 == 0 || nb % 128 == 0, -2, "FP8 factor precision: block size must be a multiple of 128");
     c.calu = o.method == SLATE_AMD_LU_CALU;
     int64_t leaf_w = c.calu ? calu_leaf_width(int(sizeof(T))) : panel_leaf_width(int(sizeof(T)));
     if (o.ib > 0 && o.ib < leaf_w) leaf_w = o.ib;
@@ -471,6 +487,26 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
                 split_rows(ns, A + (k0 + kb) * ld + k0, ld, n - k0 - kb, kb, lslot(k), Kp, lplane, st);
         }
     };
+    // FP8 updates (factor_precision FP8, FP8X2): the L panel of each step is
+    // quantized once on s1 (two alternating slots, as the bf16 planes), each
+    // U row block right before its update; scales per 128-row tile.
+    const int64_t Kp8 = f8 ? fp8_kp(nb) : 0;
+    const int64_t p8l = f8 ? fp8_rows(n) * Kp8 : 0, p8u1 = f8 ? fp8_rows(nb) * Kp8 : 0;
+    const int64_t t8 = f8 ? fp8_tiles(n) : 0;
+    if (f8) {
+        c.f8l.ensure(size_t(2 * f8) * size_t(p8l));
+        c.f8u0.ensure(size_t(f8) * size_t(p8l));
+        c.f8u1.ensure(size_t(f8) * size_t(p8u1));
+        c.f8s.ensure(sizeof(int) * size_t(4 * t8 + 4));
+    }
+    auto l8slot = [&](int64_t k) { return c.f8l.as<uint8_t>() + (k % 2) * f8 * p8l; };
+    auto l8scale = [&](int64_t k) { return c.f8s.as<int>() + (k % 2) * t8; };
+    auto quant_panel = [&](int64_t k, int64_t k0, int64_t kb, hipStream_t st) {
+        if constexpr (std::is_same<T, float>::value) {
+            if (f8 && n - k0 - kb > 0)
+                quant_fp8_rows(f8, A + (k0 + kb) * ld + k0, ld, n - k0 - kb, kb, l8slot(k), Kp8, p8l, l8scale(k), st);
+        }
+    };
     // Update of columns [cb, cb + ncols) by step k (rows below the panel):
     // C -= L21 U12, U12 already multiplied by L11^{-1}.
     // role 1 (the trailing update on s0) is the one timed by the profiling
@@ -486,6 +522,15 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             if (role == 1) prof_end(c, st, flops);
         };
         if constexpr (std::is_same<T, float>::value) {
+            if (f8) {
+                uint8_t* u8 = lookahead_block ? c.f8u1.as<uint8_t>() : c.f8u0.as<uint8_t>();
+                const int64_t pu = lookahead_block ? p8u1 : p8l;
+                int* su = c.f8s.as<int>() + (lookahead_block ? 3 : 2) * t8;
+                quant_fp8_cols_t(f8, A + k0 * ld + cb, ld, kb, ncols, u8, Kp8, pu, su, st);
+                timed([&] { gemm_fp8(f8, n - c1, ncols, Kp8, l8slot(k), p8l, l8scale(k), u8, pu, su,
+                                     A + c1 * ld + cb, ld, -1.0f, st, role, 0); });
+                return;
+            }
             if (ns > 0 && direct) {
                 float* ut = lookahead_block ? c.bfu1.as<float>() : c.bfu0.as<float>();
                 transpose_f32(A + k0 * ld + cb, ld, kb, ncols, ut, Kp, st);
@@ -562,6 +607,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     factor_panel(0, 0, std::min(nb, n));
     unit_lower_inverse<T>(A, ld, std::min(nb, n), step_inv(0), s1);
     split_panel(0, 0, std::min(nb, n), s1);
+    quant_panel(0, 0, std::min(nb, n), s1);
     SA_HIP(hipEventRecord(ev_panel(0), s1));
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t k0 = k * nb, kb = std::min(nb, n - k0);
@@ -592,6 +638,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             factor_panel(k + 1, c1, nb1);
             unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             split_panel(k + 1, c1, nb1, s1);
+            quant_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
             // s0: step k on the columns beyond k+1
             const int64_t c2 = c1 + nb1;
@@ -612,6 +659,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
             factor_panel(k + 1, c1, nb1);
             unit_lower_inverse<T>(A + c1 * ld + c1, ld, nb1, step_inv(k + 1), s1);
             split_panel(k + 1, c1, nb1, s1);
+            quant_panel(k + 1, c1, nb1, s1);
             SA_HIP(hipEventRecord(ev_panel(k + 1), s1));
         }
     }
@@ -663,6 +711,17 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     const int ns = std::is_same<T, float>::value ? split_terms(o.prec) : 0;
     const int64_t Kp = ceildiv(nb, 32) * 32, plane = n * Kp;
     if (ns > 0) c.bfu0.ensure(std::max(sizeof(__bf16) * size_t(ns), sizeof(float)) * size_t(plane));
+    // FP8 forms: U12^T quantized once per step (scales per 128-row tile) is
+    // both operands of the triangular update.
+    const int f8 = std::is_same<T, float>::value ? fp8_terms(o.prec) : 0;
+    SA_REQUIRE(f8 == 0 || nb % 128 == 0, -2, "FP8 factor precision: block size must be a multiple of 128");
+    const int64_t Kp8 = f8 ? fp8_kp(nb) : 0, p8 = f8 ? fp8_rows(n) * Kp8 : 0;
+    if (f8) {
+        c.f8u0.ensure(size_t(f8) * size_t(p8));
+        c.f8u1.ensure(size_t(f8) * size_t(p8));
+        c.f8s.ensure(sizeof(int) * size_t(2 * fp8_tiles(n) + 4));
+    }
+    if (ns > 0) c.bfu1.ensure(sizeof(float) * size_t(plane));
     const int64_t nsteps = ceildiv(n, nb);
     // Lookahead as in the LU (src/potrf.cc:84-194 with Option::Lookahead): s1
     // applies step k to the next block row and factors it (diagonal block and
@@ -698,6 +757,13 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         const T* U = A + k0 * ld + r0;                 // kb x r, row-major
         T* C = A + r0 * ld + r0;
         if constexpr (std::is_same<T, float>::value) {
+            if (f8 && &wk == &c.bfu0) {
+                uint8_t* u8 = c.f8u0.as<uint8_t>();
+                int* su = c.f8s.as<int>();
+                quant_fp8_cols_t(f8, U, ld, kb, r, u8, Kp8, p8, su, st);
+                gemm_fp8(f8, r, r, Kp8, u8, p8, su, u8, p8, su, C, ld, -1.0f, st, 0, 1);
+                return;
+            }
             if (ns > 0 && &wk == &c.bfu0) {
                 float* ut = c.bfu0.as<float>();
                 transpose_f32(U, ld, kb, r, ut, Kp, st);
@@ -715,10 +781,29 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         g.alpha = -1.0; g.beta = 1.0; g.uplo = 1;
         gemm<T>(g, A_KC, B_NC, st);
     };
-    // block row [c1, c2) -= U12(:, c1:c2)^T U12(:, c1:n), upper part only
+    // block row [c1, c2) -= U12(:, c1:c2)^T U12(:, c1:n), upper part only.
+    // The low-precision forms use the same kernel and the same operand tiles
+    // (counted from c1) as the herk, so each entry is computed exactly as the
+    // update without lookahead computes it; one transposed / quantized copy of
+    // U12(:, c1:n) serves as both operands (its first w rows are the left one).
     auto block_row = [&](int64_t k0, int64_t kb, int64_t c1, int64_t c2, hipStream_t st) {
         const int64_t w = c2 - c1, r = n - c1;
         const T* U = A + k0 * ld + c1;
+        if constexpr (std::is_same<T, float>::value) {
+            if (f8) {
+                uint8_t* u8 = c.f8u1.as<uint8_t>();
+                int* su = c.f8s.as<int>() + fp8_tiles(n);
+                quant_fp8_cols_t(f8, U, ld, kb, r, u8, Kp8, p8, su, st);
+                gemm_fp8(f8, w, r, Kp8, u8, p8, su, u8, p8, su, A + c1 * ld + c1, ld, -1.0f, st, 0, 1);
+                return;
+            }
+            if (ns > 0) {
+                float* ut = c.bfu1.as<float>();
+                transpose_f32(U, ld, kb, r, ut, Kp, st);
+                gemm_s32(ns, w, r, kb, ut, Kp, ut, Kp, A + c1 * ld + c1, ld, -1.0f, st, 0, 1);
+                return;
+            }
+        }
         c.cut1.ensure(sizeof(T) * size_t(w) * size_t(kb));
         T* ut = c.cut1.as<T>();
         transpose_convert<T, T>(U, ld, ut, kb, w, kb, st);
@@ -728,7 +813,7 @@ static void potrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         g.alpha = -1.0; g.beta = 1.0; g.uplo = 1;
         gemm<T>(g, A_KC, B_NC, st);
     };
-    Buf& wk0 = ns > 0 ? c.bfu0 : c.cut0;
+    Buf& wk0 = ns > 0 || f8 > 0 ? c.bfu0 : c.cut0;
     if (!la) {
         for (int64_t k0 = 0; k0 < n; k0 += nb) {
             const int64_t kb = std::min(nb, n - k0);
@@ -1753,10 +1838,35 @@ static int gemm_bf16_entry(slate_amd_context ctx, int terms, int64_t m, int64_t 
     }));
 }
 
-namespace sa {
+int slate_amd_sgemm_fp8(slate_amd_context ctx, int terms, int64_t m, int64_t n, int64_t k, float alpha,
+                        const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(terms == 1 || terms == 2, -2, "terms must be 1 or 2");
+        SA_REQUIRE(m >= 0 && n >= 0 && k >= 0, -3, "bad m/n/k");
+        SA_REQUIRE(lda >= std::max<int64_t>(1, m) && ldb >= std::max<int64_t>(1, k)
+                   && ldc >= std::max<int64_t>(1, m), -3, "bad leading dimension");
+        if (m == 0 || n == 0 || k == 0) return 0;
+        SA_ENTRY_SYNC();
+        sa::Timer t(c);
+        // column-major C (m x n) is row-major C^T (n x m): C^T += alpha B^T A^T
+        // with B^T (n x k) k-contiguous in place and A^T from A (k x m row-major view)
+        const int64_t Kp = sa::fp8_kp(k);
+        const int64_t pb = sa::fp8_rows(n) * Kp, pa = sa::fp8_rows(m) * Kp;
+        c.bfg.ensure(size_t(terms) * size_t(pa + pb) + 256);
+        uint8_t* bq = c.bfg.as<uint8_t>();
+        uint8_t* aq = bq + terms * pb;
+        c.f8s.ensure(sizeof(int) * size_t(sa::fp8_tiles(n) + sa::fp8_tiles(m) + 4));
+        int* sb = c.f8s.as<int>();
+        int* sa_ = sb + sa::fp8_tiles(n);
+        sa::quant_fp8_rows(terms, B, ldb, n, k, bq, Kp, pb, sb, c.s0);
+        sa::quant_fp8_cols_t(terms, A, lda, k, m, aq, Kp, pa, sa_, c.s0);
+        sa::gemm_fp8(terms, n, m, Kp, bq, pb, sb, aq, pa, sa_, C, ldc, alpha, c.s0, 0, 0);
+        t.stop();
+        return 0;
+    }));
+}
 
-
-} // namespace sa
 
 namespace sa {
 
@@ -1882,11 +1992,12 @@ int slate_amd_finalize(slate_amd_context ctx)
     sa::Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
-    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
+    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.f8l, &c.f8u0, &c.f8u1, &c.f8s, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
                        &c.tvw, &c.swaptmp, &c.swaptmp1, &c.ttri, &c.cut0, &c.cut1})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
+    for (auto e : c.prof_ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c.t_start);
     (void)hipEventDestroy(c.t_stop);
     (void)hipEventDestroy(c.ev_in);

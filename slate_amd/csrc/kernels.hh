@@ -72,6 +72,22 @@ void gemm_s32(int ns, int64_t M, int64_t N, int64_t K, const float* L, int64_t l
               float* C, int64_t ldc, float alpha, hipStream_t s, int role = 0, int uplo = 0);
 void transpose_f32(const float* src, int64_t ld, int64_t K, int64_t N, float* dst, int64_t Kp, hipStream_t s);
 
+// FP8 e4m3 update (gemm_fp8.hip): operands quantized in tiles of 128 rows
+// (128 columns of a K x N block) over the whole k with a power-of-two scale
+// per tile (sexp[t] = s, largest with max|x| 2^s <= 448), nt = 1 or 2 planes
+// (the second the 2^4-scaled remainder) of fp8_rows(M) x Kp bytes each,
+// Kp = fp8_kp(K); gemm_fp8: C (fp32 row-major) += alpha * the scaled product.
+int64_t fp8_kp(int64_t K);
+int64_t fp8_rows(int64_t M);
+int64_t fp8_tiles(int64_t M);
+void quant_fp8_rows(int nt, const float* src, int64_t ld, int64_t M, int64_t K, uint8_t* dst, int64_t Kp,
+                    int64_t plane, int* sexp, hipStream_t s);
+void quant_fp8_cols_t(int nt, const float* src, int64_t ld, int64_t K, int64_t N, uint8_t* dst, int64_t Kp,
+                      int64_t plane, int* sexp, hipStream_t s);
+void gemm_fp8(int nt, int64_t M, int64_t N, int64_t Kp, const uint8_t* A, int64_t planeA, const int* sa,
+              const uint8_t* B, int64_t planeB, const int* sb, float* C, int64_t ldc, float alpha, hipStream_t s,
+              int role = 0, int uplo = 0);
+
 // Workspace shared by the cross-workgroup panel kernel.
 struct PanelWork {
     void* slots;                 // tagged candidate words exchanged between workgroups
