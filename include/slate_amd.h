@@ -62,6 +62,10 @@ typedef struct {
     int32_t factor_precision;  /* SLATE_AMD_PREC_* for the mixed drivers         */
     int32_t method_lu;         /* Option::MethodLU: SLATE_AMD_LU_* (default PP)  */
     int32_t depth;             /* Option::Depth: butterfly depth of gesv_rbt (2) */
+    int32_t use_graph;         /* replay the LU step sequence from a HIP graph   */
+                               /* recorded on the first call with the same       */
+                               /* matrix, pivots, sizes and options (default 0;  */
+                               /* no counterpart in the reference)               */
 } slate_amd_options;
 
 /* slate::MethodLU (include/slate/enums.hh:294-301) for getrf/gesv and the
@@ -168,6 +172,10 @@ int slate_amd_dgerbt_rhs(slate_amd_context ctx, int trans, int64_t n, int64_t nr
  * Closest reference counterpart: the slate::timers map (include/slate/slate.hh
  * :35-38) filled by the drivers' Timer blocks. */
 int slate_amd_profile_enable(slate_amd_context ctx, int on);
+/* Graphs recorded by use_graph (captured), keys whose capture failed and so
+ * run eagerly (failed), and replays so far.  No counterpart in the
+ * reference. */
+int slate_amd_graph_stats(slate_amd_context ctx, int64_t* captured, int64_t* failed, int64_t* replays);
 int slate_amd_profile_read(slate_amd_context ctx, double* gemm_ms, double* gemm_flops,
                            int64_t* launches);
 
@@ -241,14 +249,19 @@ int slate_amd_tune_set_nb(slate_amd_context ctx, int kind, int count, const int6
                           const int64_t* nb);
 
 /* ---- learned solver selection (no counterpart in the reference) ---------
- * For callers that want the factorization precision / refinement scheme
- * picked per system.  Features: 24 floats (log2 n, log10 of the infinity-norm
- * condition estimate from the float factor, per-tile max |a| spread, a
- * 16-bin exponent histogram, a constant).  The MLP (3 layers, 64 hidden,
- * FP32; slate_amd_mlp_num_params() floats) maps them to a choice:
- * 0 classic IR, 1 GMRES-IR, 2 plain FP64.  Training runs on the GPU with
- * slate_amd_mlp_train_step (softmax cross-entropy, Adam).  Device pointers
- * except where noted. */
+ * For callers that want the factorization precision and refinement scheme
+ * picked per system.  A probe factor in BF16X2 (the float LU with two-term
+ * split-BF16 updates) gives the features: 24 floats (log2 n, log10 of the
+ * infinity-norm condition estimate from the probe factor, per-tile max |a|
+ * spread, a 16-bin exponent histogram, a constant).  The MLP (3 layers, 64
+ * hidden, FP32; slate_amd_mlp_num_params() floats) maps them to one of
+ * SLATE_AMD_SELECTOR_CHOICES choices:
+ *   0 BF16X2 + classic IR   1 BF16X2 + GMRES-IR   (keep the probe factor)
+ *   2 BF16X3 + classic IR   3 BF16X3 + GMRES-IR   4 FP32 + classic IR
+ *   5 FP32 + GMRES-IR       6 FP8X2 + GMRES-IR    7 plain FP64 getrf + getrs
+ * Training runs on the GPU with slate_amd_mlp_train_step (softmax
+ * cross-entropy, Adam).  Device pointers except where noted. */
+enum { SLATE_AMD_SELECTOR_CHOICES = 8 };
 int slate_amd_mlp_num_params(void);
 int64_t slate_amd_selector_features(slate_amd_context ctx, int64_t n, const double* dA, int64_t lda,
                                     const slate_amd_options* opts, float* features /* host, 24 */);
@@ -258,12 +271,16 @@ int slate_amd_mlp_forward(slate_amd_context ctx, const float* dparams, const flo
  * then Adam (step >= 1) when lr > 0; *loss (host) <- mean loss. */
 int slate_amd_mlp_train_step(slate_amd_context ctx, float* dparams, float* dgrad, float* dm, float* dv,
                              const float* dX, const int* dy, int batch, float lr, int step, float* loss);
-/* gesv_mixed with the selector: params (host) the trained MLP, or NULL for
- * SLATE's default (classic IR); *choice the scheme used, *iter as in dsgesv,
- * features (host, 24, may be NULL) the inputs it saw.  nrhs = 1. */
+/* gesv_mixed with the selector (nrhs = 1).  params (host): the trained MLP;
+ * force >= 0: run that choice instead of the network (how the training labels
+ * are measured); params NULL and force < 0: SLATE's default, float factor +
+ * classic IR with no probe (*choice = 4).  *choice the choice made, *iter as
+ * in dsgesv, features (host, 24, may be NULL) the inputs it saw (zeros
+ * without a probe).  The call's device time (slate_amd_last_time_ms)
+ * includes the probe. */
 int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int64_t lda, int64_t* ipiv,
-                              const double* dB, double* dX, const float* params, int* iter, int* choice,
-                              float* features, const slate_amd_options* opts);
+                              const double* dB, double* dX, const float* params, int force, int* iter,
+                              int* choice, float* features, const slate_amd_options* opts);
 
 /* Test matrix with a prescribed spectrum: A = U diag(sigma) V^T, U and V
  * products of n Householder reflectors whose vectors are the columns of
@@ -274,6 +291,55 @@ int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int6
  * from the QR of a Gaussian matrix instead. */
 int slate_amd_generate_svd(slate_amd_context ctx, int sigma_dist, double cond, int64_t seed, int sym,
                            int64_t n, double* dA, int64_t lda);
+
+/* ---- SLATE tile storage -------------------------------------------------
+ * A matrix as SLATE's MatrixStorage holds it after insertLocalTiles(
+ * Target::Devices) on one device (include/slate/internal/MatrixStorage.hh:151
+ * tile map, include/slate/Matrix.hh:164 insertLocalTiles,
+ * include/slate/BaseMatrix.hh:2566-2646 tileGet): mt x nt tiles of mb x nb,
+ * mt = ceil(m/mb), nt = ceil(n/nb), the last row / column of tiles ragged;
+ * tile (i, j) is an independent column-major DEVICE buffer tiles[i + j*mt]
+ * with leading dimension lds[i + j*mt] >= its row count.  For the Hermitian
+ * (Cholesky) entries only tiles with i >= j are read (lower storage, as
+ * SLATE's HermitianMatrix keeps); the others may be NULL.  The two arrays are
+ * host memory.  Each entry gathers the tiles into one dense workspace of the
+ * context, runs the dense entry of the same name, and writes the outputs back
+ * into the tiles (A: factors; B: solution; X: solution; for the mixed drivers
+ * A only when the FP64 fallback ran).  Cost: one read and one write of each
+ * tile on the way in and out (about 2 x 34 GB of HBM traffic for A at
+ * n = 65536), and an n x n FP64 workspace. */
+typedef struct {
+    int64_t m, n;              /* matrix size                                  */
+    int64_t mb, nb;            /* tile size (uniform; last tiles ragged)      */
+    void* const* tiles;        /* mt * nt device pointers, tile (i,j) at i+j*mt */
+    const int64_t* lds;        /* mt * nt leading dimensions                  */
+} slate_amd_tiles;
+
+/* slate::getrf / getrs / gesv on tiles (slate.hh:574-577, :603-607, :504-508). */
+int64_t slate_amd_dgetrf_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                               const slate_amd_options* opts);
+int64_t slate_amd_dgetrs_tiles(slate_amd_context ctx, int trans, const slate_amd_tiles* A, const int64_t* ipiv,
+                               const slate_amd_tiles* B, const slate_amd_options* opts);
+int64_t slate_amd_dgesv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                              const slate_amd_tiles* B, const slate_amd_options* opts);
+/* slate::potrf / potrs / posv on lower tiles (slate.hh:714-717, :741-745, :646-650). */
+int64_t slate_amd_dpotrf_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_options* opts);
+int64_t slate_amd_dpotrs_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                               const slate_amd_options* opts);
+int64_t slate_amd_dposv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                              const slate_amd_options* opts);
+/* slate::gesv_mixed / gesv_mixed_gmres / posv_mixed / posv_mixed_gmres on
+ * tiles (slate.hh:521-553, :667-701). */
+int64_t slate_amd_dsgesv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                               const slate_amd_tiles* B, const slate_amd_tiles* X, int* iter,
+                               const slate_amd_options* opts);
+int64_t slate_amd_dsgesv_gmres_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                                     const slate_amd_tiles* B, const slate_amd_tiles* X, int* iter,
+                                     const slate_amd_options* opts);
+int64_t slate_amd_dsposv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                               const slate_amd_tiles* X, int* iter, const slate_amd_options* opts);
+int64_t slate_amd_dsposv_gmres_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                                     const slate_amd_tiles* X, int* iter, const slate_amd_options* opts);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,7 @@ enum class MethodLU : char { PartialPiv = 'P', CALU = 'C', NoPiv = 'N' };   // i
 enum class Option : char {
     Lookahead, InnerBlocking, MaxPanelThreads, PivotThreshold, Tolerance, MaxIterations,
     UseFallbackSolver, Target, HoldLocalWorkspace, FactorPrecision, MethodLU, Depth,
+    UseGraph,   // replay the LU step sequence from a recorded HIP graph (no counterpart in slate::Option)
 };
 
 class OptionValue {
@@ -330,6 +331,7 @@ inline slate_amd_options to_c(const Options& o, int64_t nb)
     c.method_lu = m == MethodLU::NoPiv ? SLATE_AMD_LU_NOPIV
                 : m == MethodLU::CALU ? SLATE_AMD_LU_CALU : SLATE_AMD_LU_PARTIALPIV;
     c.depth = int32_t(opt_i(o, Option::Depth, 2));
+    c.use_graph = int32_t(opt_i(o, Option::UseGraph, 0));
     return c;
 }
 

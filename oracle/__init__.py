@@ -364,7 +364,7 @@ def generate_svd(sigma_dist, cond, seed, n, sym=False):
 
 
 # ---- selector MLP (float64 restatement of slate_amd/csrc/mlp.hip) ----------
-MLP_F, MLP_H, MLP_K = 24, 64, 3
+MLP_F, MLP_H, MLP_K = 24, 64, 8
 
 
 def mlp_unpack(P):

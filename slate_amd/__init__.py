@@ -42,12 +42,23 @@ C_SYMBOLS = [
     "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb", "slate_amd_dgesv_batched",
     "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_sgemm_fp8", "slate_amd_profile_enable",
     "slate_amd_profile_read", "slate_amd_dgesv_rbt", "slate_amd_drbt_generate", "slate_amd_dgerbt",
-    "slate_amd_dgerbt_rhs",
+    "slate_amd_dgerbt_rhs", "slate_amd_dgetrf_tiles", "slate_amd_dgetrs_tiles", "slate_amd_dgesv_tiles",
+    "slate_amd_dpotrf_tiles", "slate_amd_dpotrs_tiles", "slate_amd_dposv_tiles", "slate_amd_dsgesv_tiles",
+    "slate_amd_dsgesv_gmres_tiles", "slate_amd_dsposv_tiles", "slate_amd_dsposv_gmres_tiles",
+    "slate_amd_graph_stats",
 ]
 
 
 class SlateError(RuntimeError):
     pass
+
+
+class TileDesc(ctypes.Structure):
+    """Mirror of ``slate_amd_tiles`` (SLATE tile storage on one device)."""
+    _fields_ = [
+        ("m", ctypes.c_int64), ("n", ctypes.c_int64), ("mb", ctypes.c_int64), ("nb", ctypes.c_int64),
+        ("tiles", ctypes.POINTER(ctypes.c_void_p)), ("lds", ctypes.POINTER(ctypes.c_int64)),
+    ]
 
 
 class Options(ctypes.Structure):
@@ -63,6 +74,7 @@ class Options(ctypes.Structure):
         ("factor_precision", ctypes.c_int32),
         ("method_lu", ctypes.c_int32),
         ("depth", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
     ]
 
     def __init__(self, **kw):
@@ -89,6 +101,7 @@ def lib():
         pi64 = ctypes.POINTER(ctypes.c_int64)
         pint = ctypes.POINTER(ctypes.c_int)
         po = ctypes.POINTER(Options)
+        ptd = ctypes.POINTER(TileDesc)
         sig = {
             "slate_amd_init": (i32, [ctypes.POINTER(vp), i32]),
             "slate_amd_finalize": (i32, [vp]),
@@ -124,8 +137,19 @@ def lib():
             "slate_amd_mlp_forward": (i32, [vp, vp, vp, i32, vp]),
             "slate_amd_mlp_train_step": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_float, i32,
                                                ctypes.POINTER(ctypes.c_float)]),
-            "slate_amd_dsgesv_auto": (i64, [vp, i64, vp, i64, pi64, vp, vp, vp, pint, pint, vp, po]),
+            "slate_amd_dsgesv_auto": (i64, [vp, i64, vp, i64, pi64, vp, vp, vp, i32, pint, pint, vp, po]),
+            "slate_amd_dgetrf_tiles": (i64, [vp, ptd, pi64, po]),
+            "slate_amd_dgetrs_tiles": (i64, [vp, i32, ptd, pi64, ptd, po]),
+            "slate_amd_dgesv_tiles": (i64, [vp, ptd, pi64, ptd, po]),
+            "slate_amd_dpotrf_tiles": (i64, [vp, ptd, po]),
+            "slate_amd_dpotrs_tiles": (i64, [vp, ptd, ptd, po]),
+            "slate_amd_dposv_tiles": (i64, [vp, ptd, ptd, po]),
+            "slate_amd_dsgesv_tiles": (i64, [vp, ptd, pi64, ptd, ptd, pint, po]),
+            "slate_amd_dsgesv_gmres_tiles": (i64, [vp, ptd, pi64, ptd, ptd, pint, po]),
+            "slate_amd_dsposv_tiles": (i64, [vp, ptd, ptd, ptd, pint, po]),
+            "slate_amd_dsposv_gmres_tiles": (i64, [vp, ptd, ptd, ptd, pint, po]),
             "slate_amd_profile_enable": (i32, [vp, i32]),
+            "slate_amd_graph_stats": (i32, [vp, pi64, pi64, pi64]),
             "slate_amd_profile_read": (i32, [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl), pi64]),
             "slate_amd_dgemm": (i32, [vp, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
             "slate_amd_sgemm": (i32, [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
@@ -426,6 +450,13 @@ def profile_enable(on=True, device=0):
     _check(lib().slate_amd_profile_enable(context(device), int(on)))
 
 
+def graph_stats(device=0):
+    """(captured, failed, replays) of the use_graph LU sequences."""
+    a, b, r = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    _check(lib().slate_amd_graph_stats(context(device), ctypes.byref(a), ctypes.byref(b), ctypes.byref(r)))
+    return a.value, b.value, r.value
+
+
 def profile_read(device=0):
     """(gemm_ms, gemm_flops, launches) accumulated since profile_enable."""
     ms, fl, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int64(0)
@@ -443,7 +474,14 @@ def norm_inf(A, sym=False):
 
 # ---- learned selection ------------------------------------------------------
 SELECTOR_FEATURES = 24
-CHOICES = ("classic IR", "GMRES-IR", "FP64")
+# (factor precision, refinement) of each selector choice; the probe factor
+# (BF16X2) is kept by choices 0 and 1
+CHOICES = ("BF16X2 + IR", "BF16X2 + GMRES-IR", "BF16X3 + IR", "BF16X3 + GMRES-IR", "FP32 + IR",
+           "FP32 + GMRES-IR", "FP8X2 + GMRES-IR", "FP64")
+CHOICE_PREC = (PREC_BF16X2, PREC_BF16X2, PREC_BF16X3, PREC_BF16X3, PREC_FP32, PREC_FP32, PREC_FP8X2, PREC_FP64)
+CHOICE_GMRES = (False, True, False, True, False, True, True, False)
+SELECTOR_PROBE = PREC_BF16X2
+DEFAULT_CHOICE = 4            # SLATE's gesv_mixed: float factor + classic IR
 
 
 def mlp_num_params():
@@ -480,9 +518,11 @@ def mlp_train_step(params, grad, m, v, X, y, lr, step):
     return loss.value
 
 
-def gesv_auto(A, B, X, params=None, opts=None):
-    """gesv_mixed with the learned selector (params: numpy float32 MLP
-    parameters, or None for the default classic IR).  Returns
+def gesv_auto(A, B, X, params=None, opts=None, force=None):
+    """gesv_mixed with the learned selector: params (numpy float32 MLP
+    parameters) picks one of CHOICES from the probe factor's features; force
+    (an index into CHOICES) runs that choice instead; neither: SLATE's default
+    (float factor + classic IR, no probe).  Returns
     (info, iter, choice, ipiv, features)."""
     pa, lda, dev = _mat(A, ("float64",))
     pb, _, _ = _mat(B, ("float64",))
@@ -495,10 +535,13 @@ def gesv_auto(A, B, X, params=None, opts=None):
     pp = None
     if params is not None:
         params = np.ascontiguousarray(params, dtype=np.float32)
+        if params.size != mlp_num_params():
+            raise SlateError("selector parameters: expected %d floats" % mlp_num_params())
         pp = params.ctypes.data_as(ctypes.c_void_p)
     info = _check(lib().slate_amd_dsgesv_auto(context(dev), n, pa, lda, _p64(ipiv), pb, px, pp,
-                                              ctypes.byref(it), ctypes.byref(ch),
-                                              feat.ctypes.data_as(ctypes.c_void_p), _opts(opts)))
+                                              -1 if force is None else int(force), ctypes.byref(it),
+                                              ctypes.byref(ch), feat.ctypes.data_as(ctypes.c_void_p),
+                                              _opts(opts)))
     return info, it.value, ch.value, ipiv[:n], feat
 
 
@@ -560,3 +603,104 @@ def generate(A, dist, seed, ioffset=0, joffset=0):
     n = 1 if A.dim() == 1 else A.shape[1]
     _check(lib().slate_amd_generate(context(dev), dist, seed, m, n, ioffset, joffset, pa, lda))
     return A
+
+
+#---------------------------------------------------------------- SLATE tile storage
+class TiledMatrix:
+    """A matrix held the way slate::Matrix::insertLocalTiles(Target::Devices)
+    holds it on one device: every mb x nb tile its own column-major CUDA
+    buffer with its own leading dimension (include/slate/internal/
+    MatrixStorage.hh:151).  lower=True stores only tiles (i, j) with i >= j
+    (a HermitianMatrix).  pad adds rows of padding to every tile's ld."""
+
+    def __init__(self, m, n, mb, nb, lower=False, pad=0, device="cuda"):
+        import torch
+        self.m, self.n, self.mb, self.nb, self.lower = m, n, mb, nb, lower
+        self.mt, self.nt = -(-m // mb), -(-n // nb)
+        self.tiles = {}
+        for j in range(self.nt):
+            for i in range(self.mt):
+                if lower and i < j:
+                    continue
+                r = min(mb, m - i * mb)
+                c = min(nb, n - j * nb)
+                self.tiles[(i, j)] = torch.zeros((c, r + pad), dtype=torch.float64, device=device).t()[:r]
+        cnt = self.mt * self.nt
+        self._ptr = (ctypes.c_void_p * cnt)()
+        self._ld = (ctypes.c_int64 * cnt)()
+        for (i, j), t in self.tiles.items():
+            self._ptr[i + j * self.mt] = t.data_ptr()
+            self._ld[i + j * self.mt] = t.stride(1)
+        for k in range(cnt):
+            if self._ld[k] == 0:
+                self._ld[k] = 1
+        self.desc = TileDesc(m, n, mb, nb, ctypes.cast(self._ptr, ctypes.POINTER(ctypes.c_void_p)),
+                             ctypes.cast(self._ld, ctypes.POINTER(ctypes.c_int64)))
+
+    @classmethod
+    def from_dense(cls, M, mb, nb, lower=False, pad=0):
+        T = cls(M.shape[0], 1 if M.dim() == 1 else M.shape[1], mb, nb, lower, pad, M.device)
+        Md = M if M.dim() == 2 else M.reshape(-1, 1)
+        for (i, j), t in T.tiles.items():
+            t.copy_(Md[i * mb:i * mb + t.shape[0], j * nb:j * nb + t.shape[1]])
+        return T
+
+    def to_dense(self):
+        import torch
+        dev = next(iter(self.tiles.values())).device
+        M = empty_colmajor(self.m, self.n, torch.float64, device=dev)
+        M.zero_()
+        for (i, j), t in self.tiles.items():
+            M[i * self.mb:i * self.mb + t.shape[0], j * self.nb:j * self.nb + t.shape[1]] = t
+        return M
+
+    def ref(self):
+        return ctypes.byref(self.desc)
+
+    def device(self):
+        return next(iter(self.tiles.values())).device.index or 0
+
+
+def getrf_tiles(A, opts=None):
+    ipiv = _ipiv_out(A.n)
+    info = _check(lib().slate_amd_dgetrf_tiles(context(A.device()), A.ref(), _p64(ipiv), _opts(opts)))
+    return info, ipiv[:A.n]
+
+
+def getrs_tiles(A, ipiv, B, trans=False, opts=None):
+    ip = np.ascontiguousarray(ipiv, dtype=np.int64)
+    _check(lib().slate_amd_dgetrs_tiles(context(A.device()), TRANS if trans else NOTRANS, A.ref(), _p64(ip),
+                                        B.ref(), _opts(opts)))
+
+
+def gesv_tiles(A, B, opts=None):
+    ipiv = _ipiv_out(A.n)
+    info = _check(lib().slate_amd_dgesv_tiles(context(A.device()), A.ref(), _p64(ipiv), B.ref(), _opts(opts)))
+    return info, ipiv[:A.n]
+
+
+def potrf_tiles(A, opts=None):
+    return _check(lib().slate_amd_dpotrf_tiles(context(A.device()), A.ref(), _opts(opts)))
+
+
+def potrs_tiles(A, B, opts=None):
+    _check(lib().slate_amd_dpotrs_tiles(context(A.device()), A.ref(), B.ref(), _opts(opts)))
+
+
+def posv_tiles(A, B, opts=None):
+    return _check(lib().slate_amd_dposv_tiles(context(A.device()), A.ref(), B.ref(), _opts(opts)))
+
+
+def gesv_mixed_tiles(A, B, X, gmres=False, opts=None):
+    ipiv = _ipiv_out(A.n)
+    it = ctypes.c_int(0)
+    f = lib().slate_amd_dsgesv_gmres_tiles if gmres else lib().slate_amd_dsgesv_tiles
+    info = _check(f(context(A.device()), A.ref(), _p64(ipiv), B.ref(), X.ref(), ctypes.byref(it), _opts(opts)))
+    return info, it.value, ipiv[:A.n]
+
+
+def posv_mixed_tiles(A, B, X, gmres=False, opts=None):
+    it = ctypes.c_int(0)
+    f = lib().slate_amd_dsposv_gmres_tiles if gmres else lib().slate_amd_dsposv_tiles
+    info = _check(f(context(A.device()), A.ref(), B.ref(), X.ref(), ctypes.byref(it), _opts(opts)))
+    return info, it.value

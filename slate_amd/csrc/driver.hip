@@ -21,6 +21,10 @@
 namespace sa {
 
 //------------------------------------------------------------------------------
+// Workspace generation: bumped whenever any workspace is (re)allocated, so a
+// captured graph (which holds workspace addresses) knows it went stale.
+static uint64_t g_buf_generation = 1;
+
 struct Buf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -28,11 +32,30 @@ struct Buf {
         if (b <= bytes) return;
         if (p) SA_HIP(hipFree(p));
         p = nullptr; bytes = 0;
+        ++g_buf_generation;
         SA_HIP(hipMalloc(&p, b));
         bytes = b;
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    void release() { if (p) { (void)hipFree(p); ++g_buf_generation; } p = nullptr; bytes = 0; }
     template <class T> T* as() { return static_cast<T*>(p); }
+};
+
+// Identity of one captured factorization: the kernels' arguments are fixed
+// by these (matrix and pivot addresses, sizes, options, workspace
+// generation).
+struct GraphKey {
+    int kind = 0;                // 0 LU float, 1 LU double
+    const void* A = nullptr;
+    const void* ipiv = nullptr;
+    int64_t n = 0, ld = 0, nb = 0, ib = 0, lookahead = 0;
+    double threshold = 0;
+    int prec = 0, method = 0;
+    uint64_t generation = 0;
+    bool operator==(const GraphKey& o) const {
+        return kind == o.kind && A == o.A && ipiv == o.ipiv && n == o.n && ld == o.ld && nb == o.nb && ib == o.ib
+               && lookahead == o.lookahead && threshold == o.threshold && prec == o.prec && method == o.method
+               && generation == o.generation;
+    }
 };
 
 struct Context {
@@ -46,11 +69,16 @@ struct Context {
     std::vector<hipEvent_t> evs;       // step events for the two-stream schedule
     Buf bfl, bfu0, bfu1, bfg;    // split-BF16 operand planes (LU update, gemm entry)
     Buf f8l, f8u0, f8u1, f8s;    // FP8 operand planes (L panel slots, U blocks) and tile scales
+    Buf tlA, tlB, tlX, tlP;      // tile-storage entries: dense copies of A, B, X and the tile tables
     Buf npu, npt;                // no-pivoting panel: U11^{-1} and staging
     Buf bmeta, bwork;            // batched solves: per-system metadata, per-workgroup workspace
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
+    bool capturing = false;      // a factorization is being recorded into a graph
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;   // captured factorizations
+    std::vector<GraphKey> graph_failed;                         // keys whose capture failed
+    int64_t graph_replays = 0;
     Buf tvw;                     // trsv look-back exchange words (+ progress word)
     Buf swaptmp;                 // two-pass row interchange workspace (trailing-update stream)
     Buf swaptmp1;                // the same for the lookahead block (tall blocks only)
@@ -109,6 +137,7 @@ struct Opts {
     int prec = SLATE_AMD_PREC_FP32;
     int method = SLATE_AMD_LU_PARTIALPIV;
     int depth = 2;
+    bool graph = false;
 };
 
 // Number of bf16 terms per operand of the float LU's trailing update
@@ -148,6 +177,7 @@ static Opts read_opts(const slate_amd_options* o)
     r.prec = o->factor_precision;
     r.method = o->method_lu;
     r.depth = o->depth;
+    r.graph = o->use_graph != 0;
     SA_REQUIRE(r.method == SLATE_AMD_LU_PARTIALPIV || r.method == SLATE_AMD_LU_NOPIV
                || r.method == SLATE_AMD_LU_CALU, -2, "unknown method_lu");
     SA_REQUIRE(r.prec >= SLATE_AMD_PREC_FP32 && r.prec <= SLATE_AMD_PREC_FP8X2, -2, "unknown factor_precision");
@@ -179,12 +209,14 @@ static void prepare_panel_work(Context& c, int64_t n, int64_t W)
     const bool fresh = c.slots.bytes < bytes;
     if (fresh) c.slots.release();
     c.slots.ensure(bytes);
-    if (fresh || c.leaf_tag > 0xF0000000u - uint32_t(std::min<int64_t>(n, 1 << 26))) {
+    // a captured factorization clears the words and restarts the tags itself,
+    // so every replay finds no stale tag
+    if (c.capturing || fresh || c.leaf_tag > 0xF0000000u - uint32_t(std::min<int64_t>(n, 1 << 26))) {
         zero_u32(static_cast<unsigned*>(c.slots.p), int64_t(c.slots.bytes / 4), c.s0);
         c.leaf_tag = 1;
     }
     c.err.ensure(sizeof(int) * 4);
-    SA_HIP(hipMemsetAsync(c.err.p, 0, sizeof(int) * 4, c.s0));
+    zero_u32(c.err.as<unsigned>(), 4, c.s0);
 }
 
 static PanelWork panel_work(Context& c)
@@ -1018,7 +1050,61 @@ static int64_t read_info(Context& c)
 static void reset_info(Context& c)
 {
     c.dinfo.ensure(sizeof(int64_t) * 8);
-    SA_HIP(hipMemsetAsync(c.dinfo.p, 0, sizeof(int64_t) * 8, c.s0));
+    zero_u32(c.dinfo.as<unsigned>(), 16, c.s0);
+}
+
+// Run body() (a factorization issued on s0, forking to s1/s2 through events
+// and joining back) through a HIP graph when o.graph is set: the first call
+// with a given key runs eagerly (sizing every workspace) and then records the
+// same launches into a graph; later calls with the same key replay it.  No
+// memset, allocation or host synchronisation happens inside body() once the
+// workspaces are sized; the panel exchange words are cleared inside the
+// recorded sequence and its tags restart at 1 (the tag counter is restored
+// afterwards, above every tag the graph writes).  A capture that fails leaves
+// that key to the eager path.
+template <class F>
+static void run_graph(Context& c, const Opts& o, const GraphKey& key0, F&& body)
+{
+    if (!o.graph || c.prof) { body(); return; }
+    GraphKey key = key0;
+    key.generation = g_buf_generation;
+    for (auto& g : c.graphs)
+        if (g.first == key) { SA_HIP(hipGraphLaunch(g.second, c.s0)); ++c.graph_replays; return; }
+    for (auto& k : c.graph_failed) if (k == key) { body(); return; }
+    body();                                   // eager: the result, and every workspace sized
+    SA_HIP(hipStreamSynchronize(c.s0));
+    key.generation = g_buf_generation;
+    const uint32_t saved_tag = c.leaf_tag;
+    hipGraph_t graph = nullptr;
+    c.capturing = true;
+    try {
+        SA_HIP(hipStreamBeginCapture(c.s0, hipStreamCaptureModeThreadLocal));
+        try { body(); }
+        catch (...) { (void)hipStreamEndCapture(c.s0, &graph); throw; }
+        SA_HIP(hipStreamEndCapture(c.s0, &graph));
+        c.capturing = false;
+        SA_REQUIRE(key.generation == g_buf_generation, -2, "workspace changed during capture");
+        hipGraphExec_t exec = nullptr;
+        SA_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        c.graphs.push_back({key, exec});
+    }
+    catch (...) {
+        c.graph_failed.push_back(key);
+        (void)hipGetLastError();
+    }
+    c.capturing = false;
+    if (graph) (void)hipGraphDestroy(graph);
+    c.leaf_tag = std::max(saved_tag, c.leaf_tag);
+}
+
+template <class T>
+static GraphKey lu_key(const T* A, const int64_t* ipiv, int64_t n, int64_t ld, const Opts& o)
+{
+    GraphKey k;
+    k.kind = sizeof(T) == 8 ? 1 : 0;
+    k.A = A; k.ipiv = ipiv; k.n = n; k.ld = ld; k.nb = o.nb; k.ib = o.ib; k.lookahead = o.lookahead;
+    k.threshold = o.threshold; k.prec = o.prec; k.method = o.method;
+    return k;
 }
 
 // In-place getrf of a column-major n x n matrix.
@@ -1027,7 +1113,7 @@ static int64_t getrf_cm(Context& c, T* A, int64_t n, int64_t lda, const Opts& o,
 {
     reset_info(c);
     transpose_inplace<T>(A, n, lda, c.s0);
-    getrf_rm<T>(c, A, n, lda, o, d_ipiv, c.dinfo.as<int64_t>());
+    run_graph(c, o, lu_key<T>(A, d_ipiv, n, lda, o), [&] { getrf_rm<T>(c, A, n, lda, o, d_ipiv, c.dinfo.as<int64_t>()); });
     transpose_inplace<T>(A, n, lda, c.s0);
     SA_HIP(hipStreamSynchronize(c.s0));
     check_kernel_error(c);
@@ -1245,7 +1331,8 @@ static int64_t factor_lo(Context& c, MixedState& st, const double* A, int64_t ld
         potrf_rm<float>(c, st.Alo, n, n, o, c.dinfo.as<int64_t>());
     }
     else {
-        getrf_rm<float>(c, st.Alo, n, n, o, st.ipiv, c.dinfo.as<int64_t>());
+        run_graph(c, o, lu_key<float>(st.Alo, st.ipiv, n, n, o),
+                  [&] { getrf_rm<float>(c, st.Alo, n, n, o, st.ipiv, c.dinfo.as<int64_t>()); });
     }
     SA_HIP(hipStreamSynchronize(c.s0));
     if (read_info(c) == 0) {
@@ -1703,37 +1790,66 @@ static void selector_features(Context& c, MixedState& st, int64_t n, const doubl
     f[20] = 1.0f;                                   // bias-like constant input
 }
 
-// Solve with the float factor chosen by the selector (src/gesv_mixed.cc and
-// src/gesv_mixed_gmres.cc flows): choice 0 classic IR, 1 GMRES-IR, 2 plain
-// FP64 getrf + getrs.  params: the MLP parameters (host, mlp_num_params()
-// floats) or null for SLATE's default (classic IR).  A refinement that misses
+// The selector's choices: (factor precision, scheme).  The features come from
+// a probe factor in SEL_PROBE precision, which choices 0 and 1 keep; the
+// others factor again in their own precision (FP8X2 is cheaper but only pays
+// when the probe is not kept, which the labels decide).  Choice 7 is the plain
+// FP64 getrf + getrs.
+struct SelChoice { int prec; int gmres; };
+static constexpr int SEL_K = 8;
+static constexpr int SEL_PROBE = SLATE_AMD_PREC_BF16X2;
+static constexpr SelChoice SEL_CHOICES[SEL_K] = {
+    {SLATE_AMD_PREC_BF16X2, 0}, {SLATE_AMD_PREC_BF16X2, 1}, {SLATE_AMD_PREC_BF16X3, 0},
+    {SLATE_AMD_PREC_BF16X3, 1}, {SLATE_AMD_PREC_FP32, 0},   {SLATE_AMD_PREC_FP32, 1},
+    {SLATE_AMD_PREC_FP8X2, 1},  {SLATE_AMD_PREC_FP64, 0}};
+
+// gesv_mixed with the learned selector.  params (host, mlp_num_params()
+// floats): the trained MLP; force >= 0 runs that choice instead of the
+// network (the labels are the measured cost of each forced choice); with
+// neither, SLATE's default runs (float factor + classic IR,
+// src/gesv_mixed.cc) without a probe, *choice = 4.  A refinement that misses
 // the tolerance falls back to FP64 as in the reference.
-static int64_t mixed_auto(Context& c, int64_t n, double* A, int64_t lda, int64_t* ipiv_host, const double* B,
-                          double* X, int* iter, int* choice, const float* params, float* feat_out, const Opts& o)
+static int64_t mixed_select(Context& c, int64_t n, double* A, int64_t lda, int64_t* ipiv_host, const double* B,
+                            double* X, int* iter, int* choice, const float* params, int force, float* feat_out,
+                            const Opts& o)
 {
+    SA_REQUIRE(force < SEL_K, -2, "selector: choice out of range");
+    if (!params && force < 0) {
+        Opts od = o;
+        od.prec = SLATE_AMD_PREC_FP32;
+        *choice = 4;
+        if (feat_out) std::fill(feat_out, feat_out + SEL_F, 0.f);
+        return mixed_ir(c, 0, SLATE_AMD_LOWER, n, 1, A, lda, ipiv_host, B, n, X, n, iter, od);
+    }
+    Opts op = o;
+    op.prec = SEL_PROBE;
     MixedState st = mixed_setup(c, n, 1);
     int64_t finfo = 0;
     float f[SEL_F];
-    selector_features(c, st, n, A, lda, o, &finfo, f);
+    selector_features(c, st, n, A, lda, op, &finfo, f);
     if (feat_out) std::copy(f, f + SEL_F, feat_out);
-    int ch = 0;
-    if (params && finfo == 0) {
-        const int np = mlp_num_params();
-        c.tinv.ensure(sizeof(float) * size_t(np + SEL_F + 16));
-        float* dp = c.tinv.as<float>();
-        float* dx = dp + np;
-        float* dprob = dx + SEL_F;
-        SA_HIP(hipMemcpyAsync(dp, params, sizeof(float) * np, hipMemcpyHostToDevice, c.s0));
-        SA_HIP(hipMemcpyAsync(dx, f, sizeof(float) * SEL_F, hipMemcpyHostToDevice, c.s0));
-        mlp_forward(dp, dx, 1, dprob, c.s0);
-        float pr[3];
-        SA_HIP(hipMemcpyAsync(pr, dprob, sizeof(pr), hipMemcpyDeviceToHost, c.s0));
-        SA_HIP(hipStreamSynchronize(c.s0));
-        ch = 0;
-        for (int k = 1; k < 3; ++k) if (pr[k] > pr[ch]) ch = k;
+    int ch = force;
+    if (ch < 0) {
+        ch = SEL_K - 1;                 // a singular probe factor: straight to FP64
+        if (finfo == 0) {
+            const int np = mlp_num_params();
+            c.tinv.ensure(sizeof(float) * size_t(np + SEL_F + 16));
+            float* dp = c.tinv.as<float>();
+            float* dx = dp + np;
+            float* dprob = dx + SEL_F;
+            SA_HIP(hipMemcpyAsync(dp, params, sizeof(float) * np, hipMemcpyHostToDevice, c.s0));
+            SA_HIP(hipMemcpyAsync(dx, f, sizeof(float) * SEL_F, hipMemcpyHostToDevice, c.s0));
+            mlp_forward(dp, dx, 1, dprob, c.s0);
+            float pr[SEL_K];
+            SA_HIP(hipMemcpyAsync(pr, dprob, sizeof(pr), hipMemcpyDeviceToHost, c.s0));
+            SA_HIP(hipStreamSynchronize(c.s0));
+            ch = 0;
+            for (int k = 1; k < SEL_K; ++k) if (pr[k] > pr[ch]) ch = k;
+        }
     }
     *choice = ch;
-    if (ch == 2) {
+    const SelChoice sc = SEL_CHOICES[ch];
+    if (sc.prec == SLATE_AMD_PREC_FP64) {
         *iter = 0;
         c.ipiv.ensure(sizeof(int64_t) * n);
         int64_t* dip = c.ipiv.as<int64_t>();
@@ -1746,8 +1862,14 @@ static int64_t mixed_auto(Context& c, int64_t n, double* A, int64_t lda, int64_t
         SA_HIP(hipStreamSynchronize(c.s0));
         return info;
     }
-    if (ch == 1) return mixed_gmres(c, 0, SLATE_AMD_LOWER, n, A, lda, ipiv_host, B, X, iter, o, o.nb, &st, finfo);
-    return mixed_ir(c, 0, SLATE_AMD_LOWER, n, 1, A, lda, ipiv_host, B, n, X, n, iter, o, &st, finfo);
+    Opts oc = o;
+    oc.prec = sc.prec;
+    const bool keep = sc.prec == SEL_PROBE;       // the probe factor is this choice's factor
+    if (sc.gmres)
+        return mixed_gmres(c, 0, SLATE_AMD_LOWER, n, A, lda, ipiv_host, B, X, iter, oc, o.nb,
+                           keep ? &st : nullptr, keep ? finfo : 0);
+    return mixed_ir(c, 0, SLATE_AMD_LOWER, n, 1, A, lda, ipiv_host, B, n, X, n, iter, oc,
+                    keep ? &st : nullptr, keep ? finfo : 0);
 }
 
 } // namespace sa
@@ -1992,12 +2114,13 @@ int slate_amd_finalize(slate_amd_context ctx)
     sa::Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     (void)hipDeviceSynchronize();
-    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.f8l, &c.f8u0, &c.f8u1, &c.f8s, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
+    for (sa::Buf* b : {&c.bfl, &c.bfu0, &c.bfu1, &c.bfg, &c.f8l, &c.f8u0, &c.f8u1, &c.f8s, &c.tlA, &c.tlB, &c.tlX, &c.tlP, &c.npu, &c.npt, &c.bmeta, &c.bwork, &c.slots, &c.err, &c.dinfo, &c.plan0, &c.plan1, &c.ipiv, &c.work,
                        &c.lo, &c.vec, &c.basis, &c.rhs, &c.plans, &c.perm, &c.flags, &c.tinv, &c.ttmp, &c.sinv, &c.stmp1, &c.dinv,
                        &c.tvw, &c.swaptmp, &c.swaptmp1, &c.ttri, &c.cut0, &c.cut1})
         b->release();
     for (auto e : c.evs) (void)hipEventDestroy(e);
     for (auto e : c.prof_ev) (void)hipEventDestroy(e);
+    for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.second);
     (void)hipEventDestroy(c.t_start);
     (void)hipEventDestroy(c.t_stop);
     (void)hipEventDestroy(c.ev_in);
@@ -2024,6 +2147,7 @@ void slate_amd_options_init(slate_amd_options* o)
     o->factor_precision = SLATE_AMD_PREC_FP32;
     o->method_lu = SLATE_AMD_LU_PARTIALPIV;
     o->depth = 2;
+    o->use_graph = 0;
 }
 
 double slate_amd_last_time_ms(slate_amd_context ctx) { return ctx ? ctx->c.last_ms : 0.0; }
@@ -2316,6 +2440,16 @@ int slate_amd_sgemm_bf16(slate_amd_context ctx, int terms, int64_t m, int64_t n,
                          const float* dA, int64_t lda, const float* dB, int64_t ldb, float* dC, int64_t ldc)
 { return gemm_bf16_entry(ctx, terms, m, n, k, alpha, dA, lda, dB, ldb, dC, ldc); }
 
+int slate_amd_graph_stats(slate_amd_context ctx, int64_t* captured, int64_t* failed, int64_t* replays)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        if (captured) *captured = int64_t(c.graphs.size());
+        if (failed) *failed = int64_t(c.graph_failed.size());
+        if (replays) *replays = c.graph_replays;
+        return 0;
+    }));
+}
+
 int slate_amd_profile_enable(slate_amd_context ctx, int on)
 {
     return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
@@ -2382,9 +2516,12 @@ int64_t slate_amd_selector_features(slate_amd_context ctx, int64_t n, const doub
         SA_REQUIRE(n >= 2, -3, "selector features need n >= 2");
         SA_ENTRY_SYNC();
         sa::Opts o = sa::read_opts(opts);
+        o.prec = sa::SEL_PROBE;                       // the features the selector sees
+        sa::Timer t(c);
         sa::MixedState st = sa::mixed_setup(c, n, 1);
         int64_t info = 0;
         sa::selector_features(c, st, n, dA, lda, o, &info, features);
+        t.stop();
         sa::check_kernel_error(c);
         return info;
     });
@@ -2408,9 +2545,10 @@ int slate_amd_mlp_train_step(slate_amd_context ctx, float* dparams, float* dgrad
         SA_REQUIRE(batch >= 1 && batch <= 4096, -8, "batch must be 1..4096");
         SA_REQUIRE(step >= 1, -10, "step counts from 1");
         SA_ENTRY_SYNC();
-        c.work.ensure(sizeof(float) * size_t(batch) * size_t(2 * 64 + 3) + sizeof(float) * 64);
+        const size_t per = size_t(sa::mlp_scratch_per_sample());
+        c.work.ensure(sizeof(float) * size_t(batch) * per + sizeof(float) * 64);
         float* scratch = c.work.as<float>();
-        float* dloss = scratch + size_t(batch) * size_t(2 * 64 + 3);
+        float* dloss = scratch + size_t(batch) * per;
         sa::mlp_train(dparams, dgrad, dm, dv, dX, dy, batch, scratch, lr, step, dloss, c.s0);
         if (loss) SA_HIP(hipMemcpyAsync(loss, dloss, sizeof(float), hipMemcpyDeviceToHost, c.s0));
         SA_HIP(hipStreamSynchronize(c.s0));
@@ -2419,8 +2557,8 @@ int slate_amd_mlp_train_step(slate_amd_context ctx, float* dparams, float* dgrad
 }
 
 int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int64_t lda, int64_t* ipiv,
-                              const double* dB, double* dX, const float* params, int* iter, int* choice,
-                              float* features, const slate_amd_options* opts)
+                              const double* dB, double* dX, const float* params, int force, int* iter,
+                              int* choice, float* features, const slate_amd_options* opts)
 {
     return guarded(ctx, [&](sa::Context& c) -> int64_t {
         check_dims(n, lda);
@@ -2429,7 +2567,7 @@ int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int6
         SA_ENTRY_SYNC();
         sa::Timer t(c);
         sa::Opts o = sa::read_opts(opts);
-        int64_t info = sa::mixed_auto(c, n, dA, lda, ipiv, dB, dX, iter, choice, params, features, o);
+        int64_t info = sa::mixed_select(c, n, dA, lda, ipiv, dB, dX, iter, choice, params, force, features, o);
         t.stop();
         return info;
     });
@@ -2576,3 +2714,192 @@ int slate_amd_generate(slate_amd_context ctx, int dist, int64_t seed, int64_t m,
 }
 
 } // extern "C"
+
+//==============================================================================
+// SLATE tile storage (slate_amd_tiles): the dense entries on gathered copies.
+//==============================================================================
+namespace {
+
+struct TileView {
+    const slate_amd_tiles* d;
+    int64_t mt, nt;
+};
+
+TileView check_tiles(const slate_amd_tiles* t, bool lower_only, const char* what)
+{
+    SA_REQUIRE(t && t->tiles && t->lds, -2, std::string(what) + ": null tile descriptor");
+    SA_REQUIRE(t->m >= 0 && t->n >= 0 && t->mb >= 1 && t->nb >= 1, -2, std::string(what) + ": bad tile sizes");
+    TileView v{t, sa::ceildiv(t->m, t->mb), sa::ceildiv(t->n, t->nb)};
+    for (int64_t j = 0; j < v.nt; ++j)
+        for (int64_t i = 0; i < v.mt; ++i) {
+            const int64_t k = i + j * v.mt;
+            const bool needed = !lower_only || i >= j;
+            if (!needed) continue;
+            SA_REQUIRE(t->tiles[k] != nullptr, -2, std::string(what) + ": missing tile");
+            const int64_t rows = std::min(t->mb, t->m - i * t->mb);
+            SA_REQUIRE(t->lds[k] >= std::max<int64_t>(1, rows), -2, std::string(what) + ": tile ld below its rows");
+        }
+    return v;
+}
+
+// Device copies of the pointer table (NULL for unread tiles) and the lds,
+// `slot` selects one of three table areas (A, B, X).
+void upload_tables(sa::Context& c, const TileView& v, bool lower_only, int slot, double*** dptr, int64_t** dld)
+{
+    const int64_t cnt = v.mt * v.nt;
+    static thread_local std::vector<void*> hp;
+    hp.assign(size_t(cnt), nullptr);
+    for (int64_t j = 0; j < v.nt; ++j)
+        for (int64_t i = 0; i < v.mt; ++i)
+            if (!lower_only || i >= j) hp[size_t(i + j * v.mt)] = v.d->tiles[i + j * v.mt];
+    // three equal areas sized for the largest table seen
+    const size_t area = (sizeof(void*) + sizeof(int64_t)) * size_t(cnt) + 256;
+    if (c.tlP.bytes < 3 * area) { c.tlP.release(); c.tlP.ensure(3 * area); }
+    const size_t a = c.tlP.bytes / 3 / 256 * 256;
+    char* base = static_cast<char*>(c.tlP.p) + size_t(slot) * a;
+    *dptr = reinterpret_cast<double**>(base);
+    *dld = reinterpret_cast<int64_t*>(base + sizeof(void*) * size_t(cnt));
+    SA_HIP(hipMemcpyAsync(*dptr, hp.data(), sizeof(void*) * size_t(cnt), hipMemcpyHostToDevice, c.s0));
+    SA_HIP(hipMemcpyAsync(*dld, v.d->lds, sizeof(int64_t) * size_t(cnt), hipMemcpyHostToDevice, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+}
+
+// Gather (to_tiles = 0) or scatter (1) between a tiled matrix and a dense
+// workspace buffer of ld = m.
+double* tiles_dense(sa::Context& c, const TileView& v, sa::Buf& buf, bool lower_only, int slot, int to_tiles)
+{
+    double** dp = nullptr;
+    int64_t* dl = nullptr;
+    upload_tables(c, v, lower_only, slot, &dp, &dl);
+    const int64_t m = v.d->m, n = v.d->n;
+    if (!to_tiles) buf.ensure(sizeof(double) * size_t(std::max<int64_t>(m, 1)) * size_t(std::max<int64_t>(n, 1)));
+    sa::tiles_copy<double>(dp, dl, m, n, v.d->mb, v.d->nb, buf.as<double>(), std::max<int64_t>(m, 1), to_tiles, c.s0);
+    SA_HIP(hipStreamSynchronize(c.s0));
+    return buf.as<double>();
+}
+
+// Run `dense(A, lda, B, ldb, X, ldx)` on gathered copies; scatter A back when
+// scatter_a(rc), B (when scatter_b) and X when given.  slate_amd_last_time_ms
+// reports the dense call alone (without the gather and scatter).
+template <class F, class G>
+int64_t with_tiles(slate_amd_context ctx, const slate_amd_tiles* A, bool lower, const slate_amd_tiles* B,
+                   const slate_amd_tiles* X, bool scatter_b, F&& dense, G&& scatter_a, const char* what)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        const TileView va = check_tiles(A, lower, what);
+        SA_REQUIRE(A->m == A->n, -2, std::string(what) + ": A must be square");
+        TileView vb{}, vx{};
+        if (B) { vb = check_tiles(B, false, what); SA_REQUIRE(B->m == A->n, -2, std::string(what) + ": B rows != n"); }
+        if (X) {
+            vx = check_tiles(X, false, what);
+            SA_REQUIRE(X->m == A->n && B && X->n == B->n, -2, std::string(what) + ": X shape != B shape");
+        }
+        SA_ENTRY_SYNC();
+        const int64_t n = A->n, ld = std::max<int64_t>(n, 1);
+        double* dA = tiles_dense(c, va, c.tlA, lower, 0, 0);
+        double* dB = B ? tiles_dense(c, vb, c.tlB, false, 1, 0) : nullptr;
+        double* dX = nullptr;
+        if (X) { c.tlX.ensure(sizeof(double) * size_t(ld) * size_t(std::max<int64_t>(X->n, 1))); dX = c.tlX.as<double>(); }
+        const int64_t rc = dense(dA, ld, dB, ld, dX, ld);
+        if (rc < 0) return rc;
+        if (scatter_a(rc)) tiles_dense(c, va, c.tlA, lower, 0, 1);
+        if (B && scatter_b) tiles_dense(c, vb, c.tlB, false, 1, 1);
+        if (X) tiles_dense(c, vx, c.tlX, false, 2, 1);
+        return rc;
+    });
+}
+
+} // namespace
+
+int64_t slate_amd_dgetrf_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                               const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, false, nullptr, nullptr, false,
+        [&](double* a, int64_t lda, double*, int64_t, double*, int64_t) {
+            return slate_amd_dgetrf(ctx, A->n, a, lda, ipiv, opts); },
+        [](int64_t) { return true; }, "dgetrf_tiles");
+}
+
+int64_t slate_amd_dgetrs_tiles(slate_amd_context ctx, int trans, const slate_amd_tiles* A, const int64_t* ipiv,
+                               const slate_amd_tiles* B, const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, false, B, nullptr, true,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double*, int64_t) {
+            return slate_amd_dgetrs(ctx, trans, A->n, B->n, a, lda, ipiv, b, ldb, opts); },
+        [](int64_t) { return false; }, "dgetrs_tiles");
+}
+
+int64_t slate_amd_dgesv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                              const slate_amd_tiles* B, const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, false, B, nullptr, true,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double*, int64_t) {
+            return slate_amd_dgesv(ctx, A->n, B->n, a, lda, ipiv, b, ldb, opts); },
+        [](int64_t) { return true; }, "dgesv_tiles");
+}
+
+int64_t slate_amd_dpotrf_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, true, nullptr, nullptr, false,
+        [&](double* a, int64_t lda, double*, int64_t, double*, int64_t) {
+            return slate_amd_dpotrf(ctx, SLATE_AMD_LOWER, A->n, a, lda, opts); },
+        [](int64_t) { return true; }, "dpotrf_tiles");
+}
+
+int64_t slate_amd_dpotrs_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                               const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, true, B, nullptr, true,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double*, int64_t) {
+            return slate_amd_dpotrs(ctx, SLATE_AMD_LOWER, A->n, B->n, a, lda, b, ldb, opts); },
+        [](int64_t) { return false; }, "dpotrs_tiles");
+}
+
+int64_t slate_amd_dposv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                              const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, true, B, nullptr, true,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double*, int64_t) {
+            return slate_amd_dposv(ctx, SLATE_AMD_LOWER, A->n, B->n, a, lda, b, ldb, opts); },
+        [](int64_t) { return true; }, "dposv_tiles");
+}
+
+// Mixed drivers: A is written back only when the FP64 fallback factored it
+// (*iter < 0), as the reference leaves A untouched on success.
+int64_t slate_amd_dsgesv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                               const slate_amd_tiles* B, const slate_amd_tiles* X, int* iter,
+                               const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, false, B, X, false,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double* x, int64_t ldx) {
+            return slate_amd_dsgesv(ctx, A->n, B->n, a, lda, ipiv, b, ldb, x, ldx, iter, opts); },
+        [&](int64_t) { return iter && *iter < 0; }, "dsgesv_tiles");
+}
+
+int64_t slate_amd_dsgesv_gmres_tiles(slate_amd_context ctx, const slate_amd_tiles* A, int64_t* ipiv,
+                                     const slate_amd_tiles* B, const slate_amd_tiles* X, int* iter,
+                                     const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, false, B, X, false,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double* x, int64_t ldx) {
+            return slate_amd_dsgesv_gmres(ctx, A->n, B->n, a, lda, ipiv, b, ldb, x, ldx, iter, opts); },
+        [&](int64_t) { return iter && *iter < 0; }, "dsgesv_gmres_tiles");
+}
+
+int64_t slate_amd_dsposv_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                               const slate_amd_tiles* X, int* iter, const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, true, B, X, false,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double* x, int64_t ldx) {
+            return slate_amd_dsposv(ctx, SLATE_AMD_LOWER, A->n, B->n, a, lda, b, ldb, x, ldx, iter, opts); },
+        [&](int64_t) { return iter && *iter < 0; }, "dsposv_tiles");
+}
+
+int64_t slate_amd_dsposv_gmres_tiles(slate_amd_context ctx, const slate_amd_tiles* A, const slate_amd_tiles* B,
+                                     const slate_amd_tiles* X, int* iter, const slate_amd_options* opts)
+{
+    return with_tiles(ctx, A, true, B, X, false,
+        [&](double* a, int64_t lda, double* b, int64_t ldb, double* x, int64_t ldx) {
+            return slate_amd_dsposv_gmres(ctx, SLATE_AMD_LOWER, A->n, B->n, a, lda, b, ldb, x, ldx, iter, opts); },
+        [&](int64_t) { return iter && *iter < 0; }, "dsposv_gmres_tiles");
+}

@@ -47,6 +47,7 @@ void larft_from_gram(const double* G, int64_t b, double* T, hipStream_t s);
 // tmax), 16-bin exponent histogram (hist zeroed by the caller), and the MLP
 // forward pass / one training step (parameters in one float array).
 int mlp_num_params();
+int mlp_scratch_per_sample();
 void feature_tiles(const double* A, int64_t lda, int64_t n, double* tmax, hipStream_t s);
 void feature_hist(const double* A, int64_t lda, int64_t n, int emax, unsigned long long* hist, hipStream_t s);
 void mlp_forward(const float* P, const float* X, int B, float* probs, hipStream_t s);
@@ -87,6 +88,14 @@ void quant_fp8_cols_t(int nt, const float* src, int64_t ld, int64_t K, int64_t N
 void gemm_fp8(int nt, int64_t M, int64_t N, int64_t Kp, const uint8_t* A, int64_t planeA, const int* sa,
               const uint8_t* B, int64_t planeB, const int* sb, float* C, int64_t ldc, float alpha, hipStream_t s,
               int role = 0, int uplo = 0);
+
+// SLATE tile storage (tiles.hip): copy between mt x nt independent column-major
+// tiles (device pointer table dtiles, per-tile ld dld, both device arrays,
+// tile (i, j) at index i + j*mt, NULL = not stored) and a dense column-major
+// matrix D; to_tiles = 1 scatters D into the tiles, 0 gathers.
+template <class T>
+void tiles_copy(T* const* dtiles, const int64_t* dld, int64_t m, int64_t n, int64_t mb, int64_t nb, T* D,
+                int64_t ldd, int to_tiles, hipStream_t s);
 
 // Workspace shared by the cross-workgroup panel kernel.
 struct PanelWork {

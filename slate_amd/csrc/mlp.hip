@@ -19,12 +19,13 @@
 
 namespace sa {
 
-static constexpr int MLP_F = 24, MLP_H = 64, MLP_K = 3;
+static constexpr int MLP_F = 24, MLP_H = 64, MLP_K = 8;
 static constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_F * MLP_H, MLP_W2 = MLP_B1 + MLP_H,
                      MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H,
                      MLP_B3 = MLP_W3 + MLP_H * MLP_K, MLP_NPARAM = MLP_B3 + MLP_K;
 
 int mlp_num_params() { return MLP_NPARAM; }
+int mlp_scratch_per_sample() { return 2 * MLP_H + MLP_K; }   // floats of mlp_train scratch per sample
 
 //------------------------------------------------------------------------------
 // Features.

@@ -1,16 +1,25 @@
-"""Learned choice of refinement scheme for the mixed-precision solve.
+"""Learned choice of factorization precision and refinement scheme.
 
-The selector sees, per system, the features of ``slate_amd.selector_features``
-(log2 n, the condition estimate from the float factor, per-tile max |a|
-spread, an exponent histogram) and picks one of ``slate_amd.CHOICES``:
-classic IR, GMRES-IR or plain FP64.  Its training data comes from the solver
-itself: each generated system is solved all three ways and timed, and the
-label is the fastest way that reached SLATE's FP64 tolerance (a refinement
-that misses it pays for the FP64 fallback, as in src/gesv_mixed.cc).  The
-network is trained on the GPU by the library's own kernels
-(``slate_amd.mlp_train_step``).
+A probe factor in BF16X2 (slate_amd.SELECTOR_PROBE) gives the features of
+``slate_amd.selector_features``: log2 n, the condition estimate from the probe
+factor, per-tile max |a| spread and an exponent histogram.  The MLP picks one
+of ``slate_amd.CHOICES``: keep the probe factor with classic IR or GMRES-IR,
+factor again in BF16X3, FP32 or FP8X2 with IR / GMRES-IR, or solve in plain
+FP64.  Its training data comes from the solver itself: every generated system
+is solved with each choice forced (``slate_amd.gesv_auto(force=c)``), and the
+cost of a choice is that call's measured device time, probe included, with a
+refinement that misses SLATE's tolerance paying for the FP64 fallback (as in
+src/gesv_mixed.cc); the label is the cheapest choice.  Training runs on the
+GPU with the library's own kernels (``slate_amd.mlp_train_step``).
 
-Usage (on a GPU):  python -m slate_amd.selector train --systems 600 --out PATH
+Systems (the mixed-precision configurations): n log-uniform in
+[512, 8192], kappa log-uniform in [1e1, 1e14], singular values geometric,
+arithmetic or clustered (matgen/generate_sigma.hh:61-111) of
+A = U diag(sigma) V^T with U, V products of n Householder reflectors.
+
+Usage (on a GPU):
+  python -m slate_amd.selector label --count 10000 --seed 1 --out part1.npz
+  python -m slate_amd.selector train --data part1.npz part2.npz --out selector_params.npy
 """
 import argparse
 import json
@@ -24,6 +33,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PARAMS = os.path.join(HERE, "selector_params.npy")
 SHAPES = (1, 2, 3, 4)          # sigma: arith, geo, cluster0, cluster1
+NMIN, NMAX = 512, 8192
+KMIN, KMAX = 1e1, 1e14
 
 
 def init_params(seed=0):
@@ -39,60 +50,76 @@ def init_params(seed=0):
     return P
 
 
-NMIN, NMAX = 512, 4096
-
-
 def sample_system(rng, nmin=NMIN, nmax=NMAX):
-    """(n, kappa, sigma shape, seed) of one training system: n log-uniform in
+    """(n, kappa, sigma shape, seed) of one system: n log-uniform in
     [nmin, nmax], kappa log-uniform in [1e1, 1e14]."""
     n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
-    kappa = 10.0 ** rng.uniform(1.0, 14.0)
+    kappa = 10.0 ** rng.uniform(math.log10(KMIN), math.log10(KMAX))
     return n, kappa, int(rng.choice(SHAPES)), int(rng.integers(1, 2 ** 31))
 
 
-def label_system(n, kappa, shape, seed, opts=None):
-    """Solve one generated system three ways; returns a dict with the
-    features, the cost of each choice (ms), iterations and the label."""
+def label_system(n, kappa, shape, seed):
+    """Solve one generated system with every choice forced; returns the
+    features, per-choice cost (device ms, probe included), iteration codes
+    and info, the probe's own time, and the label (cheapest choice)."""
     import torch
     import slate_amd as S
     A0 = S.empty_colmajor(n, n, torch.float64)
     S.generate_svd(A0, shape, kappa, seed)
     B = S.empty_colmajor(n, 1, torch.float64)
     S.generate(B, S.DIST_RANDS, seed + 7)
-    feat, finfo = S.selector_features(A0, opts)
-    # the float factor the auto path always computes (features + factor)
-    A32 = S.colmajor(A0.float())
-    S.getrf(A32, opts)
-    t_factor = S.last_time_ms()
-    del A32
     X = S.empty_colmajor(n, 1, torch.float64)
-    A = A0.clone()
-    _, it_ir, _ = S.gesv_mixed(A, B, X, opts)
-    t_ir = S.last_time_ms()
-    A = A0.clone()
-    _, it_gm, _ = S.gesv_mixed_gmres(A, B, X, opts)
-    t_gm = S.last_time_ms()
-    A = A0.clone()
-    B2 = B.clone()
-    S.gesv(A, B2, opts)
-    t_64 = S.last_time_ms()
-    cost = [t_ir, t_gm, t_factor + t_64]
-    return {"n": n, "kappa": kappa, "shape": shape, "seed": seed, "features": feat.tolist(),
-            "cost": cost, "iters": [it_ir, it_gm], "label": int(np.argmin(cost))}
+    K = len(S.CHOICES)
+    cost = np.zeros(K)
+    iters = np.zeros(K, dtype=np.int32)
+    infos = np.zeros(K, dtype=np.int32)
+    feat = None
+    A = torch.empty_like(A0)
+    for c in range(K):
+        A.copy_(A0)
+        info, it, ch, _, f = S.gesv_auto(A, B, X, force=c)
+        assert ch == c
+        cost[c] = S.last_time_ms()
+        iters[c], infos[c] = it, info
+        if feat is None:
+            feat = f
+    A.copy_(A0)
+    S.selector_features(A)
+    t_probe = S.last_time_ms()
+    return {"n": n, "kappa": kappa, "shape": shape, "seed": seed, "features": feat, "cost": cost,
+            "iters": iters, "info": infos, "t_probe": t_probe, "label": int(np.argmin(cost))}
 
 
-def build_dataset(count, seed=1, nmin=NMIN, nmax=NMAX, log=None):
+def label_many(count, seed, nmin=NMIN, nmax=NMAX, log=None):
     rng = np.random.default_rng(seed)
     rows = []
     t0 = time.time()
     for i in range(count):
         rows.append(label_system(*sample_system(rng, nmin, nmax)))
-        if log and (i + 1) % 25 == 0:
+        if log and (i + 1) % 100 == 0:
             log("selector: %d/%d systems labelled (%.0f s)" % (i + 1, count, time.time() - t0))
     return rows
 
 
-def train(rows, steps=1500, batch=256, lr=3e-3, seed=0, log=None):
+FIELDS = ("n", "kappa", "shape", "seed", "features", "cost", "iters", "info", "t_probe", "label")
+
+
+def save_rows(path, rows):
+    np.savez_compressed(path, **{k: np.array([r[k] for r in rows]) for k in FIELDS})
+
+
+def load_rows(*paths):
+    """Labelled systems from one or more .npz files (concatenated)."""
+    cols = {k: [] for k in FIELDS}
+    for p in paths:
+        with np.load(p, allow_pickle=False) as z:
+            for k in FIELDS:
+                cols[k].append(z[k])
+    cols = {k: np.concatenate(v) for k, v in cols.items()}
+    return [{k: cols[k][i] for k in FIELDS} for i in range(len(cols["n"]))]
+
+
+def train(rows, steps=3000, batch=512, lr=3e-3, seed=0, log=None):
     """Train on the GPU with the library's kernels; returns numpy parameters."""
     import torch
     import slate_amd as S
@@ -107,61 +134,99 @@ def train(rows, steps=1500, batch=256, lr=3e-3, seed=0, log=None):
     for t in range(1, steps + 1):
         idx = torch.randint(0, X.shape[0], (min(batch, X.shape[0]),), generator=g).cuda()
         Xb, yb = X[idx].contiguous(), y[idx].contiguous()
-        loss = S.mlp_train_step(P, G, M, V, Xb, yb, lr, t)
-        if log and t % 250 == 0:
+        step_lr = lr * (0.1 if t > 0.8 * steps else 1.0)
+        loss = S.mlp_train_step(P, G, M, V, Xb, yb, step_lr, t)
+        if log and t % 500 == 0:
             log("selector: step %d loss %.4f" % (t, loss))
     return P.cpu().numpy(), loss
 
 
-def predict(params, rows):
-    import torch
+def predict_numpy(params, features):
+    """Forward pass on the host (the layout of slate_amd/csrc/mlp.hip): the
+    choice per row.  Used for evaluating committed parameters without a GPU."""
     import slate_amd as S
-    X = torch.tensor(np.array([r["features"] for r in rows], dtype=np.float32), device="cuda")
-    P = torch.tensor(np.asarray(params, dtype=np.float32), device="cuda")
-    return S.mlp_forward(P, X).argmax(dim=1).cpu().numpy()
+    F, H, K = S.SELECTOR_FEATURES, 64, len(S.CHOICES)
+    P = np.asarray(params, dtype=np.float64)
+    o = 0
+    mats = []
+    for shape in ((F, H), (H,), (H, H), (H,), (H, K), (K,)):
+        sz = int(np.prod(shape))
+        mats.append(P[o:o + sz].reshape(shape))
+        o += sz
+    W1, b1, W2, b2, W3, b3 = mats
+    X = np.asarray(features, dtype=np.float64)
+    h1 = np.maximum(X @ W1 + b1, 0)
+    h2 = np.maximum(h1 @ W2 + b2, 0)
+    return np.argmax(h2 @ W3 + b3, axis=1)
 
 
-def evaluate(params, rows):
-    """Misselection rate and total cost of the MLP's choices against SLATE's
-    default (always classic IR, FP64 fallback on a miss) and the best choice."""
-    pred = predict(params, rows)
+def evaluate(params, rows, pred=None):
+    """Misselection rate and total cost of the MLP's choices against the best
+    choice per system and against fixed policies: SLATE's default (float
+    factor + classic IR without the probe), and always the same choice."""
+    import slate_amd as S
     cost = np.array([r["cost"] for r in rows])
     label = np.array([r["label"] for r in rows])
+    t_probe = np.array([r["t_probe"] for r in rows])
+    if pred is None:
+        pred = predict_numpy(params, np.array([r["features"] for r in rows]))
     idx = np.arange(len(rows))
+    best = cost[idx, label].sum()
+    mlp = cost[idx, pred].sum()
+    fixed = {S.CHOICES[k]: float(cost[:, k].sum()) for k in range(cost.shape[1])}
+    default = float(np.maximum(cost[:, S.DEFAULT_CHOICE] - t_probe, 0).sum())
+    near = np.mean(cost[idx, pred] <= 1.1 * cost[idx, label])
     return {
         "systems": len(rows),
         "misselection_rate": float(np.mean(pred != label)),
-        "ms_mlp": float(cost[idx, pred].sum()),
-        "ms_default_ir": float(cost[:, 0].sum()),
-        "ms_always_fp64": float(cost[:, 2].sum()),
-        "ms_best": float(cost[idx, label].sum()),
+        "within_10pct_of_best_rate": float(near),
+        "ms_mlp": float(mlp), "ms_best": float(best), "mlp_over_best": float(mlp / best),
+        "ms_slate_default_no_probe": default, "mlp_over_default": float(mlp / default),
+        "ms_fixed_choice": fixed,
+        "best_fixed_choice": min(fixed, key=fixed.get),
+        "mlp_over_best_fixed": float(mlp / min(fixed.values())),
         "choices": [int(np.sum(pred == k)) for k in range(cost.shape[1])],
         "labels": [int(np.sum(label == k)) for k in range(cost.shape[1])],
     }
 
 
+def split(rows, holdout=0.2, seed=0):
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(rows))
+    k = int(len(rows) * (1 - holdout))
+    return [rows[i] for i in perm[:k]], [rows[i] for i in perm[k:]]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("cmd", choices=["train"])
-    ap.add_argument("--systems", type=int, default=600)
-    ap.add_argument("--holdout", type=float, default=0.25)
-    ap.add_argument("--steps", type=int, default=1500)
-    ap.add_argument("--out", default=DEFAULT_PARAMS)
-    ap.add_argument("--data", default=None, help="also write the labelled systems (JSON) here")
+    ap.add_argument("cmd", choices=["label", "train"])
+    ap.add_argument("--count", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--nmin", type=int, default=NMIN)
+    ap.add_argument("--nmax", type=int, default=NMAX)
+    ap.add_argument("--data", nargs="*", default=[])
+    ap.add_argument("--holdout", type=float, default=0.2)
+    ap.add_argument("--holdout-out", default=None, help="write the held-out systems here (.npz)")
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
-    rows = build_dataset(args.systems, log=log)
-    k = int(len(rows) * (1 - args.holdout))
-    params, loss = train(rows[:k], steps=args.steps, log=log)
-    np.save(args.out, params)
-    report = {"train": evaluate(params, rows[:k]), "holdout": evaluate(params, rows[k:]), "final_loss": loss,
-              "n_range": [NMIN, NMAX], "kappa_range": [1e1, 1e14]}
-    if args.data:
-        with open(args.data, "w") as f:
-            json.dump(rows, f)
+    if args.cmd == "label":
+        rows = label_many(args.count, args.seed, args.nmin, args.nmax, log=log)
+        save_rows(args.out, rows)
+        log("selector: wrote %d systems to %s" % (len(rows), args.out))
+        return
+    rows = load_rows(*args.data)
+    tr, ho = split(rows, args.holdout)
+    params, loss = train(tr, steps=args.steps, log=log)
+    np.save(args.out or DEFAULT_PARAMS, params)
+    if args.holdout_out:
+        save_rows(args.holdout_out, ho)
+    report = {"train": evaluate(params, tr), "holdout": evaluate(params, ho), "final_loss": loss,
+              "n_range": [args.nmin, args.nmax], "kappa_range": [KMIN, KMAX], "systems": len(rows)}
     print(json.dumps(report), flush=True)
 
 

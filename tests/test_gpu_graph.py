@@ -1,0 +1,69 @@
+"""HIP-graph replay of the LU step sequence (Options use_graph).
+
+The first call with a given matrix, pivot and workspace set runs eagerly and
+records the same multi-stream launch sequence (panel stream, trailing stream,
+CU-masked update stream, joined by events) into a graph; later calls replay
+it.  Each replay here gets new matrix contents, the program's own eager work
+(other solves, other sizes) and synchronisations run between replays, and
+every replay must equal the eager solve of the same input bitwise."""
+import numpy as np
+import pytest
+import torch
+
+import slate_amd as S
+from helpers import dev_rands, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _eager_other_work():
+    A = dev_rands(99, 300, 300)
+    B = dev_rands(98, 300, 1)
+    S.gesv(A, B, S.Options(block_size=128))
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("case", ["fp64", "fp32-ir", "bf16x2-gmres", "fp8x2-gmres", "calu"])
+def test_graph_replays_equal_eager(gpu, case):
+    n, nb = 2048, 256
+    say("graph replay %s n=%d" % (case, n))
+    prec = {"fp64": None, "fp32-ir": S.PREC_FP32, "bf16x2-gmres": S.PREC_BF16X2, "fp8x2-gmres": S.PREC_FP8X2,
+            "calu": S.PREC_FP32}[case]
+    method = S.LU_CALU if case == "calu" else S.LU_PARTIALPIV
+    kw = dict(block_size=nb, method_lu=method)
+    if prec is not None:
+        kw["factor_precision"] = prec
+    og, oe = S.Options(use_graph=1, **kw), S.Options(use_graph=0, **kw)
+    A = S.empty_colmajor(n, n, torch.float64)              # one buffer: the graph's key includes its address
+    X = S.empty_colmajor(n, 1, torch.float64)
+    _eager_other_work()                                   # its workspaces exist before the first capture
+    cap0, _, rep0 = S.graph_stats()
+    for rnd in range(4):
+        Ain = dev_rands(200 + rnd, n, n)
+        B = dev_rands(300 + rnd, n, 1)
+        # eager reference on a separate copy
+        Ae = Ain.clone()
+        Xe = S.empty_colmajor(n, 1, torch.float64)
+        if prec is None:
+            Be = B.clone()
+            info_e, ip_e = S.gesv(Ae, Be, oe)
+            A.copy_(Ain)
+            Bg = B.clone()
+            info_g, ip_g = S.gesv(A, Bg, og)
+            same = torch.equal(Bg, Be) and torch.equal(A, Ae)
+            it_e = it_g = 0
+        else:
+            f = S.gesv_mixed_gmres if "gmres" in case else S.gesv_mixed
+            info_e, it_e, ip_e = f(Ae, B, Xe, oe)
+            A.copy_(Ain)
+            info_g, it_g, ip_g = f(A, B, X, og)
+            same = torch.equal(X, Xe) and torch.equal(A, Ae)
+        say("  round %d: info %d/%d iter %d/%d equal %s" % (rnd, info_g, info_e, it_g, it_e, same))
+        assert info_g == info_e and it_g == it_e
+        np.testing.assert_array_equal(ip_g, ip_e)
+        assert same
+        _eager_other_work()
+    cap, failed, rep = S.graph_stats()
+    say("  graphs captured %d failed %d replays %d" % (cap, failed, rep))
+    assert failed == 0
+    assert cap > cap0 and rep - rep0 >= 3
