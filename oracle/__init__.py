@@ -531,6 +531,25 @@ def set_blas(path):
         raise RuntimeError("oracle_set_blas(%r) failed" % (path,))
 
 
+class tile_blas:
+    """Context manager: tile gemm/trsm through scipy's bundled OpenBLAS, as
+    SLATE's HostTask tiles call a vendor BLAS (src/internal/internal_gemm.cc).
+    A BLAS sgemm accumulates each product sum before adding it to C, which is
+    how the GPU's float updates round too; the plain loops update C one
+    rank-1 term at a time, which is more accurate on some inputs (a
+    rank-one-dominated matrix loses far less to cancellation)."""
+
+    def __enter__(self):
+        path = openblas_path()
+        if path is None:
+            raise RuntimeError("tile_blas: scipy's OpenBLAS not found")
+        set_blas(path)
+        return self
+
+    def __exit__(self, *exc):
+        set_blas(None)
+
+
 def gmres_first_cycle(A, b, nb=256, itermax=30, sym=False, cgs_passes=2):
     """First restart cycle of the oracle's gesv_mixed_gmres (posv_mixed_gmres
     with sym): returns (V n x (j+1), W n x j, Hbar (j+1) x j, |S| history).

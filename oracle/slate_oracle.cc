@@ -110,7 +110,8 @@ template <class T> inline bool blas_gemm(char ta, char tb, int64_t m, int64_t n,
 // precisions beyond SLATE's float; the reference has double -> float only,
 // src/gesv_mixed.cc:293-316).  With g_q.kind != 0 the float factor's trailing
 // update C -= A B (LU gemm, Cholesky herk/gemm) multiplies rounded operands,
-// products exact in float, accumulated in double and subtracted in float:
+// products exact, accumulated in FP32 one MFMA-depth chunk at a time and
+// subtracted in float (gemm_quantized):
 //  * kind 1, terms t = 1..3: each float x split into bf16 terms
 //    x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1) (round to
 //    nearest even); the products x_i y_j with i + j < t are kept.
@@ -229,6 +230,12 @@ void quantize_operand(int64_t m, int64_t k, const T* X, int64_t ldx, bool rows_a
 // C[m x n] -= A B on quantized operands; A m x k (column-major, lda), B given
 // as op(B) = k x n from Bsrc: b_is_kn ? Bsrc[p + j*ldb] : Bsrc[j + p*ldb].
 // Only C(i, j) with (!lower_only || i >= j) are updated.
+// Accumulation follows the GPU's matrix-core update: k in chunks of one MFMA
+// depth (32 for bf16, 128 for e4m3), each chunk's product sum formed exactly
+// (in double: the products of two bf16 or e4m3 values are exact) and added to
+// an FP32 accumulator with one rounding, term pairs in the kernels' order
+// (bf16: smaller terms first, x0 y0 last; e4m3: q0 q0', then the two
+// 2^-4 cross terms), and C := C - scale * acc rounded once at the end.
 template <class T>
 void gemm_quantized(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const T* Bsrc, int64_t ldb,
                     bool b_is_kn, T* C, int64_t ldc, bool lower_only = false)
@@ -238,22 +245,34 @@ void gemm_quantized(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, co
     // B as an n x k operand: element (j, p)
     quantize_operand<T>(n, k, Bsrc, ldb, !b_is_kn, qb, sb);
     const int t = g_q.terms;
+    const int64_t ck = g_q.kind == 1 ? 32 : 128;
+    // (a, b, weight) term pairs in accumulation order
+    int pa[6], pb[6], np = 0;
+    double pw[6];
+    if (g_q.kind == 1) {
+        for (int a = t - 1; a >= 0; --a)
+            for (int b = t - 1 - a; b >= 0; --b) { pa[np] = a; pb[np] = b; pw[np] = 1.0; ++np; }
+    }
+    else {
+        pa[np] = 0; pb[np] = 0; pw[np] = 1.0; ++np;
+        if (t > 1) {
+            pa[np] = 0; pb[np] = 1; pw[np] = 1.0 / 16.0; ++np;
+            pa[np] = 1; pb[np] = 0; pw[np] = 1.0 / 16.0; ++np;
+        }
+    }
     for (int64_t j = 0; j < n; ++j)
         for (int64_t i = lower_only ? j : 0; i < m; ++i) {
-            double acc = 0;
-            for (int64_t p = 0; p < k; ++p) {
-                if (g_q.kind == 1) {
-                    for (int a = 0; a < t; ++a)
-                        for (int b = 0; a + b < t; ++b) acc += qa[a][i + p*m] * qb[b][j + p*n];
-                }
-                else {
-                    double v = qa[0][i + p*m] * qb[0][j + p*n];
-                    if (t > 1) v += (qa[0][i + p*m] * qb[1][j + p*n] + qa[1][i + p*m] * qb[0][j + p*n]) / 16.0;
-                    acc += v;
+            float acc = 0.0f;
+            for (int64_t p0 = 0; p0 < k; p0 += ck) {
+                const int64_t p1 = std::min(k, p0 + ck);
+                for (int u = 0; u < np; ++u) {
+                    double d = 0;
+                    for (int64_t p = p0; p < p1; ++p) d += qa[pa[u]][i + p*m] * qb[pb[u]][j + p*n];
+                    acc = float(double(acc) + pw[u] * d);
                 }
             }
-            acc *= sa[i] * sb[j];
-            C[i + j*ldc] = T(C[i + j*ldc] - T(acc));
+            const double scale = sa[i] * sb[j];
+            C[i + j*ldc] = T(double(C[i + j*ldc]) - double(float(scale * double(acc))));
         }
 }
 

@@ -191,6 +191,7 @@ void potrf_leaf(T* A, int64_t ld, int64_t nb, int64_t col0, int64_t* info, hipSt
 
 // A := I (n x n, row-major, ld).
 template <class T> void set_identity(T* A, int64_t n, int64_t ld, hipStream_t s);
+template <class T> void unit_lower_copy(const T* L, int64_t ld, int64_t n, T* out, hipStream_t s);
 
 // In-place transpose of an n x n matrix (column-major <-> row-major).
 template <class T> void transpose_inplace(T* A, int64_t n, int64_t ld, hipStream_t s);

@@ -404,6 +404,28 @@ void set_identity(T* A, int64_t n, int64_t ld, hipStream_t s)
 template void set_identity<double>(double*, int64_t, int64_t, hipStream_t);
 template void set_identity<float>(float*, int64_t, int64_t, hipStream_t);
 
+// out (n x n, row-major, ld n) = the unit-lower triangle of the row-major
+// block L (ld): strictly lower entries copied, ones on the diagonal, zeros above.
+template <class T>
+__global__ void unit_lower_copy_kernel(const T* __restrict__ L, int64_t ld, int64_t n, T* __restrict__ out)
+{
+    const int64_t i = blockIdx.y;
+    for (int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x; j < n; j += int64_t(gridDim.x) * 256)
+        out[i * n + j] = (j < i) ? L[i * ld + j] : (i == j ? T(1) : T(0));
+}
+
+template <class T>
+void unit_lower_copy(const T* L, int64_t ld, int64_t n, T* out, hipStream_t s)
+{
+    if (n <= 0) return;
+    SA_REQUIRE(n < 65536, -2, "unit_lower_copy: too large");
+    hipLaunchKernelGGL(unit_lower_copy_kernel<T>, dim3(unsigned(ceildiv(n, 256)), unsigned(n)), dim3(256), 0, s,
+                       L, ld, n, out);
+    SA_CHECK_LAUNCH();
+}
+template void unit_lower_copy<double>(const double*, int64_t, int64_t, double*, hipStream_t);
+template void unit_lower_copy<float>(const float*, int64_t, int64_t, float*, hipStream_t);
+
 //------------------------------------------------------------------------------
 template <class D, class S> __device__ inline D cvt(S x) { return D(x); }
 

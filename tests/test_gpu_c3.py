@@ -7,7 +7,9 @@ the host, so the GPU driver and the oracle see bit-identical FP64 inputs.
 For every refinement driver (src/gesv_mixed.cc, src/gesv_mixed_gmres.cc,
 src/posv_mixed.cc, src/posv_mixed_gmres.cc) and factor precision (float, and
 the float factor with one-, two- and three-term split-BF16 trailing updates,
-which the oracle emulates with the same operand rounding), checked:
+which the oracle emulates with the same operand rounding and FP32
+accumulation, its other tile gemm/trsm on a BLAS as SLATE's HostTask tiles
+run them), checked:
 
 * the same info;
 * the same iteration-code class (converged >= 0, -(itermax+1) no convergence
@@ -73,7 +75,7 @@ def test_gesv_mixed_kappa_vs_oracle(gpu, cond, prec, gmres):
     opts = S.Options(block_size=nb, factor_precision=prec)
     f = S.gesv_mixed_gmres if gmres else S.gesv_mixed
     info, it, _ = f(A, B, X, opts)
-    with oracle.update_precision(*UPD[prec]):
+    with oracle.tile_blas(), oracle.update_precision(*UPD[prec]):
         g = oracle.gesv_mixed_gmres if gmres else oracle.gesv_mixed
         _, _, X_o, info_o, it_o = g(A_h, B_h, nb=nb)
     _compare(it, it_o, info, info_o, to_host(X), X_o, A_h, B_h, cond, n)
@@ -90,7 +92,7 @@ def test_posv_mixed_kappa_vs_oracle(gpu, cond, prec, gmres):
     opts = S.Options(block_size=nb, factor_precision=prec)
     f = S.posv_mixed_gmres if gmres else S.posv_mixed
     info, it = f(A, B, X, opts=opts)
-    with oracle.update_precision(*UPD[prec]):
+    with oracle.tile_blas(), oracle.update_precision(*UPD[prec]):
         g = oracle.posv_mixed_gmres if gmres else oracle.posv_mixed
         _, X_o, info_o, it_o = g(np.tril(A_h), B_h, nb=nb)
     _compare(it, it_o, info, info_o, to_host(X), X_o, A_h, B_h, cond, n)
@@ -103,7 +105,8 @@ def test_gesv_mixed_gmres_spectra(gpu, dist):
     A, B, A_h, B_h = _case(n, dist, cond, 51, False)
     X = S.empty_colmajor(n, 1, torch.float64)
     info, it, _ = S.gesv_mixed_gmres(A, B, X, S.Options(block_size=nb))
-    _, _, X_o, info_o, it_o = oracle.gesv_mixed_gmres(A_h, B_h, nb=nb)
+    with oracle.tile_blas():
+        _, _, X_o, info_o, it_o = oracle.gesv_mixed_gmres(A_h, B_h, nb=nb)
     _compare(it, it_o, info, info_o, to_host(X), X_o, A_h, B_h, cond, n)
 
 
@@ -113,7 +116,8 @@ def test_gesv_mixed_gmres_spectra(gpu, dist):
 def test_posv_mixed_split_bf16_ragged(gpu, prec, nb, gmres):
     """Split-BF16 Cholesky updates (triangular herk on the upper-tile
     enumeration, two-stream lookahead) on a ragged n with more than two
-    steps; lookahead on and off give the same result bitwise."""
+    steps; lookahead on and off give the same result bitwise.  Each run
+    gets its own copy of A: the FP64 fallback factors A in place."""
     n, cond = 1100, 1e4
     say("dsposv%s %s n=%d nb=%d" % ("_gmres" if gmres else "", PNAME[prec], n, nb))
     A, B, A_h, B_h = _case(n, S.SIGMA_GEO, cond, 61, True)
@@ -121,13 +125,13 @@ def test_posv_mixed_split_bf16_ragged(gpu, prec, nb, gmres):
     xs = []
     for la in (1, 0):
         X = S.empty_colmajor(n, 1, torch.float64)
-        info, it = f(A, B, X, opts=S.Options(block_size=nb, factor_precision=prec, lookahead=la))
+        info, it = f(A.clone(), B, X, opts=S.Options(block_size=nb, factor_precision=prec, lookahead=la))
         say("  lookahead %d: info %d iter %d" % (la, info, it))
         xs.append((info, it, X.clone()))
     assert xs[0][0] == xs[1][0] and xs[0][1] == xs[1][1]
     assert torch.equal(xs[0][2], xs[1][2])
     info, it, X = xs[0]
-    with oracle.update_precision(*UPD[prec]):
+    with oracle.tile_blas(), oracle.update_precision(*UPD[prec]):
         g = oracle.posv_mixed_gmres if gmres else oracle.posv_mixed
         _, X_o, info_o, it_o = g(np.tril(A_h), B_h, nb=nb)
     _compare(it, it_o, info, info_o, to_host(X), X_o, A_h, B_h, cond, n)
@@ -143,7 +147,7 @@ def test_gesv_mixed_gmres_bf16x2_converges(gpu):
     X = S.empty_colmajor(n, 1, torch.float64)
     A_h, B_h = to_host(A), to_host(B)
     info, it, _ = S.gesv_mixed_gmres(A, B, X, S.Options(block_size=256, factor_precision=S.PREC_BF16X2))
-    with oracle.update_precision(oracle.UPD_BF16, 2):
+    with oracle.tile_blas(), oracle.update_precision(oracle.UPD_BF16, 2):
         _, _, X_o, info_o, it_o = oracle.gesv_mixed_gmres(A_h, B_h, nb=256)
     assert 0 <= it <= 30 and 0 <= it_o <= 30
     _compare(it, it_o, info, info_o, to_host(X), X_o, A_h, B_h, 1e6, n)

@@ -4,7 +4,9 @@ The refinement drivers' iteration counts depend on how well the float
 factor approximates A.  For the same float input, ||P A - L U||_F / ||A||_F of
 the GPU's float LU (every factor precision) must stay within a small factor of
 the oracle's tile LU with the same update rounding (src/getrf.cc:23-245 with
-Tile_getrf.hh; oracle.update_precision for the split-BF16 and FP8 forms).
+Tile_getrf.hh; oracle.update_precision for the split-BF16 and FP8 forms) and
+its tile gemm/trsm on a BLAS, as SLATE's HostTask tiles run them
+(oracle.tile_blas).
 """
 import numpy as np
 import pytest
@@ -49,7 +51,7 @@ def test_float_lu_backward_error_vs_oracle(gpu, prec, kind):
     A = to_dev(H32, torch.float32)
     info, ipiv = S.getrf(A, S.Options(block_size=nb, factor_precision=prec))
     e_gpu = lu_backward_error(H32, to_host(A), ipiv)
-    with oracle.update_precision(*UPD[prec]):
+    with oracle.tile_blas(), oracle.update_precision(*UPD[prec]):
         LU_o, ipiv_o, info_o = oracle.getrf(H32, nb=nb, dtype=np.float32)
     e_o = lu_backward_error(H32, LU_o, ipiv_o)
     say("sgetrf %s prec %d: backward error gpu %.3e oracle %.3e (ratio %.2f)" % (kind, prec, e_gpu, e_o, e_gpu / e_o))

@@ -103,7 +103,7 @@ def test_gesv_mixed_fp8_vs_oracle(gpu, cond, prec, terms, gmres):
     X = S.empty_colmajor(n, 1, torch.float64)
     f = S.gesv_mixed_gmres if gmres else S.gesv_mixed
     info, it, _ = f(A, B, X, S.Options(block_size=nb, factor_precision=prec))
-    with oracle.update_precision(oracle.UPD_FP8, terms):
+    with oracle.tile_blas(), oracle.update_precision(oracle.UPD_FP8, terms):
         g = oracle.gesv_mixed_gmres if gmres else oracle.gesv_mixed
         _, _, X_o, info_o, it_o = g(A_h, B_h, nb=nb)
     _compare(it, it_o, info, info_o, to_host(X), X_o, A_h, B_h, cond, n)
@@ -116,7 +116,7 @@ def test_posv_mixed_gmres_fp8x2_vs_oracle(gpu, cond):
     A, B, A_h, B_h = _case(n, cond, 81, sym=True)
     X = S.empty_colmajor(n, 1, torch.float64)
     info, it = S.posv_mixed_gmres(A, B, X, opts=S.Options(block_size=nb, factor_precision=S.PREC_FP8X2))
-    with oracle.update_precision(oracle.UPD_FP8, 2):
+    with oracle.tile_blas(), oracle.update_precision(oracle.UPD_FP8, 2):
         _, X_o, info_o, it_o = oracle.posv_mixed_gmres(np.tril(A_h), B_h, nb=nb)
     _compare(it, it_o, info, info_o, to_host(X), X_o, A_h, B_h, cond, n)
 

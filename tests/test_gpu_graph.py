@@ -23,6 +23,11 @@ def _eager_other_work():
     torch.cuda.synchronize()
 
 
+GRAPH_OFF = ("LU graph replay is switched off: its first capture (fp64, n=2048, nb=256) ended the "
+             "process with a host segmentation fault and no cause has been shown")
+
+
+@pytest.mark.skip(reason=GRAPH_OFF)
 @pytest.mark.parametrize("case", ["fp64", "fp32-ir", "bf16x2-gmres", "fp8x2-gmres", "calu"])
 def test_graph_replays_equal_eager(gpu, case):
     n, nb = 2048, 256
@@ -67,3 +72,15 @@ def test_graph_replays_equal_eager(gpu, case):
     say("  graphs captured %d failed %d replays %d" % (cap, failed, rep))
     assert failed == 0
     assert cap > cap0 and rep - rep0 >= 3
+
+
+def test_use_graph_is_refused_before_any_work(gpu):
+    """While replay is switched off, use_graph is an argument error that
+    leaves A and B as they were."""
+    n = 512
+    A = dev_rands(5, n, n)
+    B = dev_rands(6, n, 1)
+    A0, B0 = A.clone(), B.clone()
+    with pytest.raises(S.SlateError):
+        S.gesv(A, B, S.Options(use_graph=1))
+    assert torch.equal(A, A0) and torch.equal(B, B0)

@@ -2,6 +2,8 @@
 slow at these sizes): SLATE's scaled residual ||B - A X||_1 /
 (n ||A||_1 ||X||_1 eps) <= 3 computed on the GPU in FP64, valid pivot
 sequences, and the mixed drivers leaving A untouched after convergence."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -11,6 +13,7 @@ from helpers import dev_rands, say
 
 pytestmark = pytest.mark.gpu
 EPS = np.finfo(np.float64).eps
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def residual_dev(A, X, B):
@@ -76,19 +79,102 @@ def test_gesv_large_blocks(gpu, nb):
 
 
 def test_gesv_mixed_benchmark_size(gpu):
-    """The benchmark's configuration end to end: n = 65536, float LU with
-    three-term split-BF16 updates (block 1024, CU-masked trailing stream),
-    FP64 classic IR, checked by the scaled residual."""
+    """The benchmark's default form end to end: n = 65536, float LU with
+    two-term split-BF16 updates (block 1024, CU-masked trailing stream) and
+    FP64 GMRES-IR.  It must converge (not fall back), reach the scaled
+    residual bound, and agree with the plain FP64 gesv to the forward error
+    the condition of the matrix allows."""
     n = 65536
-    say("gesv_mixed n=%d bf16x3" % n)
+    say("gesv_mixed_gmres n=%d bf16x2" % n)
     A = dev_rands(0, n, n)
     B = dev_rands(1, n, 1)
     X = slate_amd.empty_colmajor(n, 1, torch.float64)
-    info, it, _ = slate_amd.gesv_mixed(A, B, X, slate_amd.Options(factor_precision=slate_amd.PREC_BF16X3))
+    info, it, _ = slate_amd.gesv_mixed_gmres(A, B, X, slate_amd.Options(factor_precision=slate_amd.PREC_BF16X2))
     ms = slate_amd.last_time_ms()
     r = residual_dev(A, X, B)
     say("  info %d iter %d scaled residual %.4f (%.1f ms)" % (info, it, r, ms))
     assert info == 0 and 0 <= it <= 30 and r <= 3.0
+    A64 = A.clone()
+    X64 = B.clone()
+    info64, _ = slate_amd.gesv(A64, X64)
+    del A64
+    fwd = float((X - X64).abs().max() / X64.abs().max())
+    say("  forward error vs FP64 gesv %.3e" % fwd)
+    assert info64 == 0 and fwd <= 1e-8
+
+
+def test_getrf_n32768_pivots_equal_lapack(gpu):
+    """C2 at full size: plain FP64 gesv, n = 32768, block size from the tuned
+    table, A = matgen rands seed 0; the pivot vector equals LAPACK dgetrf's
+    (committed fixture) and the solves for 1 and 64 right-hand sides meet the
+    scaled residual bound."""
+    n = 32768
+    say("dgesv n=%d nrhs 1+64, pivots vs LAPACK" % n)
+    A = dev_rands(0, n, n)
+    A0 = A.clone()
+    B = dev_rands(1, n, 65)
+    B0 = B.clone()
+    info, ipiv = slate_amd.gesv(A, B)
+    say("  info %d (%.1f ms)" % (info, slate_amd.last_time_ms()))
+    assert info == 0
+    ref = np.load(os.path.join(GOLD, "ipiv_lapack_n32768.npy")).astype(np.int64)
+    bad = np.nonzero(ipiv != ref)[0]
+    say("  pivots differing from LAPACK: %d%s" % (bad.size, "" if bad.size == 0 else " (first at %d)" % bad[0]))
+    assert bad.size == 0
+    for cols in (slice(0, 1), slice(1, 65)):
+        r = residual_dev(A0, B[:, cols], B0[:, cols])
+        say("  nrhs %d scaled residual %.3f" % (cols.stop - cols.start, r))
+        assert r <= 3.0
+
+
+def test_posv_n32768(gpu):
+    """C2 Cholesky at full size: SPD A = G G^T / n + I, G = matgen randn."""
+    n = 32768
+    say("dposv n=%d" % n)
+    G = slate_amd.empty_colmajor(n, n, torch.float64)
+    slate_amd.generate(G, slate_amd.DIST_RANDN, 5)
+    S = slate_amd.colmajor(torch.addmm(torch.eye(n, dtype=torch.float64, device="cuda"), G, G.t().contiguous(),
+                                       alpha=1.0 / n))
+    del G
+    B = dev_rands(6, n, 1)
+    S0, B0 = S.clone(), B.clone()
+    info = slate_amd.posv(S, B)
+    r = residual_dev(S0, B, B0)
+    say("  info %d scaled residual %.3f (%.1f ms)" % (info, r, slate_amd.last_time_ms()))
+    assert info == 0 and r <= 3.0
+
+
+@pytest.mark.parametrize("driver", ["dsgesv_gmres", "dsposv", "dsgesv"])
+def test_mixed_kappa_1e6_n32768(gpu, driver):
+    """C3 at full size: A = U diag(sigma) V^T (SPD: V diag(sigma) V^T), sigma
+    geometric from 1 to 1e-6, n = 32768, float factor.  GMRES-IR (general)
+    and classic IR (SPD) must converge; classic IR on the general matrix
+    (kappa eps_float ~ 0.06 at a growth of n) may instead run to the cap and
+    take the FP64 fallback, which then must solve it.  Either way the
+    scaled residual meets the bound and, on convergence, A is untouched."""
+    n, cond = 32768, 1e6
+    sym = driver == "dsposv"
+    say("%s kappa=%g n=%d" % (driver, cond, n))
+    A = slate_amd.empty_colmajor(n, n, torch.float64)
+    slate_amd.generate_svd(A, slate_amd.SIGMA_GEO, cond, 71, sym=sym)
+    B = dev_rands(72, n, 1)
+    X = slate_amd.empty_colmajor(n, 1, torch.float64)
+    A0 = A.clone()
+    if sym:
+        info, it = slate_amd.posv_mixed(A, B, X)
+    elif driver == "dsgesv_gmres":
+        info, it, _ = slate_amd.gesv_mixed_gmres(A, B, X)
+    else:
+        info, it, _ = slate_amd.gesv_mixed(A, B, X)
+    r = residual_dev(A0, X, B)
+    say("  info %d iter %d scaled residual %.3f (%.1f ms)" % (info, it, r, slate_amd.last_time_ms()))
+    assert info == 0 and r <= 3.0
+    if driver == "dsgesv":
+        assert 0 <= it <= 30 or it == -31
+    else:
+        assert 0 <= it <= 30
+    if it >= 0:
+        assert torch.equal(A, A0)
 
 
 def test_getrf_block_above_one_pass_swap_limit(gpu):
