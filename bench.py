@@ -198,6 +198,10 @@ def main():
     ap.add_argument("--factor-precision", default="bf16x2", choices=sorted(PRECS),
                     help="factor precision of the mixed solve (bf16x3 = float-accurate split-BF16)")
     ap.add_argument("--classic", action="store_true", help="classic IR instead of GMRES-IR")
+    ap.add_argument("--matrix", default="rands", choices=["rands", "geo"],
+                    help="rands: matgen uniform(-1,1) seed 0; geo: U diag(sigma) V^T with Householder U, V "
+                         "and sigma geometric from 1 to 1/--cond")
+    ap.add_argument("--cond", type=float, default=1e6, help="condition number of --matrix geo")
     ap.add_argument("--workload", default="gesv", choices=["gesv", "batched"],
                     help="batched: many uneven systems in one launch instead of the headline solve")
     ap.add_argument("--batch", type=int, default=4096)
@@ -233,9 +237,15 @@ def main():
     opts = S.Options(block_size=args.nb, factor_precision=prec_code)
     gmres = not args.classic
     solve = S.gesv_mixed_gmres if gmres else S.gesv_mixed
-    log("bench: generating A (%d x %d, matgen rands seed 0) and b (seed 1) on the GPU" % (n, n))
     A = S.empty_colmajor(n, n, torch.float64)
-    S.generate(A, S.DIST_RANDS, 0)
+    if args.matrix == "geo":
+        mat_desc = "U diag(sigma) V^T, Householder U/V seed 11, sigma geometric 1..1/%g" % args.cond
+        log("bench: generating A (%d x %d, %s) and b (rands seed 1) on the GPU" % (n, n, mat_desc))
+        S.generate_svd(A, S.SIGMA_GEO, args.cond, 11, sym=False)
+    else:
+        mat_desc = "matgen rands seed 0"
+        log("bench: generating A (%d x %d, %s) and b (seed 1) on the GPU" % (n, n, mat_desc))
+        S.generate(A, S.DIST_RANDS, 0)
     B = S.empty_colmajor(n, 1, torch.float64)
     S.generate(B, S.DIST_RANDS, 1)
     X = S.empty_colmajor(n, 1, torch.float64)
@@ -259,6 +269,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gemm_ms, gemm_flops, launches = S.profile_read()
+    per_kernel = {k: S.profile_read_kernel(k) for k in (S.PROF_LT, S.PROF_SP, S.PROF_OTHER)}
     S.profile_enable(False)
     ms_per_step = 1e3 * wall / args.steps
     flops = 2.0 / 3.0 * n ** 3
@@ -279,7 +290,7 @@ def main():
             B64.copy_(B)
             torch.cuda.synchronize()
             t = time.perf_counter()
-            info64, _ = S.gesv(A64, B64, opts64)
+            info64, ipiv64 = S.gesv(A64, B64, opts64)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t
             assert info64 == 0, "plain FP64 gesv: info %d" % info64
@@ -288,10 +299,15 @@ def main():
             log("bench: plain FP64 gesv %s %.1f ms" % ("warmup" if rep == 0 else "run %d" % rep, 1e3 * dt))
         t64 = sorted(ts)[len(ts) // 2]
         x64 = B64[:, 0].clone()
+        # 1-norm condition estimate of A from the FP64 factor (outside the timed region)
+        anorm1 = float(A.abs().sum(dim=0).max())
+        rcond = S.gecondest(A64, ipiv64, anorm1, "1")
+        log("bench: condition estimate kappa_1(A) ~ %.3g" % (1.0 / rcond if rcond > 0 else float("inf")))
         err64 = errors_dev(A, x64, B[:, 0], None)
         assert err64["scaled_residual"] <= 3.0, "plain FP64 gesv: scaled residual %g" % err64["scaled_residual"]
         fp64 = {"ms_median": round(1e3 * t64, 2), "ms_runs": [round(1e3 * v, 2) for v in ts],
-                "tflops": round(flops / t64 / 1e12, 3), "info": info64, **err64}
+                "tflops": round(flops / t64 / 1e12, 3), "info": info64, **err64,
+                "kappa_1_estimate": float("%.3e" % (1.0 / rcond)) if rcond > 0 else None}
         log("bench: plain FP64 gesv median %.1f ms, %.2f TFLOP/s, scaled residual %.3e"
             % (1e3 * t64, fp64["tflops"], err64["scaled_residual"]))
         del A64, B64
@@ -301,26 +317,39 @@ def main():
     assert err_mixed["scaled_residual"] <= 3.0, "mixed solve: scaled residual %g" % err_mixed["scaled_residual"]
 
     # Dominant kernel: the trailing update.  roofline.achieved counts the
-    # algorithmic 2mnk flops per launch; the MFMA issue rate (the bf16 products
-    # of the split form) is reported separately.
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    # algorithmic 2mnk flops per launch of that kernel; the MFMA issue rate
+    # (the bf16 products of the split form) is reported separately.  With one-
+    # and two-term splits the large steps run the 256 x 256 plane kernel and
+    # the small ones the 128 x 128 split-at-staging kernel: the roofline is
+    # the plane kernel's, the other's share is listed beside it.
     mfma_products = products if products else 1
-    if products:
-        peak, kname = PEAK["bf16"], ("gemm_sp_kernel<%d,1,3> trailing update (float operands split into bf16 "
-                                     "terms at LDS staging, v_mfma_f32_16x16x32_bf16, %d products per update)"
-                                     % ({1: 1, 3: 2, 6: 3}[products], products))
-    else:
-        peak, kname = PEAK["fp32"], "gemm_kernel<float,0,0,128,128,32,2,2,2,1,1> trailing update (v_mfma_f32_16x16x4f32)"
+    nterms = {1: 1, 3: 2, 6: 3}.get(products, 0)
+    names = {
+        S.PROF_LT: "gemm_lt_kernel<%d,1> trailing update (256 x 256 tiles over pre-split bf16 planes, direct "
+                   "global->LDS staging, v_mfma_f32_16x16x32_bf16, %d products per update)" % (nterms, products),
+        S.PROF_SP: "gemm_sp_kernel<%d,1,3> trailing update (128 x 128 tiles, float operands split into bf16 "
+                   "terms at LDS staging, v_mfma_f32_16x16x32_bf16, %d products per update)" % (nterms, products),
+        S.PROF_OTHER: "gemm_kernel<float,0,0,128,128,32,2,2,2,1,1> trailing update (v_mfma_f32_16x16x4f32)",
+    }
+    dom = max(per_kernel, key=lambda k: per_kernel[k][0])
+    k_ms, k_flops, k_launches = per_kernel[dom]
+    peak = PEAK["bf16"] if products else PEAK["fp32"]
+    kname = names[dom]
+    achieved = k_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
     kernel_key = kname.split(" ")[0]
     traffic, traffic_src = measured_traffic(kernel_key, args.factor_precision) if n == 65536 else (None, None)
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
         "frac": round(achieved / peak, 4), "traffic": traffic,
         "kernel": kname,
-        "launches": int(launches), "avg_launch_ms": round(gemm_ms / max(launches, 1), 4),
-        "algorithmic_flops_per_launch": round(gemm_flops / max(launches, 1), 1),
+        "launches": int(k_launches), "avg_launch_ms": round(k_ms / max(k_launches, 1), 4),
+        "algorithmic_flops_per_launch": round(k_flops / max(k_launches, 1), 1),
         "mfma_issue_tflops": round(achieved * mfma_products, 3),
         "mfma_issue_frac": round(achieved * mfma_products / peak, 4),
+        "all_trailing_updates": {"ms": round(gemm_ms, 2), "launches": int(launches),
+                                 "tflops": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 3) if gemm_ms > 0 else 0.0},
+        "other_kernels": {names[k].split(" ")[0]: {"ms": round(v[0], 2), "launches": int(v[2])}
+                          for k, v in per_kernel.items() if k != dom and v[2] > 0},
     }
     if traffic_src:
         roofline["traffic_source"] = traffic_src
@@ -332,10 +361,10 @@ def main():
         "dtype": ("f32 factor" if not products else "f32 factor, %s updates" % args.factor_precision)
                  + " + f64 refinement", "data": "synthetic",
         "config": {
-            "workload": "%s (float LU, %s updates, + FP64 %s) n=%d nrhs=1, A = matgen rands seed 0, "
+            "workload": "%s (float LU, %s updates, + FP64 %s) n=%d nrhs=1, A = %s, "
                         "b = rands seed 1, already in HBM"
                         % ("gesv_mixed_gmres" if gmres else "gesv_mixed", args.factor_precision,
-                           "GMRES-IR" if gmres else "classic IR", n),
+                           "GMRES-IR" if gmres else "classic IR", n, mat_desc),
             "n": n, "nrhs": 1, "nb": args.nb or "auto", "factor_precision": args.factor_precision,
             "refinement": "GMRES-IR" if gmres else "classic IR", "iters": iters,
             **err_mixed,

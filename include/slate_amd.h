@@ -178,6 +178,13 @@ int slate_amd_profile_enable(slate_amd_context ctx, int on);
 int slate_amd_graph_stats(slate_amd_context ctx, int64_t* captured, int64_t* failed, int64_t* replays);
 int slate_amd_profile_read(slate_amd_context ctx, double* gemm_ms, double* gemm_flops,
                            int64_t* launches);
+/* The same accumulated per trailing-update kernel: SLATE_AMD_PROF_LT (the
+ * 256 x 256-tile kernel over pre-split bf16 planes), SLATE_AMD_PROF_SP (the
+ * 128 x 128 kernel splitting float operands at LDS staging), or
+ * SLATE_AMD_PROF_OTHER (FP64 / FP32 / FP8 / unaligned forms). */
+enum { SLATE_AMD_PROF_OTHER = 0, SLATE_AMD_PROF_LT = 1, SLATE_AMD_PROF_SP = 2 };
+int slate_amd_profile_read_kernel(slate_amd_context ctx, int kernel, double* gemm_ms, double* gemm_flops,
+                                  int64_t* launches);
 
 /* ---- utilities on the path ---------------------------------------------- */
 /* slate::gemm(alpha, A, B, beta, C) for column-major device matrices,

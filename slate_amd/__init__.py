@@ -41,7 +41,7 @@ C_SYMBOLS = [
     "slate_amd_selector_features", "slate_amd_mlp_forward", "slate_amd_mlp_train_step",
     "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb", "slate_amd_dgesv_batched",
     "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_sgemm_fp8", "slate_amd_profile_enable",
-    "slate_amd_profile_read", "slate_amd_dgesv_rbt", "slate_amd_drbt_generate", "slate_amd_dgerbt",
+    "slate_amd_profile_read", "slate_amd_profile_read_kernel", "slate_amd_dgesv_rbt", "slate_amd_drbt_generate", "slate_amd_dgerbt",
     "slate_amd_dgerbt_rhs", "slate_amd_dgetrf_tiles", "slate_amd_dgetrs_tiles", "slate_amd_dgesv_tiles",
     "slate_amd_dpotrf_tiles", "slate_amd_dpotrs_tiles", "slate_amd_dposv_tiles", "slate_amd_dsgesv_tiles",
     "slate_amd_dsgesv_gmres_tiles", "slate_amd_dsposv_tiles", "slate_amd_dsposv_gmres_tiles",
@@ -151,6 +151,7 @@ def lib():
             "slate_amd_profile_enable": (i32, [vp, i32]),
             "slate_amd_graph_stats": (i32, [vp, pi64, pi64, pi64]),
             "slate_amd_profile_read": (i32, [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl), pi64]),
+            "slate_amd_profile_read_kernel": (i32, [vp, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), pi64]),
             "slate_amd_dgemm": (i32, [vp, i64, i64, i64, dbl, vp, i64, vp, i64, dbl, vp, i64]),
             "slate_amd_sgemm": (i32, [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64,
                                       ctypes.c_float, vp, i64]),
@@ -463,6 +464,19 @@ def profile_read(device=0):
     _check(lib().slate_amd_profile_read(context(device), ctypes.byref(ms), ctypes.byref(fl),
                                         ctypes.byref(n)))
     return ms.value, fl.value, n.value
+
+
+PROF_OTHER, PROF_LT, PROF_SP = 0, 1, 2
+
+
+def profile_read_kernel(kernel, device=0):
+    """(gemm_ms, gemm_flops, launches) of one trailing-update kernel
+    (PROF_LT: 256 x 256 tiles over bf16 planes, PROF_SP: 128 x 128 split at
+    staging, PROF_OTHER: the rest) accumulated since profile_enable."""
+    ms, fl, ln = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int64(0)
+    _check(lib().slate_amd_profile_read_kernel(context(device), int(kernel), ctypes.byref(ms), ctypes.byref(fl),
+                                               ctypes.byref(ln)))
+    return ms.value, fl.value, ln.value
 
 
 def norm_inf(A, sym=False):

@@ -98,9 +98,13 @@ struct Context {
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
     std::vector<double> prof_flops_pending;
+    std::vector<int> prof_kind_pending;
     size_t prof_used = 0;
     double prof_ms = 0, prof_flops = 0;
     int64_t prof_launches = 0;
+    // the same split by kernel (SLATE_AMD_PROF_*: other, 256 x 256 planes, 128 x 128 split at staging)
+    double prof_ms_k[3] = {0, 0, 0}, prof_flops_k[3] = {0, 0, 0};
+    int64_t prof_launches_k[3] = {0, 0, 0};
 };
 
 // Event pair around one trailing-update GEMM on stream s (profiling only).
@@ -114,11 +118,12 @@ static void prof_begin(Context& c, hipStream_t s)
     }
     SA_HIP(hipEventRecord(c.prof_ev[c.prof_used], s));
 }
-static void prof_end(Context& c, hipStream_t s, double flops)
+static void prof_end(Context& c, hipStream_t s, double flops, int kind)
 {
     if (!c.prof) return;
     SA_HIP(hipEventRecord(c.prof_ev[c.prof_used + 1], s));
     c.prof_flops_pending.push_back(flops);
+    c.prof_kind_pending.push_back(kind);
     c.prof_used += 2;
 }
 // Fold the recorded pairs into the totals (after the streams are synchronised).
@@ -131,9 +136,14 @@ static void prof_collect(Context& c)
         c.prof_ms += ms;
         c.prof_flops += c.prof_flops_pending[i / 2];
         c.prof_launches += 1;
+        const int k = c.prof_kind_pending[i / 2];
+        c.prof_ms_k[k] += ms;
+        c.prof_flops_k[k] += c.prof_flops_pending[i / 2];
+        c.prof_launches_k[k] += 1;
     }
     c.prof_used = 0;
     c.prof_flops_pending.clear();
+    c.prof_kind_pending.clear();
 }
 
 struct Opts {
@@ -563,10 +573,25 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         c.bfu0.ensure(ub * size_t(uplane0));
         c.bfu1.ensure(ub * size_t(uplane1));
     }
+    // One- and two-term splits: trailing updates of at least LT_MIN 256 x 256
+    // tiles read pre-split planes instead (gemm_bf16_lt: direct global->LDS
+    // staging, no split work in the GEMM); the step's L panel is split into
+    // its slot on s1 and the U row block on s0 before the update.  Both
+    // kernels accumulate each element identically, so results do not depend
+    // on which one a step uses.
+    constexpr int64_t LT_MIN = 512;
+    auto use_lt = [&](int64_t k) -> bool {
+        if (!direct || (ns != 1 && ns != 2)) return false;
+        const int64_t k0 = k * nb, kb = std::min(nb, n - k0), r = n - k0 - kb;
+        const int64_t nb1 = std::min(nb, r);
+        const int64_t ncols = (la && r > nb1) ? r - nb1 : r;
+        return r > 0 && ncols > 0 && gemm_lt_tiles(r, ncols) >= LT_MIN;
+    };
+    if (ns > 0 && direct && use_lt(0)) c.bfl.ensure(sizeof(__bf16) * size_t(2 * ns) * size_t(lplane));
     auto lslot = [&](int64_t k) { return c.bfl.as<__bf16>() + (k % 2) * ns * lplane; };
     auto split_panel = [&](int64_t k, int64_t k0, int64_t kb, hipStream_t st) {
         if constexpr (std::is_same<T, float>::value) {
-            if (ns > 0 && !direct)
+            if (ns > 0 && (!direct || use_lt(k)))
                 split_rows(ns, A + (k0 + kb) * ld + k0, ld, n - k0 - kb, kb, lslot(k), Kp, lplane, st);
         }
     };
@@ -599,10 +624,10 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         const int64_t c1 = k0 + kb;
         if (ncols <= 0 || n - c1 <= 0) return;
         const double flops = 2.0 * double(n - c1) * double(ncols) * double(kb);
-        auto timed = [&](auto&& launch) {
+        auto timed = [&](int kind, auto&& launch) {
             if (role == 1) prof_begin(c, st);
             launch();
-            if (role == 1) prof_end(c, st, flops);
+            if (role == 1) prof_end(c, st, flops, kind);
         };
         if constexpr (std::is_same<T, float>::value) {
             if (f8) {
@@ -610,14 +635,21 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
                 const int64_t pu = lookahead_block ? p8u1 : p8l;
                 int* su = c.f8s.as<int>() + (lookahead_block ? 3 : 2) * t8;
                 quant_fp8_cols_t(f8, A + k0 * ld + cb, ld, kb, ncols, u8, Kp8, pu, su, st);
-                timed([&] { gemm_fp8(f8, n - c1, ncols, Kp8, l8slot(k), p8l, l8scale(k), u8, pu, su,
+                timed(SLATE_AMD_PROF_OTHER, [&] { gemm_fp8(f8, n - c1, ncols, Kp8, l8slot(k), p8l, l8scale(k), u8, pu, su,
                                      A + c1 * ld + cb, ld, -1.0f, st, role, 0); });
+                return;
+            }
+            if (ns > 0 && direct && !lookahead_block && use_lt(k)) {
+                __bf16* uh = c.bfu0.as<__bf16>();
+                split_cols_t(ns, A + k0 * ld + cb, ld, kb, ncols, uh, Kp, uplane0, st);
+                timed(SLATE_AMD_PROF_LT, [&] { gemm_bf16_lt(ns, n - c1, ncols, Kp, lslot(k), Kp, lplane, uh, Kp, uplane0,
+                                         A + c1 * ld + cb, ld, -1.0f, st, role, 0); });
                 return;
             }
             if (ns > 0 && direct) {
                 float* ut = lookahead_block ? c.bfu1.as<float>() : c.bfu0.as<float>();
                 transpose_f32(A + k0 * ld + cb, ld, kb, ncols, ut, Kp, st);
-                timed([&] { gemm_s32(ns, n - c1, ncols, kb, A + c1 * ld + k0, ld, ut, Kp, A + c1 * ld + cb, ld,
+                timed(SLATE_AMD_PROF_SP, [&] { gemm_s32(ns, n - c1, ncols, kb, A + c1 * ld + k0, ld, ut, Kp, A + c1 * ld + cb, ld,
                                      -1.0f, st, role, 0); });
                 return;
             }
@@ -625,7 +657,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
                 __bf16* uh = lookahead_block ? c.bfu1.as<__bf16>() : c.bfu0.as<__bf16>();
                 const int64_t up = lookahead_block ? uplane1 : uplane0;
                 split_cols_t(ns, A + k0 * ld + cb, ld, kb, ncols, uh, Kp, up, st);
-                timed([&] { gemm_bf16s(ns, n - c1, ncols, Kp, lslot(k), Kp, lplane, uh, Kp, up,
+                timed(SLATE_AMD_PROF_OTHER, [&] { gemm_bf16s(ns, n - c1, ncols, Kp, lslot(k), Kp, lplane, uh, Kp, up,
                                        A + c1 * ld + cb, ld, -1.0f, st, role); });
                 return;
             }
@@ -636,7 +668,7 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
         g.B = A + k0 * ld + cb; g.ldb = ld;
         g.C = A + c1 * ld + cb; g.ldc = ld;
         g.alpha = -1.0; g.beta = 1.0; g.uplo = 0;
-        timed([&] { gemm<T>(g, A_KC, B_NC, st, role); });
+        timed(SLATE_AMD_PROF_OTHER, [&] { gemm<T>(g, A_KC, B_NC, st, role); });
     };
     const int64_t plan_stride = 4 * nb + 8;
     c.plans.ensure(sizeof(int64_t) * size_t(plan_stride) * size_t(nsteps + 1));
@@ -2506,7 +2538,8 @@ int slate_amd_profile_enable(slate_amd_context ctx, int on)
     return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
         c.prof = on != 0;
         c.prof_ms = 0; c.prof_flops = 0; c.prof_launches = 0;
-        c.prof_used = 0; c.prof_flops_pending.clear();
+        c.prof_used = 0; c.prof_flops_pending.clear(); c.prof_kind_pending.clear();
+        for (int k = 0; k < 3; ++k) { c.prof_ms_k[k] = 0; c.prof_flops_k[k] = 0; c.prof_launches_k[k] = 0; }
         return 0;
     }));
 }
@@ -2517,6 +2550,18 @@ int slate_amd_profile_read(slate_amd_context ctx, double* gemm_ms, double* gemm_
         if (gemm_ms) *gemm_ms = c.prof_ms;
         if (gemm_flops) *gemm_flops = c.prof_flops;
         if (launches) *launches = c.prof_launches;
+        return 0;
+    }));
+}
+
+int slate_amd_profile_read_kernel(slate_amd_context ctx, int kernel, double* gemm_ms, double* gemm_flops,
+                                  int64_t* launches)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(kernel >= 0 && kernel < 3, -2, "profile_read_kernel: unknown kernel");
+        if (gemm_ms) *gemm_ms = c.prof_ms_k[kernel];
+        if (gemm_flops) *gemm_flops = c.prof_flops_k[kernel];
+        if (launches) *launches = c.prof_launches_k[kernel];
         return 0;
     }));
 }

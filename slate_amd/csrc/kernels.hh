@@ -32,6 +32,12 @@ void split_rows(int ns, const float* src, int64_t ld, int64_t M, int64_t K, __bf
                 int64_t plane, hipStream_t s);
 void split_cols_t(int ns, const float* src, int64_t ld, int64_t K, int64_t N, __bf16* dst, int64_t Kp,
                   int64_t plane, hipStream_t s);
+// 256 x 256-tile split-BF16 update over pre-split planes (direct global->LDS
+// staging), ns = 1 or 2; gemm_lt_tiles = its workgroup count without uplo.
+void gemm_bf16_lt(int ns, int64_t M, int64_t N, int64_t Kp, const __bf16* A, int64_t lda, int64_t planeA,
+                  const __bf16* B, int64_t ldb, int64_t planeB, float* C, int64_t ldc, float alpha,
+                  hipStream_t s, int role, int uplo);
+int64_t gemm_lt_tiles(int64_t M, int64_t N);
 void gemm_bf16s(int ns, int64_t M, int64_t N, int64_t Kp, const __bf16* A, int64_t lda, int64_t planeA,
                 const __bf16* B, int64_t ldb, int64_t planeB, float* C, int64_t ldc, float alpha,
                 hipStream_t s, int role = 0, int uplo = 0);

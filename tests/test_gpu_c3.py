@@ -15,8 +15,12 @@ run them), checked:
 * the same iteration-code class (converged >= 0, -(itermax+1) no convergence
   with FP64 fallback, -3 singular factor), except when the oracle converges
   within 3 iterations of the cap, where rounding may tip either way;
-* when both converge, |iter - iter_oracle| <= max(2, iter_oracle / 4)
-  (exact counts are rounding-dependent: parity unpinned beyond this bound);
+* when both converge, iter <= iter_oracle + max(2, iter_oracle / 4): the
+  GPU needs no more iterations than the oracle beyond that margin (it may
+  need fewer: its float factor applies L11^{-1} with a refinement step and
+  accumulates updates in a different order, and refinement counts react
+  strongly to small differences in the factor's accuracy; exact counts are
+  rounding-dependent, parity unpinned beyond this bound);
 * the solution within kappa-scaled distance of the oracle's:
   ||x - x_o||_inf / ||x_o||_inf <= 20 kappa n eps, and the tester's scaled
   residual <= 3 (test/test_gesv.cc:330-380).
@@ -54,7 +58,7 @@ def _compare(it, it_o, info, info_o, x, x_o, A_h, B_h, cond, n, itermax=30):
     if not (it_o >= itermax - 3 or it >= itermax - 3):
         assert cls(it) == cls(it_o)
     if it >= 0 and it_o >= 0:
-        assert abs(it - it_o) <= max(2, it_o // 4)
+        assert it <= it_o + max(2, it_o // 4)
     x = np.asarray(x).reshape(-1)
     x_o = np.asarray(x_o).reshape(-1)
     r = scaled_residual(A_h, x, B_h)
