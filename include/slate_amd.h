@@ -101,6 +101,18 @@ int64_t slate_amd_sgetrs(slate_amd_context ctx, int trans, int64_t n, int64_t nr
                          const float* dA, int64_t lda, const int64_t* ipiv,
                          float* dB, int64_t ldb, const slate_amd_options* opts);
 
+/* getrs for one right-hand side (B := A^{-1} B, NoTrans) that allocates
+ * nothing: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  Its workspace (device pivots, interchange plans,
+ * inverted diagonal blocks, look-back exchange words) is a static device
+ * array of 18 MiB that every GPU the library runs on holds until the process
+ * ends; n <= 16384 (larger n: -3).  Calls on one device share that array:
+ * calls that overlap on one device race.  ipiv: host, 1-based (getrf's).
+ * Same result, bit for bit, as slate_amd_dgetrs with nrhs = 1. */
+int64_t slate_amd_dgetrs_noalloc(slate_amd_context ctx, int64_t n, const double* dA, int64_t lda,
+                                 const int64_t* ipiv, double* dB);
+
 /* slate::gesv(A, Pivots, B, Options) — slate.hh:504-508, src/gesv.cc:85-109. */
 int64_t slate_amd_dgesv(slate_amd_context ctx, int64_t n, int64_t nrhs, double* dA, int64_t lda,
                         int64_t* ipiv, double* dB, int64_t ldb, const slate_amd_options* opts);

@@ -33,7 +33,7 @@ LU_PARTIALPIV, LU_NOPIV, LU_CALU = 0, 1, 2
 C_SYMBOLS = [
     "slate_amd_init", "slate_amd_finalize", "slate_amd_last_error", "slate_amd_options_init",
     "slate_amd_last_time_ms",
-    "slate_amd_dgetrf", "slate_amd_sgetrf", "slate_amd_dgetrs", "slate_amd_sgetrs",
+    "slate_amd_dgetrf", "slate_amd_sgetrf", "slate_amd_dgetrs", "slate_amd_sgetrs", "slate_amd_dgetrs_noalloc",
     "slate_amd_dgesv", "slate_amd_dpotrf", "slate_amd_spotrf", "slate_amd_dpotrs",
     "slate_amd_dposv", "slate_amd_dsgesv", "slate_amd_dsgesv_gmres", "slate_amd_dsposv",
     "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate", "slate_amd_generate_svd",
@@ -112,6 +112,7 @@ def lib():
             "slate_amd_sgetrf": (i64, [vp, i64, vp, i64, pi64, po]),
             "slate_amd_dgetrs": (i64, [vp, i32, i64, i64, vp, i64, pi64, vp, i64, po]),
             "slate_amd_sgetrs": (i64, [vp, i32, i64, i64, vp, i64, pi64, vp, i64, po]),
+            "slate_amd_dgetrs_noalloc": (i64, [vp, i64, vp, i64, pi64, vp]),
             "slate_amd_dgesv": (i64, [vp, i64, i64, vp, i64, pi64, vp, i64, po]),
             "slate_amd_dpotrf": (i64, [vp, i32, i64, vp, i64, po]),
             "slate_amd_spotrf": (i64, [vp, i32, i64, vp, i64, po]),

@@ -86,6 +86,7 @@ struct Context {
     std::vector<GraphKey> graph_failed;                         // keys whose capture failed
     int64_t graph_replays = 0;
     Buf tvw;                     // trsv look-back exchange words (+ progress word)
+    unsigned char* scratch = nullptr;   // this device's no-allocation scratch (noalloc.hip)
     Buf swaptmp;                 // two-pass row interchange workspace (trailing-update stream)
     Buf swaptmp1;                // the same for the lookahead block (tall blocks only)
     Buf ttri;                    // row-major copy of a column-major-view triangle (Cholesky solves)
@@ -2184,6 +2185,7 @@ int slate_amd_init(slate_amd_context* out, int device)
         }
         SA_HIP(hipEventCreate(&h->c.t_start));
         SA_HIP(hipEventCreate(&h->c.t_stop));
+        h->c.scratch = static_cast<unsigned char*>(sa::noalloc_scratch_setup(device));
         *out = h;
         return 0;
     }
@@ -2283,6 +2285,53 @@ int64_t slate_amd_dgetrs(slate_amd_context ctx, int trans, int64_t n, int64_t nr
         sa::Timer t(c);
         sa::getrs_cm<double>(c, trans, dA, lda, n, dip, dB, ldb, nrhs);
         t.stop();
+        return 0;
+    });
+}
+
+int64_t slate_amd_dgetrs_noalloc(slate_amd_context ctx, int64_t n, const double* dA, int64_t lda,
+                                 const int64_t* ipiv, double* dB)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        check_dims(n, lda);
+        if (n == 0) return 0;
+        SA_REQUIRE(n <= sa::NOALLOC_NMAX, -3, "dgetrs_noalloc: n above the scratch limit (16384)");
+        SA_REQUIRE(ipiv != nullptr && dB != nullptr, -5, "null ipiv or B");
+        for (int64_t i = 0; i < n; ++i)
+            SA_REQUIRE(ipiv[i] >= i + 1 && ipiv[i] <= n, -5, "ipiv entry out of range");
+        SA_REQUIRE(c.scratch != nullptr, -100, "dgetrs_noalloc: no scratch on this device");
+        SA_ENTRY_SYNC();
+        using L = sa::NoallocLayout;
+        unsigned char* sc = c.scratch;
+        int64_t* ip1 = reinterpret_cast<int64_t*>(sc + L::ipiv1);
+        int64_t* ip0 = reinterpret_cast<int64_t*>(sc + L::ipiv0);
+        int64_t* plan = reinterpret_cast<int64_t*>(sc + L::plan);
+        int* err = reinterpret_cast<int*>(sc + L::err);
+        uint64_t* words = reinterpret_cast<uint64_t*>(sc + L::tvw);
+        double* d0 = reinterpret_cast<double*>(sc + L::dinv);
+        double* d1 = d0 + L::dinv_per;
+        hipStream_t s = c.s0;
+        sa::Timer t(c);
+        // the previous call's exchange words and error flag are cleared here
+        sa::zero_u32(reinterpret_cast<unsigned*>(err), 64, s);
+        sa::zero_u32(reinterpret_cast<unsigned*>(words), int64_t(2 * L::tv_words), s);
+        SA_HIP(hipMemcpyAsync(ip1, ipiv, sizeof(int64_t) * size_t(n), hipMemcpyHostToDevice, s));
+        sa::ipiv_zero_based(ip1, ip0, n, s);
+        // B := P B (the LAPACK interchange sequence, 1024 pivots per plan), then
+        // the unit-lower and upper look-back solves in place
+        for (int64_t k1 = 0; k1 < n; k1 += 1024)
+            sa::laswp<double>(dB, 1, n, 0, 1, k1, std::min(n, k1 + 1024), ip0, plan, s);
+        sa::trsv_diag_inverse<double>(dA, lda, 1, 1, 1, n, d0, s);
+        sa::trsv_diag_inverse<double>(dA, lda, 1, 0, 0, n, d1, s);
+        sa::TvSync tv{words, words + (L::tv_words - 16), err, 1u};
+        sa::trsv_lookback<double>(dA, lda, 1, 1, n, d0, dB, tv, s);
+        tv.tag = 2u;
+        sa::trsv_lookback<double>(dA, lda, 1, 0, n, d1, dB, tv, s);
+        int e = 0;
+        SA_HIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
+        SA_HIP(hipStreamSynchronize(s));
+        t.stop();
+        SA_REQUIRE(e == 0, -102, "triangular solve: completion chain timed out");
         return 0;
     });
 }

@@ -264,4 +264,26 @@ void rbt_weights(int64_t seed, int64_t n, double* u, hipStream_t s);
 void zero_u32(unsigned* p, int64_t n, hipStream_t s);
 void zero_f64(double* p, int64_t n, hipStream_t s);
 
+//------------------------------------------------------------------------------
+// No-allocation scratch (noalloc.hip): one device array per GPU, NOALLOC_BYTES,
+// holding the workspace of slate_amd_dgetrs_noalloc for n <= NOALLOC_NMAX.
+constexpr int64_t NOALLOC_NMAX = 16384;
+constexpr size_t NOALLOC_BYTES = size_t(18) << 20;
+constexpr size_t noalloc_align(size_t b) { return (b + 255) & ~size_t(255); }
+struct NoallocLayout {
+    static constexpr size_t ipiv1 = 0;
+    static constexpr size_t ipiv0 = ipiv1 + noalloc_align(8 * size_t(NOALLOC_NMAX));
+    static constexpr size_t plan = ipiv0 + noalloc_align(8 * size_t(NOALLOC_NMAX));
+    static constexpr size_t plan_words = 4 * 2048 + 8;
+    static constexpr size_t err = plan + noalloc_align(8 * plan_words);
+    static constexpr size_t tvw = err + 256;
+    static constexpr size_t tv_words = 2 * size_t(NOALLOC_NMAX) + 32;
+    static constexpr size_t dinv = tvw + noalloc_align(8 * tv_words);
+    static constexpr size_t dinv_per = size_t(NOALLOC_NMAX) * 64;     // doubles per triangle (64-row blocks)
+    static constexpr size_t end = dinv + 2 * 8 * dinv_per;
+};
+static_assert(NoallocLayout::end <= NOALLOC_BYTES, "no-allocation scratch too small");
+void* noalloc_scratch_setup(int device);
+void ipiv_zero_based(const int64_t* ipiv1, int64_t* ipiv0, int64_t n, hipStream_t s);
+
 } // namespace sa
