@@ -1384,8 +1384,9 @@ static MixedState mixed_setup(Context& c, int64_t n, int64_t nrhs)
     st.dinvL = st.Alo + size_t(n) * size_t(n);
     st.dinvU = st.dinvL + dper;
     int64_t we = std::max<int64_t>(residual_work_elems(n, 4), ceildiv(n, 1024) * n + n + 64);
+    // seven pieces, each rounded up to 256 bytes by take()
     size_t vec_bytes = 2 * sizeof(int64_t) * n + 2 * sizeof(float) * n * nrhs + sizeof(double) * n * nrhs
-                       + sizeof(double) * we + sizeof(double) * (nrhs + 8) + 1024;
+                       + sizeof(double) * we + sizeof(double) * (nrhs + 8) + 7 * 256;
     c.vec.ensure(vec_bytes);
     char* p = static_cast<char*>(c.vec.p);
     auto take = [&](size_t b) { char* q = p; p += (b + 255) & ~size_t(255); return q; };

@@ -121,3 +121,25 @@ def test_norm_inf(gpu):
     S = spd(3, n)
     L = np.tril(S)
     assert abs(slate_amd.norm_inf(to_dev(L), sym=True) - np.abs(S).sum(axis=1).max()) < 1e-11
+
+
+@pytest.mark.parametrize("n", [129, 2049, 4097])
+def test_mixed_workspace_ragged_n(gpu, n):
+    """Ragged n where the refinement workspace's seven 256-byte-rounded
+    pieces used to extend past its allocation (the norms slot started beyond
+    the end for these sizes): every mixed driver and the selector's probe
+    path run and meet the residual bound."""
+    say("mixed drivers n=%d (workspace carving)" % n)
+    A = dev_rands(81, n, n)
+    B = dev_rands(82, n, 1)
+    A_h, B_h = to_host(A), to_host(B)
+    for name, run in (("dsgesv", lambda M, X: slate_amd.gesv_mixed(M, B, X)),
+                      ("dsgesv_gmres", lambda M, X: slate_amd.gesv_mixed_gmres(M, B, X)),
+                      ("auto probe kept", lambda M, X: slate_amd.gesv_auto(M, B, X, force=1)),
+                      ("auto refactor", lambda M, X: slate_amd.gesv_auto(M, B, X, force=4))):
+        X = slate_amd.empty_colmajor(n, 1, torch.float64)
+        out = run(A.clone(), X)          # a fallback would factor its A in place
+        info, it = out[0], out[1]
+        r = scaled_residual(A_h, to_host(X), B_h)
+        say("  %s: info %d iter %d scaled residual %.3g" % (name, info, it, r))
+        assert info == 0 and r <= 3.0
