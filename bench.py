@@ -39,23 +39,24 @@ def log(msg):
 
 def measured_traffic(kernel_key, factor_precision):
     """HBM bytes per launch of the trailing-update kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "trailing_update_hbm_traffic_%s.txt" % factor_precision)
-    if not os.path.exists(path):
-        return None, None
-    fields = {}
-    with open(path) as fh:
-        for line in fh:
-            if line.startswith("#") or not line.strip():
-                continue
-            k, _, v = line.strip().partition(" ")
-            fields[k] = v.strip()
-    if fields.get("kernel") != kernel_key or fields.get("factor_precision") != factor_precision:
-        return None, None
-    try:
-        return float(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
-    except (KeyError, ValueError):
-        return None, None
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload in profiles/
+    (the file whose `kernel` and `factor_precision` fields match; None if absent)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*hbm_traffic*.txt"))):
+        fields = {}
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("#") or not line.strip():
+                    continue
+                k, _, v = line.strip().partition(" ")
+                fields[k] = v.strip()
+        if fields.get("kernel") != kernel_key or fields.get("factor_precision") != factor_precision:
+            continue
+        try:
+            return float(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+        except (KeyError, ValueError):
+            return None, None
+    return None, None
 
 
 def cpu_baseline(n_cpu, gmres, gpu_c1=None):
@@ -198,6 +199,11 @@ def main():
     ap.add_argument("--factor-precision", default="bf16x2", choices=sorted(PRECS),
                     help="factor precision of the mixed solve (bf16x3 = float-accurate split-BF16)")
     ap.add_argument("--classic", action="store_true", help="classic IR instead of GMRES-IR")
+    sel_params = os.path.join(ROOT, "slate_amd", "selector_params.npy")
+    ap.add_argument("--form", default="auto" if os.path.exists(sel_params) else "fixed", choices=["auto", "fixed"],
+                    help="auto: the learned selector picks the factor precision and the refinement scheme "
+                         "(slate_amd_dsgesv_auto with the committed parameters; probe factor and features "
+                         "inside the timed step); fixed: --factor-precision with GMRES-IR or --classic")
     ap.add_argument("--matrix", default="rands", choices=["rands", "geo"],
                     help="rands: matgen uniform(-1,1) seed 0; geo: U diag(sigma) V^T with Householder U, V "
                          "and sigma geometric from 1 to 1/--cond")
@@ -236,7 +242,17 @@ def main():
     prec_code, products = PRECS[args.factor_precision]
     opts = S.Options(block_size=args.nb, factor_precision=prec_code)
     gmres = not args.classic
-    solve = S.gesv_mixed_gmres if gmres else S.gesv_mixed
+    choices = []
+    if args.form == "auto":
+        import numpy as np
+        params = np.load(sel_params, allow_pickle=False).astype(np.float32)
+
+        def solve(A_, B_, X_, o_):
+            info_, it_, ch_, ipiv_, _ = S.gesv_auto(A_, B_, X_, params=params, opts=o_)
+            choices.append(int(ch_))
+            return info_, it_, ipiv_
+    else:
+        solve = S.gesv_mixed_gmres if gmres else S.gesv_mixed
     A = S.empty_colmajor(n, n, torch.float64)
     if args.matrix == "geo":
         mat_desc = "U diag(sigma) V^T, Householder U/V seed 11, sigma geometric 1..1/%g" % args.cond
@@ -274,6 +290,16 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     flops = 2.0 / 3.0 * n ** 3
     value = flops / (ms_per_step * 1e-3) / 1e12
+    if args.form == "auto":
+        # the form the selector picked in the timed steps (all the same here)
+        ch = choices[-1]
+        assert all(c == ch for c in choices[-args.steps:]), "selector choice changed between steps"
+        pcode = S.CHOICE_PREC[ch]
+        gmres = bool(S.CHOICE_GMRES[ch])
+        pname = {v[0]: k for k, v in PRECS.items()}.get(pcode, "fp64")
+        args.factor_precision = pname
+        products = PRECS[pname][1] if pname in PRECS else 0
+        log("bench: selector picked choice %d (%s)" % (ch, S.CHOICES[ch]))
     log("bench: %s %.1f ms/step (device %.1f ms), %.2f TFLOP/s, iters %s"
         % ("gesv_mixed_gmres" if gmres else "gesv_mixed", ms_per_step, dev_ms / args.steps, value, iters))
 
@@ -361,12 +387,15 @@ def main():
         "dtype": ("f32 factor" if not products else "f32 factor, %s updates" % args.factor_precision)
                  + " + f64 refinement", "data": "synthetic",
         "config": {
-            "workload": "%s (float LU, %s updates, + FP64 %s) n=%d nrhs=1, A = %s, "
+            "workload": "%s%s (float LU, %s updates, + FP64 %s) n=%d nrhs=1, A = %s, "
                         "b = rands seed 1, already in HBM"
-                        % ("gesv_mixed_gmres" if gmres else "gesv_mixed", args.factor_precision,
+                        % ("MLP-selected " if args.form == "auto" else "",
+                           "gesv_mixed_gmres" if gmres else "gesv_mixed", args.factor_precision,
                            "GMRES-IR" if gmres else "classic IR", n, mat_desc),
             "n": n, "nrhs": 1, "nb": args.nb or "auto", "factor_precision": args.factor_precision,
             "refinement": "GMRES-IR" if gmres else "classic IR", "iters": iters,
+            "form": ("selector (slate_amd_dsgesv_auto, probe + features in the timed step): choice %d %s"
+                     % (choices[-1], S.CHOICES[choices[-1]])) if args.form == "auto" else "fixed",
             **err_mixed,
             "plain_fp64_gesv": fp64,
             "speedup_vs_plain_fp64": round(fp64["ms_median"] / ms_per_step, 3) if fp64 else None,

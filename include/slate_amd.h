@@ -266,6 +266,9 @@ enum { SLATE_AMD_TUNE_LU_FP64 = 0, SLATE_AMD_TUNE_LU_FP32 = 1, SLATE_AMD_TUNE_LU
        SLATE_AMD_TUNE_CHOL = 3 };
 int slate_amd_tune_set_nb(slate_amd_context ctx, int kind, int count, const int64_t* n_upper,
                           const int64_t* nb);
+/* The block size a factorization of this kind and size uses when
+ * Options::block_size is 0 (the installed table, else the built-in one). */
+int64_t slate_amd_tune_get_nb(slate_amd_context ctx, int kind, int64_t n);
 
 /* ---- learned solver selection (no counterpart in the reference) ---------
  * For callers that want the factorization precision and refinement scheme

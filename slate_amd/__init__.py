@@ -39,7 +39,7 @@ C_SYMBOLS = [
     "slate_amd_dsposv_gmres", "slate_amd_dnorm_inf", "slate_amd_generate", "slate_amd_generate_svd",
     "slate_amd_dgecondest", "slate_amd_dpocondest", "slate_amd_mlp_num_params",
     "slate_amd_selector_features", "slate_amd_mlp_forward", "slate_amd_mlp_train_step",
-    "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb", "slate_amd_dgesv_batched",
+    "slate_amd_dsgesv_auto", "slate_amd_tune_set_nb", "slate_amd_tune_get_nb", "slate_amd_dgesv_batched",
     "slate_amd_dgemm", "slate_amd_sgemm", "slate_amd_sgemm_bf16", "slate_amd_sgemm_fp8", "slate_amd_profile_enable",
     "slate_amd_profile_read", "slate_amd_profile_read_kernel", "slate_amd_dgesv_rbt", "slate_amd_drbt_generate", "slate_amd_dgerbt",
     "slate_amd_dgerbt_rhs", "slate_amd_dgetrf_tiles", "slate_amd_dgetrs_tiles", "slate_amd_dgesv_tiles",
@@ -133,6 +133,7 @@ def lib():
             "slate_amd_dpocondest": (i64, [vp, i32, i64, vp, i64, dbl, ctypes.POINTER(dbl)]),
             "slate_amd_mlp_num_params": (i32, []),
             "slate_amd_tune_set_nb": (i32, [vp, i32, i32, pi64, pi64]),
+            "slate_amd_tune_get_nb": (i64, [vp, i32, i64]),
             "slate_amd_dgesv_batched": (i64, [vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, po]),
             "slate_amd_selector_features": (i64, [vp, i64, vp, i64, po, vp]),
             "slate_amd_mlp_forward": (i32, [vp, vp, vp, i32, vp]),

@@ -8,7 +8,7 @@ the library (slate_amd_tune_set_nb) for every n up to the next grid point.
 Results persist as JSON (tuning.json beside this file by default) and are
 applied to each new context by slate_amd.context() when the file exists.
 
-Usage (on a GPU):  python -m slate_amd.autotune --sizes 4096,8192,16384,32768
+Usage (on a GPU):  python -m slate_amd.autotune [--sizes 2048,...,65536] [--out path]
 """
 import argparse
 import ctypes
@@ -45,35 +45,55 @@ def load(path=DEFAULT_PATH, device=0):
     return table
 
 
-def _time_one(kind, n, nb, reps=2):
+def _source(kind, n):
+    """The matrix a kind is timed on (kept on the device and copied per run):
+    matgen rands seed 0, or G^T G / n + I for Cholesky."""
     import torch
+    import slate_amd as S
+    A = S.empty_colmajor(n, n, torch.float64)
+    S.generate(A, S.DIST_RANDS, 0)
+    if kind == "chol":
+        A = S.colmajor(torch.addmm(torch.eye(n, dtype=torch.float64, device=A.device), A.t().contiguous(), A,
+                                   alpha=1.0 / n))
+    elif kind != "lu_fp64":
+        A = S.colmajor(A.float())
+    return A
+
+
+def _time_one(kind, A0, nb, reps=2):
     import slate_amd as S
     best = float("inf")
     for _ in range(reps):
-        A = S.empty_colmajor(n, n, torch.float64)
-        S.generate(A, S.DIST_RANDS, 0)
+        A = A0.clone()
         if kind == "chol":
-            A = S.colmajor(A.t() @ A / n + torch.eye(n, dtype=torch.float64, device=A.device))
             S.potrf(A, opts=S.Options(block_size=nb))
         elif kind == "lu_fp64":
             S.getrf(A, S.Options(block_size=nb))
         else:
-            A = S.colmajor(A.float())
-            prec = S.PREC_BF16X3 if kind == "lu_bf16" else S.PREC_FP32
+            prec = S.PREC_BF16X2 if kind == "lu_bf16" else S.PREC_FP32
             S.getrf(A, S.Options(block_size=nb, factor_precision=prec))
         best = min(best, S.last_time_ms())
         del A
     return best
 
 
-def tune(sizes, kinds=("lu_fp64", "lu_fp32", "lu_bf16", "chol"), log=None):
-    """Measure every (kind, n, nb); returns the table and the raw timings."""
+def tune(sizes, kinds=("lu_fp64", "lu_fp32", "lu_bf16", "chol"), log=None, chol_max=32768):
+    """Measure every (kind, n, nb); returns the table and the raw timings.
+    The split-BF16 kind is timed with the benchmark's two-term form (BF16X2);
+    sizes from 32768 on are timed once per nb, smaller ones twice (best);
+    Cholesky stops at chol_max (its SPD source is built by an FP64 product)."""
+    import torch
     table, raw = {}, {}
     for kind in kinds:
         rows = []
         for n in sorted(sizes):
+            if kind == "chol" and n > chol_max:
+                continue
             cands = [nb for nb in CANDIDATES if nb <= max(128, n // 4)]
-            times = {nb: _time_one(kind, n, nb) for nb in cands}
+            A0 = _source(kind, n)
+            times = {nb: _time_one(kind, A0, nb, reps=1 if n >= 32768 else 2) for nb in cands}
+            del A0
+            torch.cuda.empty_cache()
             best = min(times, key=times.get)
             raw["%s/%d" % (kind, n)] = times
             rows.append([n, best])
@@ -87,7 +107,7 @@ def tune(sizes, kinds=("lu_fp64", "lu_fp32", "lu_bf16", "chol"), log=None):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--sizes", default="2048,4096,8192,16384,32768")
+    ap.add_argument("--sizes", default="2048,4096,8192,16384,32768,65536")
     ap.add_argument("--kinds", default="lu_fp64,lu_fp32,lu_bf16,chol")
     ap.add_argument("--out", default=DEFAULT_PATH)
     args = ap.parse_args(argv)

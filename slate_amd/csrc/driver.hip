@@ -537,8 +537,14 @@ static void getrf_rm(Context& c, T* A, int64_t n, int64_t ld, const Opts& o, int
     // application once against an explicit unit-lower copy of L11 (same
     // slots), so U12 carries a triangular solve's error, not the explicit
     // inverse's.
+    // SLATE_AMD_REFINE_INV=1 / 0 turns it on / off for every float factor.
+    static const int refine_env = [] {
+        const char* e = std::getenv("SLATE_AMD_REFINE_INV");
+        return e ? std::atoi(e) : -1;
+    }();
     const bool refine = std::is_same<T, float>::value
-                        && (o.prec == SLATE_AMD_PREC_FP32 || o.prec == SLATE_AMD_PREC_BF16X3);
+                        && (refine_env >= 0 ? refine_env != 0
+                                            : (o.prec == SLATE_AMD_PREC_FP32 || o.prec == SLATE_AMD_PREC_BF16X3));
     c.sinv.ensure(sizeof(T) * size_t((refine ? 4 : 2) * nb * nb));
     c.stmp1.ensure(sizeof(T) * size_t(nb) * size_t(nb));
     c.ttmp.ensure(sizeof(T) * size_t(nb) * size_t(std::max<int64_t>(n, 1)));
@@ -2647,6 +2653,15 @@ int slate_amd_tune_set_nb(slate_amd_context ctx, int kind, int count, const int6
         c.nb_table[kind] = t;
         return 0;
     }));
+}
+
+int64_t slate_amd_tune_get_nb(slate_amd_context ctx, int kind, int64_t n)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(kind >= 0 && kind < 4, -2, "unknown tuning kind");
+        SA_REQUIRE(n >= 0, -3, "bad n");
+        return sa::choose_nb(c, kind, n, 0);
+    });
 }
 
 // Learned selection: features, MLP forward / training step, and the solve

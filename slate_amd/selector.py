@@ -6,10 +6,11 @@ factor, per-tile max |a| spread and an exponent histogram.  The MLP picks one
 of ``slate_amd.CHOICES``: keep the probe factor with classic IR or GMRES-IR,
 factor again in BF16X3, FP32 or FP8X2 with IR / GMRES-IR, or solve in plain
 FP64.  Its training data comes from the solver itself: every generated system
-is solved with each choice forced (``slate_amd.gesv_auto(force=c)``), and the
-cost of a choice is that call's measured device time, probe included, with a
-refinement that misses SLATE's tolerance paying for the FP64 fallback (as in
-src/gesv_mixed.cc); the label is the cheapest choice.  Training runs on the
+is solved with each choice (the probe-factor choices through
+``slate_amd.gesv_auto(force=c)``, the others as the probe's time plus their own
+solve), and the cost of a choice is its measured device time, probe included,
+with a refinement that misses SLATE's tolerance paying for the FP64 fallback
+(as in src/gesv_mixed.cc); the label is the cheapest choice.  Training runs on the
 GPU with the library's own kernels (``slate_amd.mlp_train_step``).
 
 Systems (the mixed-precision configurations): n log-uniform in
@@ -59,9 +60,16 @@ def sample_system(rng, nmin=NMIN, nmax=NMAX):
 
 
 def label_system(n, kappa, shape, seed):
-    """Solve one generated system with every choice forced; returns the
-    features, per-choice cost (device ms, probe included), iteration codes
-    and info, the probe's own time, and the label (cheapest choice)."""
+    """Solve one generated system with every choice; returns the features,
+    per-choice cost (device ms, probe included), iteration codes and info,
+    the probe's own time, and the label (cheapest choice).
+
+    Choices 0 and 1 keep the probe factor and run through the selector entry
+    itself (slate_amd.gesv_auto(force=c)).  Every other choice costs the probe
+    (which the selector always pays) plus that choice's own solve, which is
+    timed by running the solve directly (gesv_mixed / gesv_mixed_gmres with its
+    factor precision, or the plain FP64 gesv) instead of repeating the probe
+    inside gesv_auto."""
     import torch
     import slate_amd as S
     A0 = S.empty_colmajor(n, n, torch.float64)
@@ -75,17 +83,26 @@ def label_system(n, kappa, shape, seed):
     infos = np.zeros(K, dtype=np.int32)
     feat = None
     A = torch.empty_like(A0)
-    for c in range(K):
-        A.copy_(A0)
-        info, it, ch, _, f = S.gesv_auto(A, B, X, force=c)
-        assert ch == c
-        cost[c] = S.last_time_ms()
-        iters[c], infos[c] = it, info
-        if feat is None:
-            feat = f
     A.copy_(A0)
     S.selector_features(A)
     t_probe = S.last_time_ms()
+    for c in range(K):
+        A.copy_(A0)
+        if c in (0, 1):
+            info, it, ch, _, f = S.gesv_auto(A, B, X, force=c)
+            assert ch == c
+            cost[c] = S.last_time_ms()
+            feat = f
+        elif S.CHOICE_PREC[c] == S.PREC_FP64:
+            X.copy_(B)
+            info, _ = S.gesv(A, X)
+            it = 0
+            cost[c] = t_probe + S.last_time_ms()
+        else:
+            f_ = S.gesv_mixed_gmres if S.CHOICE_GMRES[c] else S.gesv_mixed
+            info, it, _ = f_(A, B, X, S.Options(factor_precision=S.CHOICE_PREC[c]))
+            cost[c] = t_probe + S.last_time_ms()
+        iters[c], infos[c] = it, info
     return {"n": n, "kappa": kappa, "shape": shape, "seed": seed, "features": feat, "cost": cost,
             "iters": iters, "info": infos, "t_probe": t_probe, "label": int(np.argmin(cost))}
 
