@@ -28,9 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "FP64-accurate gesv TFLOP/s, mixed+IR vs plain FP64 LU, n=65536, 1 GPU"
-PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}
-# factor precision -> (C-ABI value, bf16 products per update (0 = FP32 MFMA))
-PRECS = {"fp32": (0, 0), "bf16": (1, 1), "bf16x2": (2, 3), "bf16x3": (4, 6)}
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0, "fp8": 5000.0}
+# factor precision -> (C-ABI value, low-precision MFMA products per update (0 = FP32 MFMA))
+PRECS = {"fp32": (0, 0), "bf16": (1, 1), "bf16x2": (2, 3), "bf16x3": (4, 6), "fp8": (5, 1), "fp8x2": (6, 3)}
 
 
 def log(msg):
@@ -140,50 +140,74 @@ def errors_dev(A, x, b, x_ref):
 
 
 def bench_batched(args):
-    """Uneven batch: n uniform in [32, 1024], half general (kappa log-uniform
-    in [1e1, 1e8], geometric spectrum) and half SPD, one right-hand side each;
-    float factor + FP64 refinement vs the plain FP64 batched path."""
+    """Batched uneven solves: args.batch systems, n uniform in [32, 1024], half
+    general and half SPD, kappa log-uniform in [1e1, 1e8] (Householder
+    generator), one right-hand side each, in one persistent launch.  Timed
+    forms: the learned per-system choice (committed network; BF16 probe,
+    keep / FP32 / FP64), fixed FP32 factor + FP64 IR (SLATE's gesv_mixed
+    form), fixed BF16, and the plain FP64 batched path, which also serves as
+    the reference solution for the per-instance forward error."""
     import numpy as np
     import torch
     import slate_amd as S
-    rng = np.random.default_rng(2024)
+    from slate_amd import batched
     cnt = args.batch
-    ns = rng.integers(32, 1025, cnt).astype(np.int32)
-    kappa = 10.0 ** rng.uniform(1.0, 8.0, cnt)
-    log("bench: generating %d systems (sum n^2 = %.3g)" % (cnt, float(np.sum(ns.astype(np.float64) ** 2))))
-    offs = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)])
-    A = torch.empty(int(offs[-1]), dtype=torch.float64, device="cuda")
-    for i in range(cnt):
-        n = int(ns[i])
-        Ai = A[offs[i]:offs[i + 1]].view(n, n).t()          # column-major view
-        S.generate_svd(Ai, S.SIGMA_GEO, float(kappa[i]), 1000 + i, sym=bool(i % 2))
-    B = torch.empty(int(ns.sum()), dtype=torch.float64, device="cuda")
-    S.generate(B, S.DIST_RANDS, 7)
-    X = torch.zeros_like(B)
-    res = {}
-    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_FP32)):
-        S.gesv_batched(ns, A, B, X, prec)                    # warmup
+    log("bench: generating %d systems (n uniform [32, 1024], half SPD, kappa log-uniform [1e1, 1e8])" % cnt)
+    b = batched.make_batch(cnt, 2024, log=log)
+    ns, A, B = b["ns"], b["A"], b["B"]
+    params = S.batched_params()
+    forms = [("plain_fp64", lambda X: S.gesv_batched(ns, A, B, X, S.PREC_FP64) + (None,)),
+             ("fp32_ir", lambda X: S.gesv_batched(ns, A, B, X, S.PREC_FP32) + (None,)),
+             ("bf16_ir", lambda X: S.gesv_batched(ns, A, B, X, S.PREC_BF16) + (None,))]
+    if params is not None:
+        forms.append(("learned", lambda X: S.gesv_batched_auto(ns, A, B, X, params=params)[:4]))
+    res, xs = {}, {}
+    for name, run in forms:
+        X = torch.zeros_like(B)
+        for _ in range(args.warmup):
+            run(X)
         times = []
         for _ in range(args.steps):
             torch.cuda.synchronize()
             t = time.perf_counter()
-            iters, info, berr = S.gesv_batched(ns, A, B, X, prec)
+            iters, info, berr, ch = run(X)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t)
-        ms = 1e3 * min(times)
+        ms = 1e3 * float(np.median(times))
+        xs[name] = X
         res[name] = {"ms": round(ms, 3), "systems_per_s": round(cnt / (ms * 1e-3), 1),
-                     "max_berr": float(berr.max()), "median_berr": float(np.median(berr)),
-                     "info_nonzero": int(np.sum(info != 0)),
+                     "max_berr": float("%.3e" % berr.max()), "median_berr": float("%.3e" % np.median(berr)),
+                     "max_berr_over_n_eps": round(float(np.max(berr / (ns * np.finfo(np.float64).eps))), 3),
+                     "info_nonzero": int(np.sum(info != 0)), "fallbacks": int(np.sum(iters == -31)),
                      "iters_hist": np.bincount(np.clip(iters, -1, 31) + 1).tolist()}
-        log("bench: batched %s %.1f ms, %.0f systems/s, max berr %.3g" % (name, ms, cnt / (ms * 1e-3), berr.max()))
+        if ch is not None:
+            res[name]["choices"] = dict(zip(S.BATCHED_CHOICES, np.bincount(ch, minlength=3).tolist()))
+        log("bench: batched %s %.1f ms, %.0f systems/s, max berr %.3g"
+            % (name, ms, cnt / (ms * 1e-3), berr.max()))
+    # per-instance forward error of every form against the plain FP64 batched solution
+    x64 = xs["plain_fp64"].cpu().numpy()
+    offB = b["offB"]
+    for name in res:
+        if name == "plain_fp64":
+            continue
+        x = xs[name].cpu().numpy()
+        fe = np.array([np.abs(x[offB[i]:offB[i + 1]] - x64[offB[i]:offB[i + 1]]).max()
+                       / np.abs(x64[offB[i]:offB[i + 1]]).max() for i in range(cnt)])
+        res[name]["max_forward_error_vs_fp64"] = float("%.3e" % fe.max())
+        res[name]["max_forward_error_over_kappa"] = float("%.3e" % np.max(fe / b["kappa"]))
+    head = "learned" if "learned" in res else "fp32_ir"
     out = {
-        "metric": "batched uneven solves, systems/s (FP64-accurate)", "value": res["mixed"]["systems_per_s"],
-        "unit": "systems/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
-        "ms_per_step": res["mixed"]["ms"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 factor + f64 refinement", "data": "synthetic",
-        "config": {"workload": "%d systems, n uniform [32,1024], half general kappa log-uniform [1e1,1e8], "
-                               "half SPD, nrhs=1, one persistent launch" % cnt,
-                   "mixed": res["mixed"], "plain_fp64": res["fp64"]},
+        "metric": "batched uneven solves, systems/s (FP64-accurate)", "value": res[head]["systems_per_s"],
+        "unit": "systems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res[head]["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16/f32 factor (per system) + f64 refinement" if head == "learned"
+        else "f32 factor + f64 refinement", "data": "synthetic",
+        "config": {"workload": "%d systems, n uniform [32,1024], half general / half SPD, kappa log-uniform "
+                               "[1e1,1e8] (Householder generator), nrhs=1, one persistent launch; %s form"
+                               % (cnt, head),
+                   "tile_config": dict(zip(("nbf", "bt", "wgs_per_cu", "max_wgs_per_cu"), S.tune_get_batched())),
+                   "forms": res,
+                   "speedup_vs_plain_fp64_batched": round(res["plain_fp64"]["ms"] / res[head]["ms"], 3)},
     }
     print(json.dumps(out), flush=True)
 
@@ -357,9 +381,14 @@ def main():
                    "terms at LDS staging, v_mfma_f32_16x16x32_bf16, %d products per update)" % (nterms, products),
         S.PROF_OTHER: "gemm_kernel<float,0,0,128,128,32,2,2,2,1,1> trailing update (v_mfma_f32_16x16x4f32)",
     }
+    fp8 = args.factor_precision.startswith("fp8")
+    if fp8:
+        names[S.PROF_OTHER] = ("gemm_f8_kernel<%d,1> trailing update (e4m3 planes with a power-of-two scale per "
+                               "tile, v_mfma_scale_f32_16x16x128_f8f6f4, %d products per update)"
+                               % (nterms, products))
     dom = max(per_kernel, key=lambda k: per_kernel[k][0])
     k_ms, k_flops, k_launches = per_kernel[dom]
-    peak = PEAK["bf16"] if products else PEAK["fp32"]
+    peak = PEAK["fp8"] if fp8 else (PEAK["bf16"] if products else PEAK["fp32"])
     kname = names[dom]
     achieved = k_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
     kernel_key = kname.split(" ")[0]

@@ -248,14 +248,37 @@ int64_t slate_amd_dpocondest(slate_amd_context ctx, int uplo, int64_t n, const d
  * count systems A_i x_i = b_i, n[i] in [1, 1024] (host array), packed on the
  * device back to back: A_i column-major n_i x n_i at dA + sum_{k<i} n_k^2,
  * b_i and x_i at dB / dX + sum_{k<i} n_k.  One persistent launch (a work
- * queue over the systems, largest first).  factor_precision FP32: float
- * factor + FP64 refinement to SLATE's stopping rule with the FP64 fallback
- * (mode[i] = 1, host, may be NULL, sends system i straight to FP64); FP64:
- * plain FP64 LU.  Outputs (host): iters[i] as in dsgesv (0 for FP64),
- * info[i], berr[i] = ||b - A x||_inf / (||A||_inf ||x||_inf). */
+ * queue over the systems, largest first; options.use_graph = 1 runs it from
+ * a one-node HIP graph, rebuilt only when an argument changes).
+ * factor_precision FP32: float factor + FP64 refinement to SLATE's stopping
+ * rule with the FP64 fallback; BF16: the same with bf16-operand MFMA
+ * trailing updates (FP32 accumulation); FP64: plain FP64 LU.  mode (host,
+ * may be NULL): mode[i] = 1 sends system i straight to FP64.  Outputs
+ * (host): iters[i] as in dsgesv (0 for FP64), info[i], berr[i] =
+ * ||b - A x||_inf / (||A||_inf ||x||_inf).  Per-system steps mirror
+ * src/gesv.cc:85-109 and src/gesv_mixed.cc:106-290. */
 int64_t slate_amd_dgesv_batched(slate_amd_context ctx, int count, const int* n, const double* dA,
                                 const double* dB, double* dX, const int* mode, int factor_precision,
                                 int* iters, int* info, double* berr, const slate_amd_options* opts);
+/* The same with the factorization precision chosen per system by the
+ * selector network (SLATE_AMD_BATCHED_* choices): each system is factored in
+ * BF16 first (the probe), solved once, and the probe's normwise quantities
+ * (n, ||A|| ||x|| / ||b||, ||b - A x|| / (||A|| ||x||), ||b - A x|| / ||b||)
+ * go through the network, which keeps the probe factor and refines (0),
+ * factors again in FP32 and refines (1), or solves in FP64 (2); a
+ * refinement that misses the tolerance falls back to FP64.  params: host
+ * array of slate_amd_mlp_num_params() floats (the layout of the selector
+ * network); force in 0..2 runs that choice for every system instead (params
+ * may then be NULL).  choice (may be NULL): what ran per system;
+ * features (may be NULL): count x 24 floats; ms (may be NULL): each
+ * system's time on its workgroup. */
+enum { SLATE_AMD_BATCHED_BF16 = 0, SLATE_AMD_BATCHED_FP32 = 1, SLATE_AMD_BATCHED_FP64 = 2 };
+int64_t slate_amd_dgesv_batched_auto(slate_amd_context ctx, int count, const int* n, const double* dA,
+                                     const double* dB, double* dX, const float* params, int force, int* iters,
+                                     int* info, double* berr, int* choice, float* features, double* ms,
+                                     const slate_amd_options* opts);
+/* Graphs built and replayed by the batched entries under use_graph. */
+int slate_amd_batched_graph_stats(slate_amd_context ctx, int64_t* builds, int64_t* replays);
 
 /* ---- tuning ---------------------------------------------------------------
  * Block size nb per kind of factorization, used when options.block_size is 0
@@ -269,6 +292,13 @@ int slate_amd_tune_set_nb(slate_amd_context ctx, int kind, int count, const int6
 /* The block size a factorization of this kind and size uses when
  * Options::block_size is 0 (the installed table, else the built-in one). */
 int64_t slate_amd_tune_get_nb(slate_amd_context ctx, int kind, int64_t n);
+/* Tile configuration of the batched solver: float panel width nbf (16 or
+ * 32; the FP64 panel is half as wide), workgroup size bt (256, or 512 with
+ * nbf 16) and workgroups per CU (at most what LDS and registers allow, reported in
+ * max_wgs_per_cu).  Default 32 / 256 / 1 until the autotuner installs its
+ * measured choice. */
+int slate_amd_tune_set_batched(slate_amd_context ctx, int nbf, int bt, int wgs_per_cu);
+int slate_amd_tune_get_batched(slate_amd_context ctx, int* nbf, int* bt, int* wgs_per_cu, int* max_wgs_per_cu);
 
 /* ---- learned solver selection (no counterpart in the reference) ---------
  * For callers that want the factorization precision and refinement scheme

@@ -45,7 +45,8 @@ C_SYMBOLS = [
     "slate_amd_dgerbt_rhs", "slate_amd_dgetrf_tiles", "slate_amd_dgetrs_tiles", "slate_amd_dgesv_tiles",
     "slate_amd_dpotrf_tiles", "slate_amd_dpotrs_tiles", "slate_amd_dposv_tiles", "slate_amd_dsgesv_tiles",
     "slate_amd_dsgesv_gmres_tiles", "slate_amd_dsposv_tiles", "slate_amd_dsposv_gmres_tiles",
-    "slate_amd_graph_stats",
+    "slate_amd_graph_stats", "slate_amd_dgesv_batched_auto", "slate_amd_batched_graph_stats",
+    "slate_amd_tune_set_batched", "slate_amd_tune_get_batched",
 ]
 
 
@@ -135,6 +136,10 @@ def lib():
             "slate_amd_tune_set_nb": (i32, [vp, i32, i32, pi64, pi64]),
             "slate_amd_tune_get_nb": (i64, [vp, i32, i64]),
             "slate_amd_dgesv_batched": (i64, [vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, po]),
+            "slate_amd_dgesv_batched_auto": (i64, [vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, po]),
+            "slate_amd_batched_graph_stats": (i32, [vp, pi64, pi64]),
+            "slate_amd_tune_set_batched": (i32, [vp, i32, i32, i32]),
+            "slate_amd_tune_get_batched": (i32, [vp, pint, pint, pint, pint]),
             "slate_amd_selector_features": (i64, [vp, i64, vp, i64, po, vp]),
             "slate_amd_mlp_forward": (i32, [vp, vp, vp, i32, vp]),
             "slate_amd_mlp_train_step": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_float, i32,
@@ -563,7 +568,8 @@ def gesv_auto(A, B, X, params=None, opts=None, force=None):
 
 def gesv_batched(ns, A, B, X, factor_precision=PREC_FP32, mode=None, opts=None):
     """Batched uneven solves: ns (sizes), A/B/X packed float64 CUDA buffers
-    (A_i column-major back to back, then b_i / x_i).  Returns (iters, info,
+    (A_i column-major back to back, then b_i / x_i).  factor_precision FP32,
+    BF16 or FP64; mode[i] = 1 sends system i to FP64.  Returns (iters, info,
     berr) numpy arrays."""
     ns = np.ascontiguousarray(ns, dtype=np.int32)
     cnt = int(ns.size)
@@ -571,6 +577,7 @@ def gesv_batched(ns, A, B, X, factor_precision=PREC_FP32, mode=None, opts=None):
     info = np.zeros(cnt, dtype=np.int32)
     berr = np.zeros(cnt, dtype=np.float64)
     md = None if mode is None else np.ascontiguousarray(mode, dtype=np.int32)
+    _batched_check(ns, A, B, X)
     dev = A.device.index or 0
     vp = ctypes.c_void_p
     _check(lib().slate_amd_dgesv_batched(context(dev), cnt, ns.ctypes.data_as(vp), A.data_ptr(), B.data_ptr(),
@@ -578,6 +585,78 @@ def gesv_batched(ns, A, B, X, factor_precision=PREC_FP32, mode=None, opts=None):
                                          int(factor_precision), iters.ctypes.data_as(vp),
                                          info.ctypes.data_as(vp), berr.ctypes.data_as(vp), _opts(opts)))
     return iters, info, berr
+
+
+# choices of the batched selector (slate_amd_dgesv_batched_auto)
+BATCHED_CHOICES = ("BF16 + IR", "FP32 + IR", "FP64")
+BATCHED_DEFAULT = 1           # SLATE's gesv_mixed form: FP32 factor + classic IR
+BATCHED_PARAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_batched_params.npy")
+
+
+def batched_params():
+    """The committed batched-selector parameters (None if absent)."""
+    if not os.path.exists(BATCHED_PARAMS):
+        return None
+    return np.load(BATCHED_PARAMS, allow_pickle=False).astype(np.float32)
+
+
+def gesv_batched_auto(ns, A, B, X, params=None, force=None, opts=None, want_features=False, want_ms=False):
+    """Batched uneven solves with the precision chosen per system by the
+    selector network (params: its parameters, default the committed ones);
+    force (0..2, BATCHED_CHOICES) runs one choice for every system.  Returns
+    (iters, info, berr, choice, features or None, ms or None)."""
+    ns = np.ascontiguousarray(ns, dtype=np.int32)
+    cnt = int(ns.size)
+    _batched_check(ns, A, B, X)
+    if force is None:
+        if params is None:
+            params = batched_params()
+        if params is None:
+            raise SlateError("no batched selector parameters (pass params or force a choice)")
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        if params.size != mlp_num_params():
+            raise SlateError("selector parameters: expected %d floats" % mlp_num_params())
+    iters = np.zeros(cnt, dtype=np.int32)
+    info = np.zeros(cnt, dtype=np.int32)
+    berr = np.zeros(cnt, dtype=np.float64)
+    choice = np.zeros(cnt, dtype=np.int32)
+    feat = np.zeros((cnt, SELECTOR_FEATURES), dtype=np.float32) if want_features else None
+    ms = np.zeros(cnt, dtype=np.float64) if want_ms else None
+    vp = ctypes.c_void_p
+    dev = A.device.index or 0
+    _check(lib().slate_amd_dgesv_batched_auto(
+        context(dev), cnt, ns.ctypes.data_as(vp), A.data_ptr(), B.data_ptr(), X.data_ptr(),
+        None if force is not None else params.ctypes.data_as(vp), -1 if force is None else int(force),
+        iters.ctypes.data_as(vp), info.ctypes.data_as(vp), berr.ctypes.data_as(vp), choice.ctypes.data_as(vp),
+        None if feat is None else feat.ctypes.data_as(vp), None if ms is None else ms.ctypes.data_as(vp),
+        _opts(opts)))
+    return iters, info, berr, choice, feat, ms
+
+
+def _batched_check(ns, A, B, X):
+    import torch
+    for t in (A, B, X):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+            raise SlateError("batched: A, B, X must be contiguous float64 CUDA tensors")
+    if int(np.sum(ns.astype(np.int64) ** 2)) > A.numel() or int(np.sum(ns)) > min(B.numel(), X.numel()):
+        raise SlateError("batched: packed buffers smaller than the systems")
+
+
+def tune_set_batched(nbf, bt, wgs_per_cu, device=0):
+    _check(lib().slate_amd_tune_set_batched(context(device), int(nbf), int(bt), int(wgs_per_cu)))
+
+
+def tune_get_batched(device=0):
+    """(nbf, bt, wgs_per_cu, max_wgs_per_cu) of the batched solver."""
+    v = [ctypes.c_int(0) for _ in range(4)]
+    _check(lib().slate_amd_tune_get_batched(context(device), *[ctypes.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def batched_graph_stats(device=0):
+    b, r = ctypes.c_int64(0), ctypes.c_int64(0)
+    _check(lib().slate_amd_batched_graph_stats(context(device), ctypes.byref(b), ctypes.byref(r)))
+    return b.value, r.value
 
 
 def gecondest(LU, ipiv, anorm, norm="1"):

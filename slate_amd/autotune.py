@@ -8,7 +8,14 @@ the library (slate_amd_tune_set_nb) for every n up to the next grid point.
 Results persist as JSON (tuning.json beside this file by default) and are
 applied to each new context by slate_amd.context() when the file exists.
 
+The batched solver's tile configuration (float panel width, workgroup
+size, workgroups per CU) is chosen the same way: every valid configuration
+solves one batch of the standard batched workload in each fixed precision
+(BF16, FP32, FP64 factors), and the configuration with the smallest summed
+time is installed (slate_amd_tune_set_batched) and stored under "batched".
+
 Usage (on a GPU):  python -m slate_amd.autotune [--sizes 2048,...,65536] [--out path]
+                   python -m slate_amd.autotune --batched [--batch 1024]   (adds the batched entry)
 """
 import argparse
 import ctypes
@@ -36,13 +43,58 @@ def install(table, device=0):
                                                nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
 
 
+def install_batched(cfg, device=0):
+    import slate_amd as S
+    S.tune_set_batched(int(cfg["nbf"]), int(cfg["bt"]), int(cfg["wgs"]), device=device)
+
+
 def load(path=DEFAULT_PATH, device=0):
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        table = json.load(f)["table"]
+        data = json.load(f)
+    table = data["table"]
     install(table, device)
+    if "batched" in data:
+        install_batched(data["batched"], device)
     return table
+
+
+BATCHED_CONFIGS = ((16, 256), (32, 256), (16, 512))
+
+
+def tune_batched(count=1024, seed=77, log=None):
+    """Time every valid batched configuration on one generated batch; returns
+    the fastest {nbf, bt, wgs} and the timings (ms per precision)."""
+    import torch
+    import slate_amd as S
+    from . import batched
+    b = batched.make_batch(count, seed)
+    X = torch.zeros_like(b["B"])
+    saved = S.tune_get_batched()
+    raw = {}
+    try:
+        for nbf, bt in BATCHED_CONFIGS:
+            S.tune_set_batched(nbf, bt, 1)
+            mx = S.tune_get_batched()[3]
+            for wgs in range(1, min(mx, 2) + 1):
+                S.tune_set_batched(nbf, bt, wgs)
+                t = {}
+                for name, prec in (("bf16", S.PREC_BF16), ("fp32", S.PREC_FP32), ("fp64", S.PREC_FP64)):
+                    S.gesv_batched(b["ns"], b["A"], b["B"], X, prec)              # warm
+                    best = float("inf")
+                    for _ in range(2):
+                        S.gesv_batched(b["ns"], b["A"], b["B"], X, prec)
+                        best = min(best, S.last_time_ms())
+                    t[name] = round(best, 3)
+                raw["%d/%d/%d" % (nbf, bt, wgs)] = t
+                if log:
+                    log("autotune batched nbf %d bt %d wgs %d: %s" % (nbf, bt, wgs, t))
+    finally:
+        S.tune_set_batched(*saved[:3])
+    key = min(raw, key=lambda k: sum(raw[k].values()))
+    nbf, bt, wgs = (int(x) for x in key.split("/"))
+    return {"nbf": nbf, "bt": bt, "wgs": wgs, "systems": count}, raw
 
 
 def _source(kind, n):
@@ -110,11 +162,32 @@ def main(argv=None):
     ap.add_argument("--sizes", default="2048,4096,8192,16384,32768,65536")
     ap.add_argument("--kinds", default="lu_fp64,lu_fp32,lu_bf16,chol")
     ap.add_argument("--out", default=DEFAULT_PATH)
+    ap.add_argument("--batched", action="store_true", help="tune the batched solver and add its entry")
+    ap.add_argument("--batch", type=int, default=1024)
     args = ap.parse_args(argv)
+    if args.batched:
+        cfg, raw = tune_batched(args.batch, log=lambda m: print(m, file=sys.stderr, flush=True))
+        data = {}
+        if os.path.exists(args.out):
+            with open(args.out) as f:
+                data = json.load(f)
+        data["batched"] = cfg
+        data["batched_timings_ms"] = raw
+        with open(args.out, "w") as f:
+            json.dump(data, f, indent=1)
+        install_batched(cfg)
+        print(json.dumps({"batched": cfg}), flush=True)
+        return
     sizes = [int(x) for x in args.sizes.split(",")]
     table, raw = tune(sizes, args.kinds.split(","), log=lambda m: print(m, file=sys.stderr, flush=True))
+    data = {}
+    if os.path.exists(args.out):
+        with open(args.out) as f:
+            data = json.load(f)
+    data["table"] = table
+    data["timings_ms"] = raw
     with open(args.out, "w") as f:
-        json.dump({"table": table, "timings_ms": raw}, f, indent=1)
+        json.dump(data, f, indent=1)
     install(table)
     print(json.dumps({"table": table}), flush=True)
 

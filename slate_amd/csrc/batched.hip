@@ -1,179 +1,427 @@
 // Batched factor-and-solve of many small systems of uneven size (gfx950).
 //
 // One persistent launch: each workgroup takes the next system from a work
-// queue (an atomic counter over a largest-first order), copies it into its
-// own workspace in the factor precision T (row-major, so interchanges move
-// contiguous rows), factors it with partial pivoting, solves, and - when
-// T = float - refines in FP64 against the original matrix to SLATE's
-// stopping rule (src/gesv_mixed.cc:190-259, internal_util.hh:121-138), with
-// the FP64 factorization as the fallback when refinement misses the
-// tolerance.  The per-system steps mirror slate::gesv / gesv_mixed on a
-// single tile column; the batching (one launch for thousands of systems of
-// different n, no per-system launches or host round trips) is this
-// library's own design.
+// queue (an atomic ticket over a largest-first order), copies it into its
+// own workspace in row-major order (interchanges then move contiguous rows),
+// factors it with partial pivoting, solves, and refines in FP64 against the
+// original matrix to SLATE's stopping rule (src/gesv_mixed.cc:190-259,
+// internal_util.hh:121-138), with the FP64 factorization as the fallback when
+// the refinement misses the tolerance.  The per-system steps mirror
+// slate::gesv / gesv_mixed on a single tile column; the batching (one launch
+// for thousands of systems of different n, no per-system launches or host
+// round trips) is this library's own design.
+//
+// Per-system modes (BatchArgs::mode):
+//   0 FP32 factor + FP64 IR, 1 FP64 factor, 2 BF16 factor (FP32 storage,
+//   bf16-operand MFMA trailing updates with FP32 accumulation) + FP64 IR,
+//   3 learned choice, 4/5/6 the learned path with its choice forced to
+//   0/1/2 (labelling).
+// The learned path factors in BF16 first (the probe), solves once and forms
+// the FP64 residual (iteration 0 of the refinement), and feeds the probe's
+// normwise quantities to the selector network (the MLP of mlp.hip, run by one
+// wave): choice 0 keeps the probe factor and refines, 1 factors again in
+// FP32 and refines, 2 factors in FP64.
 //
 // Factorization: blocked right-looking LU with NB-column panels held in LDS
-// (up to 1024 rows x NB), pivot search by wave shuffles, the panel's
-// interchanges applied to the rest of the rows, U12 = L11^{-1} A12, and the
-// trailing update A22 -= L21 U12 register-blocked over the workgroup.
+// (up to 1024 rows, odd row stride so column reads are conflict free).  A
+// panel column costs two barriers: the pivot candidates of column j + 1 are
+// found while column j's rank-1 update is applied, and each thread reads the
+// pivot row and row j into registers before either is rewritten.  The
+// panel's interchanges are composed by one wave into at most 2 NB row moves
+// (destination, source), which every column left and right of the panel and
+// the permutation vector apply with all loads before any store (one memory
+// round trip per panel instead of one per swap); U12 = L11^{-1} A12 runs one
+// column per thread in registers, and A22 -= L21 U12 as 32 x 32 wave tiles of MFMA
+// (v_mfma_f32_16x16x4f32, v_mfma_f32_16x16x32_bf16 or v_mfma_f64_16x16x4f64)
+// with L21 read from the LDS panel.
 // Triangular solves: 64-row blocks, each diagonal block solved by one wave
 // (no barriers), the rows below updated by the whole workgroup.
+//
+// The ticket counter and a count of finished workgroups live in `queue`; the
+// last workgroup to finish zeroes both, so consecutive launches (and replays
+// of a graph holding the launch) need no separate reset.
 #include "common.hh"
 #include "kernels.hh"
 
+#include <cstring>
 #include <limits>
 
 namespace sa {
 
-static constexpr int BT = 256;              // threads per workgroup
 static constexpr int BMAXN = 1024;          // largest system
+static constexpr int SEL_F = 24, SEL_H = 64, SEL_K = 8;   // selector network (mlp.hip layout)
+static constexpr int SEL_CHOICES = 3;
 
-template <class T> struct BNB;
-template <> struct BNB<float> { static constexpr int v = 32; };
-template <> struct BNB<double> { static constexpr int v = 16; };
+// Panel of storage type T for a float panel width NBF: the FP64 panel is half
+// as wide, so both fit the same LDS allocation.
+template <class T, int NBF> struct Pan {
+    static constexpr int NB = sizeof(T) == 8 ? NBF / 2 : NBF;
+    static constexpr int LD = NB + 1;
+};
+template <int NBF> constexpr int pan_lds_bytes()
+{
+    return BMAXN * Pan<float, NBF>::LD * 4 > BMAXN * Pan<double, NBF>::LD * 8
+         ? BMAXN * Pan<float, NBF>::LD * 4 : BMAXN * Pan<double, NBF>::LD * 8;
+}
 
-__device__ inline void block_sync() { __syncthreads(); }
+__host__ __device__ inline int b_ldw(int n) { return (n + 3) & ~3; }
 
 template <class T>
 __device__ inline bool bt_better(T a, int ia, T b, int ib) { return a > b || (a == b && ia < ib); }
 
-// In-workgroup LU with partial pivoting of the row-major n x n matrix W
-// (leading dimension ldw).  piv[j] = 0-based row swapped with row j.
-// Returns info (first exactly-zero pivot, 1-based) in *info (thread 0 writes).
-template <class T>
-__device__ __noinline__ void wg_getrf(T* __restrict__ W, int n, int ldw, int* __restrict__ piv, int* info,
-                                      T* s_p, T* s_red, int* s_ired)
+// W (row-major, ldw) = A (column-major n x n, FP64), cast to T.  Wave tiles of
+// 64 rows x 32 columns: the column reads are coalesced, each lane then writes
+// 32 consecutive elements of its row.
+template <class T, int BT>
+__device__ void wg_load_rowmajor(const double* __restrict__ A, int n, T* __restrict__ W, int ldw)
 {
-    constexpr int NB = BNB<T>::v;
+    constexpr int NW = BT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ti = (n + 63) >> 6, tj = (n + 31) >> 5;
+    for (int t = wave; t < ti * tj; t += NW) {
+        const int i = (t % ti) * 64 + lane, j0 = (t / ti) * 32;
+        T v[32];
+        #pragma unroll
+        for (int c = 0; c < 32; ++c)
+            v[c] = (i < n && j0 + c < n) ? T(A[i + int64_t(j0 + c) * n]) : T(0);
+        if (i < n) {
+            T* w = W + int64_t(i) * ldw + j0;
+            if (j0 + 32 <= n) {
+                #pragma unroll
+                for (int c = 0; c < 32; ++c) w[c] = v[c];
+            }
+            else {
+                #pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (j0 + c < n) w[c] = v[c];
+            }
+        }
+    }
+}
+
+// Factor the mr x kb panel in LDS (row stride LD) with partial pivoting
+// (Tile_getrf.hh:197-366: largest magnitude, ties to the lower index, scaling
+// by the reciprocal unless the pivot is below the safe minimum).  s_piv[j]
+// receives the chosen local row of column j; *s_info the first exactly-zero
+// pivot (1-based global column k0 + j + 1) if none was recorded yet.
+template <class T, int NB, int LD, int BT>
+__device__ void wg_panel(T* __restrict__ s_p, int mr, int kb, int k0, int* s_piv, int* s_info, T* s_red,
+                         int* s_ired)
+{
+    constexpr int NW = BT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const T safe_min = std::numeric_limits<T>::min();
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        const int kb = min(NB, n - k0);
-        const int mr = n - k0;                           // panel rows
-        // panel -> LDS (row-major mr x NB)
-        for (int e = tid; e < mr * NB; e += BT) {
-            const int i = e / NB, c = e % NB;
-            s_p[e] = c < kb ? W[(k0 + i) * ldw + k0 + c] : T(0);
+    T ba = T(-1);
+    int bi = 0x7fffffff;
+    for (int i = tid; i < mr; i += BT) {
+        const T a = fabs(s_p[i * LD]);
+        if (bt_better(a, i, ba, bi)) { ba = a; bi = i; }
+    }
+    for (int j = 0; j < kb; ++j) {
+        #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const T oa = __shfl_xor(ba, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            if (bt_better(oa, oi, ba, bi)) { ba = oa; bi = oi; }
         }
-        block_sync();
-        for (int j = 0; j < kb; ++j) {
-            // argmax |s_p[i][j]| for i >= j
-            T ba = T(-1);
-            int bi = 0x7fffffff;
-            for (int i = j + tid; i < mr; i += BT) {
-                const T a = fabs(s_p[i * NB + j]);
+        if (lane == 0) { s_red[wave] = ba; s_ired[wave] = bi; }
+        __syncthreads();
+        ba = s_red[0];
+        bi = s_ired[0];
+        #pragma unroll
+        for (int w = 1; w < NW; ++w)
+            if (bt_better(s_red[w], s_ired[w], ba, bi)) { ba = s_red[w]; bi = s_ired[w]; }
+        const int p = (bi == 0x7fffffff) ? j : bi;
+        T prow[NB], rj[NB];
+        #pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            prow[c] = s_p[p * LD + c];
+            rj[c] = s_p[j * LD + c];
+        }
+        __syncthreads();                                   // rows p and j read by every thread
+        T pv = T(0);
+        #pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c == j) pv = prow[c];
+        if (tid == 0) {
+            s_piv[j] = p;
+            if (pv == T(0) && *s_info == 0) *s_info = k0 + j + 1;
+            if (p != j) {
+                #pragma unroll
+                for (int c = 0; c < NB; ++c) s_p[j * LD + c] = prow[c];
+            }
+        }
+        const bool tiny = fabs(pv) < safe_min;
+        const T inv = T(1) / pv;
+        ba = T(-1);
+        bi = 0x7fffffff;
+        for (int i = j + 1 + tid; i < mr; i += BT) {
+            const bool isp = (i == p);
+            T* row = s_p + i * LD;
+            // the row's values (old row j's for the pivot row's position),
+            // selected as values so no load goes through a mixed pointer
+            T cur[NB];
+            #pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const T rv = row[c];
+                cur[c] = isp ? rj[c] : rv;
+            }
+            T l = T(0);
+            #pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c == j) l = cur[c];
+            if (pv != T(0)) l = tiny ? l / pv : l * inv;
+            T nxt = T(0);
+            #pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < j) {
+                    if (isp) row[c] = cur[c];               // L part of old row j moves to row p
+                }
+                else if (c == j) {
+                    row[c] = l;
+                }
+                else if (c < kb) {
+                    const T v = cur[c] - l * prow[c];
+                    row[c] = v;
+                    if (c == j + 1) nxt = v;
+                }
+            }
+            if (j + 1 < kb) {
+                const T a = fabs(nxt);
                 if (bt_better(a, i, ba, bi)) { ba = a; bi = i; }
             }
+        }
+    }
+    __syncthreads();
+}
+
+// C (nr x nr, row stride ldw) -= L (nr x NB, LDS, row stride LD) * U (NB x nr,
+// row stride ldw), 32 x 32 wave tiles.
+template <class T, int NB, int LD, int BT, bool BFU>
+__device__ void wg_update(T* __restrict__ C, const T* __restrict__ U, int ldw, const T* __restrict__ L, int nr)
+{
+    constexpr int NW = BT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int fr = lane & 15, fg = lane >> 4;
+    const int tt = (nr + 31) >> 5;
+    for (int t = wave; t < tt * tt; t += NW) {
+        const int i0 = (t / tt) << 5, j0 = (t % tt) << 5;
+        if constexpr (sizeof(T) == 8) {
+            v4d acc[2][2];
             #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const T oa = __shfl_xor(ba, o, 64);
-                const int oi = __shfl_xor(bi, o, 64);
-                if (bt_better(oa, oi, ba, bi)) { ba = oa; bi = oi; }
-            }
-            if (lane == 0) { s_red[wave] = ba; s_ired[wave] = bi; }
-            block_sync();
-            ba = s_red[0]; bi = s_ired[0];
+            for (int a = 0; a < 2; ++a)
+                #pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
             #pragma unroll
-            for (int w = 1; w < BT / 64; ++w)
-                if (bt_better(s_red[w], s_ired[w], ba, bi)) { ba = s_red[w]; bi = s_ired[w]; }
-            const int p = (bi == 0x7fffffff) ? j : bi;
-            if (tid == 0) {
-                piv[k0 + j] = k0 + p;
-                if (s_p[p * NB + j] == T(0) && *info == 0) *info = k0 + j + 1;
+            for (int s = 0; s < NB / 4; ++s) {
+                const int k = 4 * s + fg;
+                double av[2], bv[2];
+                #pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const int i = i0 + 16 * a + fr;
+                    av[a] = i < nr ? L[i * LD + k] : 0.0;
+                }
+                #pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int jj = j0 + 16 * b + fr;
+                    bv[b] = jj < nr ? U[int64_t(k) * ldw + jj] : 0.0;
+                }
+                #pragma unroll
+                for (int a = 0; a < 2; ++a)
+                    #pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
             }
-            // swap rows j and p of the panel
-            if (p != j && tid < NB) {
-                const T t = s_p[j * NB + tid];
-                s_p[j * NB + tid] = s_p[p * NB + tid];
-                s_p[p * NB + tid] = t;
+            // register r holds D[4 r + l/16][l % 16]
+            #pragma unroll
+            for (int a = 0; a < 2; ++a)
+                #pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    #pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = i0 + 16 * a + 4 * r + fg, jj = j0 + 16 * b + fr;
+                        if (i < nr && jj < nr) C[int64_t(i) * ldw + jj] -= acc[a][b][r];
+                    }
+        }
+        else {
+            v4f acc[2][2];
+            #pragma unroll
+            for (int a = 0; a < 2; ++a)
+                #pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+            if constexpr (BFU) {
+                // lane supplies 8 consecutive k (8 * (l/16) + e); k >= NB is zero padding
+                v8bf av[2], bv[2];
+                #pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const int i = i0 + 16 * a + fr;
+                    #pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int k = 8 * fg + e;
+                        av[a][e] = __bf16((i < nr && k < NB) ? L[i * LD + k] : 0.f);
+                    }
+                }
+                #pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int jj = j0 + 16 * b + fr;
+                    #pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int k = 8 * fg + e;
+                        bv[b][e] = __bf16((jj < nr && k < NB) ? U[int64_t(k) * ldw + jj] : 0.f);
+                    }
+                }
+                #pragma unroll
+                for (int a = 0; a < 2; ++a)
+                    #pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
             }
-            block_sync();
-            const T pv = s_p[j * NB + j];
-            const bool tiny = fabs(pv) < safe_min;
-            const T inv = T(1) / pv;
-            for (int i = j + 1 + tid; i < mr; i += BT) {
-                T l = s_p[i * NB + j];
-                if (pv != T(0)) l = tiny ? l / pv : l * inv;
-                s_p[i * NB + j] = l;
-                for (int c = j + 1; c < kb; ++c) s_p[i * NB + c] -= l * s_p[j * NB + c];
+            else {
+                #pragma unroll
+                for (int s = 0; s < NB / 4; ++s) {
+                    const int k = 4 * s + fg;
+                    float av[2], bv[2];
+                    #pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        const int i = i0 + 16 * a + fr;
+                        av[a] = i < nr ? L[i * LD + k] : 0.f;
+                    }
+                    #pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const int jj = j0 + 16 * b + fr;
+                        bv[b] = jj < nr ? U[int64_t(k) * ldw + jj] : 0.f;
+                    }
+                    #pragma unroll
+                    for (int a = 0; a < 2; ++a)
+                        #pragma unroll
+                        for (int b = 0; b < 2; ++b)
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+                }
             }
-            block_sync();
+            // register r holds D[4 (l/16) + r][l % 16]
+            #pragma unroll
+            for (int a = 0; a < 2; ++a)
+                #pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    #pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = i0 + 16 * a + 4 * fg + r, jj = j0 + 16 * b + fr;
+                        if (i < nr && jj < nr) C[int64_t(i) * ldw + jj] -= acc[a][b][r];
+                    }
+        }
+    }
+}
+
+// In-workgroup LU with partial pivoting of the row-major n x n matrix W.
+// perm[i] = original row now at position i (the interchanges composed).
+// *s_info: first exactly-zero pivot.
+template <class T, int NBF, int BT, bool BFU>
+__device__ __forceinline__ void wg_getrf(T* __restrict__ W, int n, int ldw, int* __restrict__ perm, int* s_info,
+                                      T* s_p, T* s_red, int* s_ired, int* s_piv, int* s_dst, int* s_src)
+{
+    constexpr int NB = Pan<T, NBF>::NB, LD = Pan<T, NBF>::LD;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < n; i += BT) perm[i] = i;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0);
+        const int mr = n - k0;
+        for (int e = tid; e < mr * NB; e += BT) {
+            const int i = e / NB, c = e % NB;
+            s_p[i * LD + c] = c < kb ? W[int64_t(k0 + i) * ldw + k0 + c] : T(0);
+        }
+        __syncthreads();
+        wg_panel<T, NB, LD, BT>(s_p, mr, kb, k0, s_piv, s_info, s_red, s_ired);
+        // Net row moves of the panel's interchanges: lane t < kb follows row
+        // k0 + t, lane kb + t the pivot row of column t; each lane tracks
+        // which original row ends up in its row (lanes naming the same row
+        // stay equal).  A row whose source is itself does not move.
+        if (wave == 0) {
+            const int cand = lane < kb ? k0 + lane : (lane < 2 * kb ? k0 + s_piv[lane - kb] : -1);
+            int src = cand;
+            for (int j = 0; j < kb; ++j) {
+                const int ra = k0 + j, rb = k0 + s_piv[j];
+                const int va = __shfl(src, j, 64), vb = __shfl(src, kb + j, 64);
+                if (ra != rb) {
+                    if (cand == ra) src = vb;
+                    else if (cand == rb) src = va;
+                }
+            }
+            s_dst[lane] = cand;
+            s_src[lane] = src;
         }
         // panel back to W
         for (int e = tid; e < mr * NB; e += BT) {
             const int i = e / NB, c = e % NB;
-            if (c < kb) W[(k0 + i) * ldw + k0 + c] = s_p[e];
+            if (c < kb) W[int64_t(k0 + i) * ldw + k0 + c] = s_p[i * LD + c];
         }
-        // interchanges of this panel on the columns left and right of it
-        for (int j = 0; j < kb; ++j) {
-            const int p = piv[k0 + j];
-            block_sync();                                  // piv visible, previous swap done
-            if (p != k0 + j) {
-                for (int c = tid; c < n; c += BT) {
-                    if (c >= k0 && c < k0 + kb) continue;
-                    const T t = W[(k0 + j) * ldw + c];
-                    W[(k0 + j) * ldw + c] = W[p * ldw + c];
-                    W[p * ldw + c] = t;
+        __syncthreads();
+        // the moves on the columns left and right of the panel and on perm
+        // (column n): one column per thread, every source loaded before any
+        // row is written
+        for (int c = tid; c <= n; c += BT) {
+            if (c >= k0 && c < k0 + kb) continue;
+            asm volatile("" ::: "memory");          // reread the move list per column (keeps registers free)
+            if (c < n) {
+                T v[2 * NB];
+                #pragma unroll
+                for (int t = 0; t < 2 * NB; ++t) {
+                    const int d = s_dst[t], sr = s_src[t];
+                    v[t] = (d >= 0 && sr != d) ? W[sr * ldw + c] : T(0);
+                }
+                asm volatile("" ::: "memory");      // reread the list below instead of keeping it live
+                #pragma unroll
+                for (int t = 0; t < 2 * NB; ++t) {
+                    const int d = s_dst[t], sr = s_src[t];
+                    if (d >= 0 && sr != d) W[d * ldw + c] = v[t];
+                }
+            }
+            else {
+                int v[2 * NB];
+                #pragma unroll
+                for (int t = 0; t < 2 * NB; ++t) {
+                    const int d = s_dst[t], sr = s_src[t];
+                    v[t] = (d >= 0 && sr != d) ? perm[sr] : 0;
+                }
+                asm volatile("" ::: "memory");
+                #pragma unroll
+                for (int t = 0; t < 2 * NB; ++t) {
+                    const int d = s_dst[t], sr = s_src[t];
+                    if (d >= 0 && sr != d) perm[d] = v[t];
                 }
             }
         }
-        block_sync();
+        __syncthreads();
         const int c1 = k0 + kb, nr = n - c1;
         if (nr <= 0) break;
-        // U12 = L11^{-1} A12: forward substitution down the kb rows, one
-        // column per thread (the column stays in the L1 cache)
+        // U12 = L11^{-1} A12, one column per thread in registers
         for (int c = c1 + tid; c < n; c += BT) {
-            for (int r = 1; r < kb; ++r) {
-                T s = W[(k0 + r) * ldw + c];
-                for (int q = 0; q < r; ++q) s -= s_p[r * NB + q] * W[(k0 + q) * ldw + c];
-                W[(k0 + r) * ldw + c] = s;
-            }
-        }
-        block_sync();
-        // A22 -= L21 U12: each thread a 4 x 4 block per pass over the trailing matrix
-        const int tr = (nr + 3) / 4;                       // 4-row groups
-        const int tc = (nr + 3) / 4;
-        for (int t = tid; t < tr * tc; t += BT) {
-            const int bi = (t / tc) * 4, bj = (t % tc) * 4;
-            T acc[4][4];
+            asm volatile("" ::: "memory");          // read L11 from LDS per column, not hoisted into registers
+            T x[NB];
             #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int r = 0; r < NB; ++r) x[r] = W[int64_t(k0 + r) * ldw + c];
+            #pragma unroll
+            for (int r = 1; r < NB; ++r) {
+                asm volatile("" ::: "memory");      // row r's L11 reads stay inside its step
+                T s = x[r];
                 #pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
-            for (int q = 0; q < kb; ++q) {
-                T l[4], u[4];
-                #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int i = bi + a;
-                    l[a] = i < nr ? s_p[(kb + i) * NB + q] : T(0);
-                }
-                #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int jj = bj + b;
-                    u[b] = jj < nr ? W[(k0 + q) * ldw + c1 + jj] : T(0);
-                }
-                #pragma unroll
-                for (int a = 0; a < 4; ++a)
-                    #pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] += l[a] * u[b];
+                for (int q = 0; q < r; ++q) s -= s_p[r * LD + q] * x[q];
+                x[r] = s;
             }
             #pragma unroll
-            for (int a = 0; a < 4; ++a)
-                #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int i = bi + a, jj = bj + b;
-                    if (i < nr && jj < nr) W[(c1 + i) * ldw + c1 + jj] -= acc[a][b];
-                }
+            for (int r = 1; r < NB; ++r) W[int64_t(k0 + r) * ldw + c] = x[r];
         }
-        block_sync();
+        __syncthreads();
+        wg_update<T, NB, LD, BT, BFU>(W + int64_t(c1) * ldw + c1, W + int64_t(k0) * ldw + c1, ldw,
+                                      s_p + kb * LD, nr);
+        __syncthreads();
     }
 }
 
 // x := op(LU)^{-1} x for the factors in W (row-major), x of length n in
-// global memory.  Lower (unit) then upper, 64-row blocks: one wave solves the
-// diagonal block, the workgroup updates the rows below (above).
-template <class T, class V>
-__device__ __noinline__ void wg_trsv(const T* __restrict__ W, int n, int ldw, V* __restrict__ x, bool lower)
+// global memory (FP64).  Lower (unit) or upper, 64-row blocks: one wave solves
+// the diagonal block, the workgroup updates the rows below (above).
+template <class T, int BT>
+__device__ __forceinline__ void wg_trsv(const T* __restrict__ W, int n, int ldw, double* __restrict__ x, bool lower)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = (n + 63) / 64;
@@ -181,31 +429,31 @@ __device__ __noinline__ void wg_trsv(const T* __restrict__ W, int n, int ldw, V*
         const int blk = lower ? bb : nblk - 1 - bb;
         const int b0 = blk * 64, bn = min(64, n - b0);
         if (wave == 0) {
-            V y = lane < bn ? x[b0 + lane] : V(0);
+            double y = lane < bn ? x[b0 + lane] : 0.0;
             if (lower) {
                 for (int j = 0; j < bn; ++j) {
-                    const V yj = __shfl(y, j, 64);
-                    if (lane > j && lane < bn) y -= V(W[(b0 + lane) * ldw + b0 + j]) * yj;
+                    const double yj = __shfl(y, j, 64);
+                    if (lane > j && lane < bn) y -= double(W[int64_t(b0 + lane) * ldw + b0 + j]) * yj;
                 }
             }
             else {
                 for (int j = bn - 1; j >= 0; --j) {
-                    if (lane == j) y /= V(W[(b0 + j) * ldw + b0 + j]);
-                    const V yj = __shfl(y, j, 64);
-                    if (lane < j) y -= V(W[(b0 + lane) * ldw + b0 + j]) * yj;
+                    if (lane == j) y /= double(W[int64_t(b0 + j) * ldw + b0 + j]);
+                    const double yj = __shfl(y, j, 64);
+                    if (lane < j) y -= double(W[int64_t(b0 + lane) * ldw + b0 + j]) * yj;
                 }
             }
             if (lane < bn) x[b0 + lane] = y;
         }
-        block_sync();
-        // rows outside the block: x_i -= sum_j W[i][b0 + j] y_j
+        __syncthreads();
         const int r0 = lower ? b0 + bn : 0, r1 = lower ? n : b0;
         for (int i = r0 + tid; i < r1; i += BT) {
-            V s = V(0);
-            for (int j = 0; j < bn; ++j) s += V(W[i * ldw + b0 + j]) * x[b0 + j];
+            double s = 0.0;
+            const T* wr = W + int64_t(i) * ldw + b0;
+            for (int j = 0; j < bn; ++j) s += double(wr[j]) * x[b0 + j];
             x[i] -= s;
         }
-        block_sync();
+        __syncthreads();
     }
 }
 
@@ -213,7 +461,7 @@ struct BatchArgs {
     int count;
     const int* order;               // systems, largest first
     const int* n;                   // sizes
-    const int* mode;                // per system: 0 float factor + FP64 refinement, 1 FP64 factor
+    const int* mode;                // per system, see the file comment
     const int64_t* offA;            // A_i at A + offA[i], column-major, ld n_i (FP64)
     const int64_t* offB;            // b_i / x_i at B + offB[i]
     const double* A;
@@ -222,18 +470,24 @@ struct BatchArgs {
     int* iters;                     // refinement steps (>= 0), -(itermax+1) fallback, -3 singular factor
     int* info;                      // factorization info (FP64 path or fallback)
     double* berr;                   // ||b - A x||_inf / (||A||_inf ||x||_inf)
-    int* queue;                     // work counter (zeroed)
+    int* choice;                    // what ran: 0 BF16 + IR, 1 FP32 + IR, 2 FP64
+    float* feat;                    // learned path: SEL_F features per system (may be null)
+    unsigned long long* ticks;      // wall-clock ticks per system (100 MHz; may be null)
+    const float* sel;               // selector network parameters (learned path)
+    int* queue;                     // [0] ticket, [1] finished workgroups; zero between launches
     char* work;                     // per-workgroup workspace
     int64_t work_stride;            // bytes per workgroup
     int itermax;
     double tol;                     // <= 0: eps * sqrt(n)
 };
 
-// r = b - A x (FP64, A column-major) and the infinity norms; returns ||r||_inf
-// in s_dred[0], ||x||_inf in s_dred[1].
+// r = b - A x (FP64, A column-major) and infinity norms: ||r|| in s_dred[0],
+// ||x|| in s_dred[1].
+template <int BT>
 __device__ void wg_residual(const double* __restrict__ A, int n, const double* __restrict__ b,
                             const double* __restrict__ x, double* __restrict__ r, double* s_dred)
 {
+    constexpr int NW = BT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double rmax = 0, xmax = 0;
     for (int i = tid; i < n; i += BT) {
@@ -248,187 +502,342 @@ __device__ void wg_residual(const double* __restrict__ A, int n, const double* _
         rmax = fmax(rmax, __shfl_xor(rmax, o, 64));
         xmax = fmax(xmax, __shfl_xor(xmax, o, 64));
     }
-    block_sync();
-    if (lane == 0) { s_dred[2 + wave] = rmax; s_dred[6 + wave] = xmax; }
-    block_sync();
+    __syncthreads();
+    if (lane == 0) { s_dred[2 + wave] = rmax; s_dred[2 + NW + wave] = xmax; }
+    __syncthreads();
     if (tid == 0) {
         double a = 0, c = 0;
-        for (int w = 0; w < BT / 64; ++w) { a = fmax(a, s_dred[2 + w]); c = fmax(c, s_dred[6 + w]); }
+        for (int w = 0; w < NW; ++w) { a = fmax(a, s_dred[2 + w]); c = fmax(c, s_dred[2 + NW + w]); }
         s_dred[0] = a;
         s_dred[1] = c;
     }
-    block_sync();
+    __syncthreads();
 }
 
-// ||A||_inf (FP64 column-major) into s_dred[0].
-__device__ void wg_norm_inf(const double* __restrict__ A, int n, double* s_dred)
+// max over rows of the row sums of |A| (column-major) -> s_dred[0];
+// max |v| of v (length n) -> s_dred[1].
+template <int BT>
+__device__ void wg_norms(const double* __restrict__ A, int n, const double* __restrict__ v, double* s_dred)
 {
+    constexpr int NW = BT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double m = 0;
+    double m = 0, vm = 0;
     for (int i = tid; i < n; i += BT) {
         double s = 0;
         for (int j = 0; j < n; ++j) s += fabs(A[i + int64_t(j) * n]);
         m = fmax(m, s);
+        vm = fmax(vm, fabs(v[i]));
     }
     #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    block_sync();
-    if (lane == 0) s_dred[2 + wave] = m;
-    block_sync();
-    if (tid == 0) {
-        double a = 0;
-        for (int w = 0; w < BT / 64; ++w) a = fmax(a, s_dred[2 + w]);
-        s_dred[0] = a;
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmax(m, __shfl_xor(m, o, 64));
+        vm = fmax(vm, __shfl_xor(vm, o, 64));
     }
-    block_sync();
+    __syncthreads();
+    if (lane == 0) { s_dred[2 + wave] = m; s_dred[2 + NW + wave] = vm; }
+    __syncthreads();
+    if (tid == 0) {
+        double a = 0, c = 0;
+        for (int w = 0; w < NW; ++w) { a = fmax(a, s_dred[2 + w]); c = fmax(c, s_dred[2 + NW + w]); }
+        s_dred[0] = a;
+        s_dred[1] = c;
+    }
+    __syncthreads();
 }
 
-// Factor (precision T) and solve every system of the batch.
-template <class T>
+// Selector network forward pass by wave 0 (h1 = relu(W1^T f + b1),
+// h2 = relu(W2^T h1 + b2), z = W3^T h2 + b3; lane h holds hidden unit h);
+// returns argmax over the first SEL_CHOICES outputs (lane 0 valid).
+__device__ int sel_choose(const float* __restrict__ P, const float* f)
+{
+    const int lane = threadIdx.x & 63;
+    const float* W1 = P;
+    const float* b1 = W1 + SEL_F * SEL_H;
+    const float* W2 = b1 + SEL_H;
+    const float* b2 = W2 + SEL_H * SEL_H;
+    const float* W3 = b2 + SEL_H;
+    const float* b3 = W3 + SEL_H * SEL_K;
+    float h = b1[lane];
+    for (int q = 0; q < SEL_F; ++q) h += f[q] * W1[q * SEL_H + lane];
+    h = fmaxf(h, 0.f);
+    float h2 = b2[lane];
+    for (int g = 0; g < SEL_H; ++g) h2 += __shfl(h, g, 64) * W2[g * SEL_H + lane];
+    h2 = fmaxf(h2, 0.f);
+    const int k = lane & (SEL_K - 1);
+    float z = b3[k];
+    for (int g = 0; g < SEL_H; ++g) z += __shfl(h2, g, 64) * W3[g * SEL_K + k];
+    int best = 0;
+    float bz = __shfl(z, 0, 64);
+    for (int c = 1; c < SEL_CHOICES; ++c) {
+        const float zc = __shfl(z, c, 64);
+        if (zc > bz) { bz = zc; best = c; }
+    }
+    return best;
+}
+
+template <int NBF, int BT>
 __global__ __launch_bounds__(BT)
 void batched_solve_kernel(BatchArgs a)
 {
-    constexpr int NB = BNB<T>::v;
-    __shared__ T s_p[BMAXN * NB];
-    __shared__ T s_red[BT / 64];
-    __shared__ int s_ired[BT / 64];
-    __shared__ double s_dred[16];
-    __shared__ int s_sys, s_info;
+    constexpr int NW = BT / 64;
+    __shared__ __attribute__((aligned(16))) char s_pan[pan_lds_bytes<NBF>()];
+    __shared__ double s_red[NW];
+    __shared__ int s_ired[NW];
+    __shared__ int s_piv[NBF];
+    __shared__ int s_dst[64], s_src[64];
+    __shared__ double s_dred[2 + 2 * NW];
+    __shared__ float s_feat[SEL_F];
+    __shared__ int s_sys, s_info, s_choice;
     const int tid = threadIdx.x;
     char* ws = a.work + int64_t(blockIdx.x) * a.work_stride;
-    // workspace: Wlo (n*n T) | W64 (n*n double, fallback) | piv (n int) | r, y (n double)
-    T* W = reinterpret_cast<T*>(ws);
-    double* W64 = reinterpret_cast<double*>(ws + int64_t(BMAXN) * BMAXN * sizeof(T));
-    int* piv = reinterpret_cast<int*>(W64 + int64_t(BMAXN) * BMAXN);
-    double* r = reinterpret_cast<double*>(piv + BMAXN);
+    // workspace: W32 (1024^2 float) | W64 (1024^2 double) | perm (1024 int) | r, y (1024 double)
+    float* W = reinterpret_cast<float*>(ws);
+    double* W64 = reinterpret_cast<double*>(ws + int64_t(BMAXN) * BMAXN * sizeof(float));
+    int* perm = reinterpret_cast<int*>(W64 + int64_t(BMAXN) * BMAXN);
+    double* r = reinterpret_cast<double*>(perm + BMAXN);
     double* y = r + BMAXN;
+    float* s_pf = reinterpret_cast<float*>(s_pan);
+    double* s_pd = reinterpret_cast<double*>(s_pan);
     const double eps = std::numeric_limits<double>::epsilon();
     while (true) {
         if (tid == 0) {
-            const int q = atomicAdd(a.queue, 1);
+            const int q = atomicAdd(&a.queue[0], 1);
             s_sys = q < a.count ? a.order[q] : -1;
             s_info = 0;
         }
-        block_sync();
+        __syncthreads();
         const int sys = s_sys;
         if (sys < 0) break;                               // every wave leaves here
-        const int n = a.n[sys];
+        const unsigned long long t0 = wall_clock64();
+        const int n = a.n[sys], ldw = b_ldw(n);
+        const int mode = a.mode[sys];
         const double* A = a.A + a.offA[sys];
         const double* b = a.B + a.offB[sys];
         double* x = a.X + a.offB[sys];
-        // copy / cast into the row-major working matrix
-        for (int e = tid; e < n * n && !(sizeof(T) != 8 && a.mode && a.mode[sys] == 1); e += BT) {
-            const int i = e / n, j = e % n;
-            W[i * n + j] = T(A[i + int64_t(j) * n]);
-        }
-        const bool direct64 = sizeof(T) != 8 && a.mode && a.mode[sys] == 1;
-        block_sync();
-        if (!direct64) wg_getrf<T>(W, n, n, piv, &s_info, s_p, s_red, s_ired);
-        block_sync();
-        int info = s_info;
-        int iters = 0;
-        bool need64 = direct64;
-        // x = (LU)^{-1} P b, in FP64 storage
-        auto lo_solve = [&](const double* rhs, double* out) {
-            for (int i = tid; i < n; i += BT) out[i] = rhs[i];
-            block_sync();
-            if (tid == 0)
-                for (int j = 0; j < n; ++j) {
-                    const int p = piv[j];
-                    if (p != j) { const double t = out[j]; out[j] = out[p]; out[p] = t; }
-                }
-            block_sync();
-            if constexpr (sizeof(T) == 8) {
-                wg_trsv<T, double>(W, n, n, out, true);
-                wg_trsv<T, double>(W, n, n, out, false);
+        const double tol = a.tol > 0 ? a.tol : eps * sqrt(double(n));
+
+        // ||A||_inf and ||b||_inf
+        wg_norms<BT>(A, n, b, s_dred);
+        const double anorm = s_dred[0], bnorm = s_dred[1];
+        __syncthreads();
+
+        // out = (LU)^{-1} P rhs in FP64 storage, LU in W (float) or W64
+        auto lu_solve = [&](bool f64, const double* rhs, double* out) {
+            for (int i = tid; i < n; i += BT) out[i] = rhs[perm[i]];
+            __syncthreads();
+            if (f64) {
+                wg_trsv<double, BT>(W64, n, ldw, out, true);
+                wg_trsv<double, BT>(W64, n, ldw, out, false);
             }
             else {
-                wg_trsv<T, double>(W, n, n, out, true);
-                wg_trsv<T, double>(W, n, n, out, false);
+                wg_trsv<float, BT>(W, n, ldw, out, true);
+                wg_trsv<float, BT>(W, n, ldw, out, false);
             }
         };
-        if (direct64) {
-            // the FP64 factorization below is the chosen path, not a fallback
+        auto factor32 = [&](bool bf) {
+            wg_load_rowmajor<float, BT>(A, n, W, ldw);
+            if (tid == 0) s_info = 0;
+            __syncthreads();
+            if (bf)
+                wg_getrf<float, NBF, BT, true>(W, n, ldw, perm, &s_info, s_pf, reinterpret_cast<float*>(s_red),
+                                               s_ired, s_piv, s_dst, s_src);
+            else
+                wg_getrf<float, NBF, BT, false>(W, n, ldw, perm, &s_info, s_pf, reinterpret_cast<float*>(s_red),
+                                                s_ired, s_piv, s_dst, s_src);
+            __syncthreads();
+            return s_info;
+        };
+        // Refinement of x with the float factor in W; have_r: the residual of
+        // x is already in r / s_dred.  Returns the iterations, or -1 on a miss.
+        auto refine = [&](bool have_r) {
+            for (int it = 0; it <= a.itermax; ++it) {
+                if (!(have_r && it == 0)) wg_residual<BT>(A, n, b, x, r, s_dred);
+                const bool conv = s_dred[0] <= s_dred[1] * anorm * tol;
+                __syncthreads();
+                if (conv) return it;
+                if (it == a.itermax) break;
+                lu_solve(false, r, y);
+                for (int i = tid; i < n; i += BT) x[i] += y[i];
+                __syncthreads();
+            }
+            return -1;
+        };
+
+        int info = 0, iters = 0, ch;
+        bool need64 = false;
+        if (mode == 1) {
+            ch = 2;
+            need64 = true;
         }
-        else if (info != 0) {
-            need64 = sizeof(T) != 8;
-            iters = -3;
+        else if (mode == 0 || mode == 2) {
+            ch = mode == 0 ? 1 : 0;
+            if (factor32(mode == 2) != 0) { need64 = true; iters = -3; }
+            else {
+                lu_solve(false, b, x);
+                const int it = refine(false);
+                if (it < 0) { need64 = true; iters = -a.itermax - 1; }
+                else iters = it;
+            }
         }
         else {
-            lo_solve(b, x);
-            if constexpr (sizeof(T) != 8) {
-                wg_norm_inf(A, n, s_dred);
-                const double anorm = s_dred[0];
-                const double tol = a.tol > 0 ? a.tol : eps * sqrt(double(n));
-                bool conv = false;
-                for (int it = 0; it <= a.itermax; ++it) {
-                    wg_residual(A, n, b, x, r, s_dred);
-                    if (s_dred[0] <= s_dred[1] * anorm * tol) { iters = it; conv = true; break; }
-                    if (it == a.itermax) break;
-                    lo_solve(r, y);
-                    for (int i = tid; i < n; i += BT) x[i] += y[i];
-                    block_sync();
+            // learned path: BF16 probe factor, one solve, iteration-0 residual
+            const int pinfo = factor32(true);
+            double xnorm = 0, rnorm = 0;
+            if (pinfo == 0) {
+                lu_solve(false, b, x);
+                wg_residual<BT>(A, n, b, x, r, s_dred);
+                rnorm = s_dred[0];
+                xnorm = s_dred[1];
+            }
+            if (tid == 0) {
+                for (int q = 0; q < SEL_F; ++q) s_feat[q] = 0.f;
+                const double tiny = 1e-300;
+                s_feat[0] = 1.f;
+                s_feat[1] = float(log2(double(n)) / 10.0);
+                const bool ok = pinfo == 0 && isfinite(rnorm) && isfinite(xnorm) && xnorm > 0 && bnorm > 0
+                                && anorm > 0;
+                if (ok) {
+                    s_feat[2] = float(log10(fmax(anorm * xnorm / bnorm, tiny)) / 8.0);
+                    s_feat[3] = float(log10(fmax(rnorm / (anorm * xnorm), tiny)) / 8.0);
+                    s_feat[5] = float(log10(fmax(rnorm / bnorm, tiny)) / 8.0);
+                    s_feat[6] = (rnorm <= xnorm * anorm * tol) ? 1.f : 0.f;
                 }
-                if (!conv) { need64 = true; iters = -a.itermax - 1; }
+                else {
+                    s_feat[4] = 1.f;
+                }
+            }
+            __syncthreads();
+            if (a.feat && tid < SEL_F) a.feat[int64_t(sys) * SEL_F + tid] = s_feat[tid];
+            int c = mode - 4;
+            if (mode == 3) {
+                if (tid < 64) {
+                    const int cc = sel_choose(a.sel, s_feat);
+                    if (tid == 0) s_choice = cc;
+                }
+                __syncthreads();
+                c = s_choice;
+            }
+            if (c == 0 && s_feat[4] != 0.f) c = 1;          // singular or non-finite probe: cannot keep it
+            ch = c;
+            if (c == 0) {
+                const int it = refine(true);
+                if (it < 0) { need64 = true; iters = -a.itermax - 1; }
+                else iters = it;
+            }
+            else if (c == 1) {
+                if (factor32(false) != 0) { need64 = true; iters = -3; }
+                else {
+                    lu_solve(false, b, x);
+                    const int it = refine(false);
+                    if (it < 0) { need64 = true; iters = -a.itermax - 1; }
+                    else iters = it;
+                }
+            }
+            else {
+                need64 = true;
+                iters = 0;
             }
         }
         if (need64) {
-            // FP64 fallback: the same steps in double
-            for (int e = tid; e < n * n; e += BT) {
-                const int i = e / n, j = e % n;
-                W64[i * n + j] = A[i + int64_t(j) * n];
-            }
+            wg_load_rowmajor<double, BT>(A, n, W64, ldw);
             if (tid == 0) s_info = 0;
-            block_sync();
-            wg_getrf<double>(W64, n, n, piv, &s_info, reinterpret_cast<double*>(s_p),
-                             reinterpret_cast<double*>(s_dred + 8), s_ired);
-            block_sync();
+            __syncthreads();
+            wg_getrf<double, NBF, BT, false>(W64, n, ldw, perm, &s_info, s_pd, s_red, s_ired, s_piv, s_dst, s_src);
+            __syncthreads();
             info = s_info;
-            for (int i = tid; i < n; i += BT) x[i] = b[i];
-            block_sync();
-            if (tid == 0)
-                for (int j = 0; j < n; ++j) {
-                    const int p = piv[j];
-                    if (p != j) { const double t = x[j]; x[j] = x[p]; x[p] = t; }
-                }
-            block_sync();
-            wg_trsv<double, double>(W64, n, n, x, true);
-            wg_trsv<double, double>(W64, n, n, x, false);
+            lu_solve(true, b, x);
         }
         // backward error
-        wg_norm_inf(A, n, s_dred);
-        const double anorm = s_dred[0];
-        wg_residual(A, n, b, x, r, s_dred);
+        wg_residual<BT>(A, n, b, x, r, s_dred);
         if (tid == 0) {
             a.iters[sys] = iters;
             a.info[sys] = info;
             const double den = anorm * s_dred[1];
             a.berr[sys] = den > 0 ? s_dred[0] / den : 0.0;
+            if (a.choice) a.choice[sys] = ch;
+            if (a.ticks) a.ticks[sys] = wall_clock64() - t0;
         }
-        block_sync();
+        __syncthreads();
+    }
+    // the last workgroup to finish leaves the queue zeroed for the next launch
+    if (tid == 0) {
+        __threadfence();
+        const int done = atomicAdd(&a.queue[1], 1);
+        if (done == int(gridDim.x) - 1) {
+            atomicExch(&a.queue[0], 0);
+            atomicExch(&a.queue[1], 0);
+        }
     }
 }
 
-int64_t batched_work_stride(int elem_bytes)
+int64_t batched_work_stride(int)
 {
-    int64_t b = int64_t(BMAXN) * BMAXN * (elem_bytes + 8) + int64_t(BMAXN) * (4 + 16) + 256;
+    int64_t b = int64_t(BMAXN) * BMAXN * (4 + 8) + int64_t(BMAXN) * (4 + 16) + 256;
     return (b + 255) & ~int64_t(255);
 }
 
-void batched_solve(int elem_bytes, int count, const int* order, const int* n, const int* mode,
-                   const int64_t* offA, const int64_t* offB, const double* A, const double* B, double* X,
-                   int* iters, int* info, double* berr, int* queue, char* work, int grid, int itermax,
-                   double tol, hipStream_t s)
+// Kernel of a tile configuration: float panel width nbf (16 or 32) and
+// workgroup size bt (256; 512 with nbf 16 only: the 32-wide panel at 512
+// threads spills most of its working registers).
+static const void* batched_kernel_ptr(int nbf, int bt)
 {
-    if (count <= 0) return;
-    BatchArgs a{count, order, n, mode, offA, offB, A, B, X, iters, info, berr, queue, work,
-                batched_work_stride(elem_bytes), itermax, tol};
-    zero_u32(reinterpret_cast<unsigned*>(queue), 1, s);
-    if (elem_bytes == 8)
-        hipLaunchKernelGGL(batched_solve_kernel<double>, dim3(unsigned(grid)), dim3(BT), 0, s, a);
-    else
-        hipLaunchKernelGGL(batched_solve_kernel<float>, dim3(unsigned(grid)), dim3(BT), 0, s, a);
+    if (nbf == 16) return bt == 512 ? (const void*)batched_solve_kernel<16, 512> : (const void*)batched_solve_kernel<16, 256>;
+    return (const void*)batched_solve_kernel<32, 256>;
+}
+
+bool batched_config_valid(int nbf, int bt) { return (nbf == 16 && (bt == 256 || bt == 512)) || (nbf == 32 && bt == 256); }
+
+int batched_max_wgs_per_cu(int nbf, int bt)
+{
+    SA_REQUIRE(batched_config_valid(nbf, bt), -2, "batched: bad tile configuration");
+    int nb = 0;
+    SA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batched_kernel_ptr(nbf, bt), bt, 0));
+    return nb;
+}
+
+void batched_fill_args(void* out, int count, const int* order, const int* n, const int* mode, const int64_t* offA,
+                       const int64_t* offB, const double* A, const double* B, double* X, int* iters, int* info,
+                       double* berr, int* choice, float* feat, unsigned long long* ticks, const float* sel,
+                       int* queue, char* work, int itermax, double tol)
+{
+    BatchArgs a{count, order, n, mode, offA, offB, A, B, X, iters, info, berr, choice, feat, ticks, sel, queue,
+                work, batched_work_stride(4), itermax, tol};
+    static_assert(sizeof(BatchArgs) <= BATCH_ARGS_BYTES, "BatchArgs size");
+    std::memcpy(out, &a, sizeof(a));
+}
+
+void batched_launch(const void* args, int nbf, int bt, int grid, hipStream_t s)
+{
+    SA_REQUIRE(batched_config_valid(nbf, bt), -2, "batched: bad tile configuration");
+    const BatchArgs& a = *static_cast<const BatchArgs*>(args);
+    if (a.count <= 0) return;
+    if (nbf == 16) {
+        if (bt == 512) hipLaunchKernelGGL((batched_solve_kernel<16, 512>), dim3(unsigned(grid)), dim3(512), 0, s, a);
+        else hipLaunchKernelGGL((batched_solve_kernel<16, 256>), dim3(unsigned(grid)), dim3(256), 0, s, a);
+    }
+    else {
+        hipLaunchKernelGGL((batched_solve_kernel<32, 256>), dim3(unsigned(grid)), dim3(256), 0, s, a);
+    }
     SA_CHECK_LAUNCH();
+}
+
+// A one-node graph holding the launch (the kernel resets its own queue, so
+// replays need nothing else).  args must stay valid while the graph exists.
+void batched_graph(void* args, int nbf, int bt, int grid, hipGraph_t* g, hipGraphExec_t* ge)
+{
+    SA_REQUIRE(batched_config_valid(nbf, bt), -2, "batched: bad tile configuration");
+    SA_HIP(hipGraphCreate(g, 0));
+    hipKernelNodeParams kp{};
+    void* kargs[1] = {args};
+    kp.func = const_cast<void*>(batched_kernel_ptr(nbf, bt));
+    kp.gridDim = dim3(unsigned(grid));
+    kp.blockDim = dim3(unsigned(bt));
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = kargs;
+    kp.extra = nullptr;
+    hipGraphNode_t node;
+    SA_HIP(hipGraphAddKernelNode(&node, *g, nullptr, 0, &kp));
+    SA_HIP(hipGraphInstantiate(ge, *g, nullptr, nullptr, 0));
 }
 
 int batched_max_n() { return BMAXN; }

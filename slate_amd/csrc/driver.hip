@@ -78,6 +78,13 @@ struct Context {
     Buf tlA, tlB, tlX, tlP;      // tile-storage entries: dense copies of A, B, X and the tile tables
     Buf npu, npt;                // no-pivoting panel: U11^{-1} and staging
     Buf bmeta, bwork;            // batched solves: per-system metadata, per-workgroup workspace
+    Buf bqueue;                  // batched ticket counters (zeroed when allocated; launches leave them zero)
+    int b_nbf = 32, b_bt = 256, b_wgs = 1;   // batched tile configuration (slate_amd_tune_set_batched)
+    hipGraph_t bgraph = nullptr;             // one-node graph of the last batched launch (use_graph)
+    hipGraphExec_t bgraph_exec = nullptr;
+    alignas(16) char bgraph_args[BATCH_ARGS_BYTES] = {};
+    int bgraph_cfg[3] = {0, 0, 0};
+    int64_t bgraph_builds = 0, bgraph_replays = 0;
     Buf slots, err, dinfo, plan0, plan1, plans, ipiv, work, lo, vec, basis, rhs, perm, flags, tinv, ttmp, sinv, stmp1, dinv;
     double last_ms = 0;
     uint32_t leaf_tag = 0;       // next free tag of the panel exchange words
@@ -2213,6 +2220,9 @@ int slate_amd_finalize(slate_amd_context ctx)
     for (auto e : c.evs) (void)hipEventDestroy(e);
     for (auto e : c.prof_ev) (void)hipEventDestroy(e);
     for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.second);
+    if (c.bgraph_exec) (void)hipGraphExecDestroy(c.bgraph_exec);
+    if (c.bgraph) (void)hipGraphDestroy(c.bgraph);
+    c.bqueue.release();
     (void)hipEventDestroy(c.t_start);
     (void)hipEventDestroy(c.t_stop);
     (void)hipEventDestroy(c.ev_in);
@@ -2735,7 +2745,109 @@ int64_t slate_amd_dsgesv_auto(slate_amd_context ctx, int64_t n, double* dA, int6
 }
 
 // Batched uneven solves (no counterpart in the reference; SLATE solves one
-// distributed system per call).
+// distributed system per call).  Shared by the fixed-precision and the
+// learned entries: per-system modes as in batched.hip.
+static int64_t batched_run(sa::Context& c, int count, const int* n, const double* dA, const double* dB, double* dX,
+                           const int* modes, const float* sel, int* iters, int* info, double* berr, int* choice,
+                           float* features, double* ms, const slate_amd_options* opts)
+{
+    const int maxn = sa::batched_max_n();
+    std::vector<int64_t> offA(count), offB(count);
+    std::vector<int> order(count);
+    int64_t a = 0, b = 0;
+    for (int i = 0; i < count; ++i) {
+        SA_REQUIRE(n[i] >= 1 && n[i] <= maxn, -3, "system sizes must be in [1, 1024]");
+        offA[i] = a; offB[i] = b;
+        a += int64_t(n[i]) * n[i];
+        b += n[i];
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+    // use_graph means the batched launch's own one-node graph here (the LU
+    // step-sequence graph that read_opts guards is a different path)
+    slate_amd_options oc;
+    if (opts) oc = *opts;
+    else slate_amd_options_init(&oc);
+    const bool use_graph = oc.use_graph != 0;
+    oc.use_graph = 0;
+    sa::Opts o = sa::read_opts(&oc);
+    SA_ENTRY_SYNC();
+    sa::Timer t(c);
+    // metadata: offA, offB, ticks (8 bytes) | berr (double) | n, mode, order, iters, info, choice (int) |
+    //           features (float) | selector parameters (float)
+    const int nsel = sel ? sa::mlp_num_params() : 0;
+    const size_t meta = (sizeof(int64_t) * 3 + sizeof(double)) * size_t(count) + sizeof(int) * 6 * size_t(count)
+                        + sizeof(float) * (24 * size_t(count) + nsel) + 1024;
+    c.bmeta.ensure(meta);
+    char* p = static_cast<char*>(c.bmeta.p);
+    int64_t* d_offA = reinterpret_cast<int64_t*>(p);
+    int64_t* d_offB = d_offA + count;
+    unsigned long long* d_ticks = reinterpret_cast<unsigned long long*>(d_offB + count);
+    double* d_berr = reinterpret_cast<double*>(d_ticks + count);
+    int* d_n = reinterpret_cast<int*>(d_berr + count);
+    int* d_mode = d_n + count;
+    int* d_order = d_mode + count;
+    int* d_iters = d_order + count;
+    int* d_info = d_iters + count;
+    int* d_choice = d_info + count;
+    float* d_feat = reinterpret_cast<float*>(d_choice + count);
+    float* d_sel = d_feat + 24 * size_t(count);
+    if (!c.bqueue.p) {
+        c.bqueue.ensure(256);
+        SA_HIP(hipMemsetAsync(c.bqueue.p, 0, 256, c.s0));
+    }
+    SA_HIP(hipMemcpyAsync(d_offA, offA.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice, c.s0));
+    SA_HIP(hipMemcpyAsync(d_offB, offB.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice, c.s0));
+    SA_HIP(hipMemcpyAsync(d_n, n, sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
+    SA_HIP(hipMemcpyAsync(d_mode, modes, sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
+    SA_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
+    if (nsel) SA_HIP(hipMemcpyAsync(d_sel, sel, sizeof(float) * nsel, hipMemcpyHostToDevice, c.s0));
+    int cus = 0;
+    SA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    const int grid = std::max(1, std::min(count, cus * c.b_wgs));
+    c.bwork.ensure(size_t(grid) * size_t(sa::batched_work_stride()));
+    alignas(16) char args[sa::BATCH_ARGS_BYTES] = {};
+    sa::batched_fill_args(args, count, d_order, d_n, d_mode, d_offA, d_offB, dA, dB, dX, d_iters, d_info, d_berr,
+                          d_choice, features ? d_feat : nullptr, ms ? d_ticks : nullptr, nsel ? d_sel : nullptr,
+                          c.bqueue.as<int>(), static_cast<char*>(c.bwork.p), int(o.itermax), o.tol);
+    if (use_graph) {
+        const int cfg[3] = {c.b_nbf, c.b_bt, grid};
+        if (!c.bgraph_exec || memcmp(args, c.bgraph_args, sizeof(args)) != 0
+            || memcmp(cfg, c.bgraph_cfg, sizeof(cfg)) != 0) {
+            if (c.bgraph_exec) { SA_HIP(hipGraphExecDestroy(c.bgraph_exec)); c.bgraph_exec = nullptr; }
+            if (c.bgraph) { SA_HIP(hipGraphDestroy(c.bgraph)); c.bgraph = nullptr; }
+            memcpy(c.bgraph_args, args, sizeof(args));
+            memcpy(c.bgraph_cfg, cfg, sizeof(cfg));
+            sa::batched_graph(c.bgraph_args, c.b_nbf, c.b_bt, grid, &c.bgraph, &c.bgraph_exec);
+            ++c.bgraph_builds;
+        }
+        else {
+            ++c.bgraph_replays;
+        }
+        SA_HIP(hipGraphLaunch(c.bgraph_exec, c.s0));
+    }
+    else {
+        sa::batched_launch(args, c.b_nbf, c.b_bt, grid, c.s0);
+    }
+    SA_HIP(hipMemcpyAsync(iters, d_iters, sizeof(int) * count, hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipMemcpyAsync(info, d_info, sizeof(int) * count, hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipMemcpyAsync(berr, d_berr, sizeof(double) * count, hipMemcpyDeviceToHost, c.s0));
+    if (choice) SA_HIP(hipMemcpyAsync(choice, d_choice, sizeof(int) * count, hipMemcpyDeviceToHost, c.s0));
+    if (features)
+        SA_HIP(hipMemcpyAsync(features, d_feat, sizeof(float) * 24 * size_t(count), hipMemcpyDeviceToHost, c.s0));
+    std::vector<unsigned long long> ticks(ms ? count : 0);
+    if (ms) SA_HIP(hipMemcpyAsync(ticks.data(), d_ticks, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipStreamSynchronize(c.s0));
+    if (ms) {
+        int khz = 0;
+        SA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device));
+        SA_REQUIRE(khz > 0, -100, "wall clock rate unavailable");
+        for (int i = 0; i < count; ++i) ms[i] = double(ticks[i]) / double(khz);
+    }
+    t.stop();
+    return 0;
+}
+
 int64_t slate_amd_dgesv_batched(slate_amd_context ctx, int count, const int* n, const double* dA,
                                 const double* dB, double* dX, const int* mode, int factor_precision,
                                 int* iters, int* info, double* berr, const slate_amd_options* opts)
@@ -2744,57 +2856,67 @@ int64_t slate_amd_dgesv_batched(slate_amd_context ctx, int count, const int* n, 
         SA_REQUIRE(count >= 0, -2, "bad count");
         if (count == 0) return 0;
         SA_REQUIRE(n && dA && dB && dX && iters && info && berr, -3, "null argument");
-        SA_REQUIRE(factor_precision == SLATE_AMD_PREC_FP32 || factor_precision == SLATE_AMD_PREC_FP64, -8,
-                   "factor_precision must be FP32 or FP64");
-        const int maxn = sa::batched_max_n();
-        std::vector<int64_t> offA(count), offB(count);
-        std::vector<int> order(count), md(count, 0);
-        int64_t a = 0, b = 0;
-        for (int i = 0; i < count; ++i) {
-            SA_REQUIRE(n[i] >= 1 && n[i] <= maxn, -3, "system sizes must be in [1, 1024]");
-            offA[i] = a; offB[i] = b;
-            a += int64_t(n[i]) * n[i];
-            b += n[i];
-            order[i] = i;
-            if (mode) { SA_REQUIRE(mode[i] == 0 || mode[i] == 1, -7, "mode must be 0 or 1"); md[i] = mode[i]; }
+        SA_REQUIRE(factor_precision == SLATE_AMD_PREC_FP32 || factor_precision == SLATE_AMD_PREC_FP64
+                   || factor_precision == SLATE_AMD_PREC_BF16, -8, "factor_precision must be FP32, BF16 or FP64");
+        const int base = factor_precision == SLATE_AMD_PREC_FP64 ? 1 : factor_precision == SLATE_AMD_PREC_BF16 ? 2 : 0;
+        std::vector<int> md(count, base);
+        for (int i = 0; i < count && mode; ++i) {
+            SA_REQUIRE(mode[i] == 0 || mode[i] == 1, -7, "mode must be 0 or 1");
+            if (mode[i] == 1) md[i] = 1;
         }
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
-        sa::Opts o = sa::read_opts(opts);
-        SA_ENTRY_SYNC();
-        sa::Timer t(c);
-        // metadata: offA, offB (int64) | n, mode, order, iters, info (int) | berr (double) | queue
-        const size_t meta = sizeof(int64_t) * 2 * count + sizeof(int) * 5 * size_t(count)
-                            + sizeof(double) * count + 1024;
-        c.bmeta.ensure(meta);
-        char* p = static_cast<char*>(c.bmeta.p);
-        int64_t* d_offA = reinterpret_cast<int64_t*>(p);
-        int64_t* d_offB = d_offA + count;
-        int* d_n = reinterpret_cast<int*>(d_offB + count);
-        int* d_mode = d_n + count;
-        int* d_order = d_mode + count;
-        int* d_iters = d_order + count;
-        int* d_info = d_iters + count;
-        double* d_berr = reinterpret_cast<double*>((reinterpret_cast<uintptr_t>(d_info + count) + 7) & ~uintptr_t(7));
-        int* d_queue = reinterpret_cast<int*>(d_berr + count);
-        SA_HIP(hipMemcpyAsync(d_offA, offA.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice, c.s0));
-        SA_HIP(hipMemcpyAsync(d_offB, offB.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice, c.s0));
-        SA_HIP(hipMemcpyAsync(d_n, n, sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
-        SA_HIP(hipMemcpyAsync(d_mode, md.data(), sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
-        SA_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, c.s0));
-        int cus = 0;
-        SA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
-        const int eb = factor_precision == SLATE_AMD_PREC_FP64 ? 8 : 4;
-        const int grid = std::max(1, std::min(count, cus));
-        c.bwork.ensure(size_t(grid) * size_t(sa::batched_work_stride(eb)));
-        sa::batched_solve(eb, count, d_order, d_n, d_mode, d_offA, d_offB, dA, dB, dX, d_iters, d_info, d_berr,
-                          d_queue, static_cast<char*>(c.bwork.p), grid, int(o.itermax), o.tol, c.s0);
-        SA_HIP(hipMemcpyAsync(iters, d_iters, sizeof(int) * count, hipMemcpyDeviceToHost, c.s0));
-        SA_HIP(hipMemcpyAsync(info, d_info, sizeof(int) * count, hipMemcpyDeviceToHost, c.s0));
-        SA_HIP(hipMemcpyAsync(berr, d_berr, sizeof(double) * count, hipMemcpyDeviceToHost, c.s0));
-        SA_HIP(hipStreamSynchronize(c.s0));
-        t.stop();
-        return 0;
+        return batched_run(c, count, n, dA, dB, dX, md.data(), nullptr, iters, info, berr, nullptr, nullptr,
+                           nullptr, opts);
     });
+}
+
+int64_t slate_amd_dgesv_batched_auto(slate_amd_context ctx, int count, const int* n, const double* dA,
+                                     const double* dB, double* dX, const float* params, int force, int* iters,
+                                     int* info, double* berr, int* choice, float* features, double* ms,
+                                     const slate_amd_options* opts)
+{
+    return guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(count >= 0, -2, "bad count");
+        if (count == 0) return 0;
+        SA_REQUIRE(n && dA && dB && dX && iters && info && berr, -3, "null argument");
+        SA_REQUIRE(force >= -1 && force <= 2, -7, "force must be -1 (network) or a choice 0..2");
+        SA_REQUIRE(force >= 0 || params, -7, "selector parameters required unless a choice is forced");
+        std::vector<int> md(count, force < 0 ? 3 : 4 + force);
+        return batched_run(c, count, n, dA, dB, dX, md.data(), force < 0 ? params : nullptr, iters, info, berr,
+                           choice, features, ms, opts);
+    });
+}
+
+int slate_amd_tune_set_batched(slate_amd_context ctx, int nbf, int bt, int wgs_per_cu)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        SA_REQUIRE(sa::batched_config_valid(nbf, bt), -2, "batched tile configuration: nbf 16/32, bt 256/512");
+        SA_REQUIRE(wgs_per_cu >= 1 && wgs_per_cu <= sa::batched_max_wgs_per_cu(nbf, bt), -4,
+                   "batched: workgroups per CU above what LDS and registers allow");
+        c.b_nbf = nbf;
+        c.b_bt = bt;
+        c.b_wgs = wgs_per_cu;
+        return 0;
+    }));
+}
+
+int slate_amd_tune_get_batched(slate_amd_context ctx, int* nbf, int* bt, int* wgs_per_cu, int* max_wgs_per_cu)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        if (nbf) *nbf = c.b_nbf;
+        if (bt) *bt = c.b_bt;
+        if (wgs_per_cu) *wgs_per_cu = c.b_wgs;
+        if (max_wgs_per_cu) *max_wgs_per_cu = sa::batched_max_wgs_per_cu(c.b_nbf, c.b_bt);
+        return 0;
+    }));
+}
+
+int slate_amd_batched_graph_stats(slate_amd_context ctx, int64_t* builds, int64_t* replays)
+{
+    return int(guarded(ctx, [&](sa::Context& c) -> int64_t {
+        if (builds) *builds = c.bgraph_builds;
+        if (replays) *replays = c.bgraph_replays;
+        return 0;
+    }));
 }
 
 // slate::gecondest (src/gecondest.cc): reciprocal condition number in the 1

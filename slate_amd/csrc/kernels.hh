@@ -61,15 +61,24 @@ void mlp_train(float* P, float* grad, float* m, float* v, const float* X, const 
                float lr, int t, float* loss, hipStream_t s);
 
 // Batched uneven solves (batched.hip): one persistent launch of `grid`
-// workgroups; elem_bytes 4 = float factor + FP64 refinement (mode[i] = 1
-// forces FP64 for system i), 8 = FP64 factor.  Device arrays throughout;
-// work: grid * batched_work_stride(elem_bytes) bytes; queue: one int.
-int64_t batched_work_stride(int elem_bytes);
+// workgroups of a tile configuration (float panel width nbf 16 / 32,
+// workgroup size bt 256 / 512).  Per-system mode: 0 FP32 factor + FP64 IR,
+// 1 FP64, 2 BF16 factor + FP64 IR, 3 learned choice, 4..6 learned path with
+// the choice forced to 0..2.  Device arrays throughout; work: grid *
+// batched_work_stride() bytes; queue: two ints, zero before the first launch
+// (each launch leaves them zero).  batched_fill_args writes the kernel's
+// argument block (at most BATCH_ARGS_BYTES) into out.
+constexpr int BATCH_ARGS_BYTES = 256;
+int64_t batched_work_stride(int elem_bytes = 4);
 int batched_max_n();
-void batched_solve(int elem_bytes, int count, const int* order, const int* n, const int* mode,
-                   const int64_t* offA, const int64_t* offB, const double* A, const double* B, double* X,
-                   int* iters, int* info, double* berr, int* queue, char* work, int grid, int itermax,
-                   double tol, hipStream_t s);
+bool batched_config_valid(int nbf, int bt);
+int batched_max_wgs_per_cu(int nbf, int bt);
+void batched_fill_args(void* out, int count, const int* order, const int* n, const int* mode, const int64_t* offA,
+                       const int64_t* offB, const double* A, const double* B, double* X, int* iters, int* info,
+                       double* berr, int* choice, float* feat, unsigned long long* ticks, const float* sel,
+                       int* queue, char* work, int itermax, double tol);
+void batched_launch(const void* args, int nbf, int bt, int grid, hipStream_t s);
+void batched_graph(void* args, int nbf, int bt, int grid, hipGraph_t* g, hipGraphExec_t* ge);
 
 // gemm_s32: C (fp32 row-major) += alpha * L R^T with float L (M x K) and
 // R (N x K), k contiguous, rows 16-byte aligned; each operand split into ns

@@ -59,7 +59,7 @@ def sample_system(rng, nmin=NMIN, nmax=NMAX):
     return n, kappa, int(rng.choice(SHAPES)), int(rng.integers(1, 2 ** 31))
 
 
-def label_system(n, kappa, shape, seed):
+def label_system(n, kappa, shape, seed, choices=None):
     """Solve one generated system with every choice; returns the features,
     per-choice cost (device ms, probe included), iteration codes and info,
     the probe's own time, and the label (cheapest choice).
@@ -69,7 +69,10 @@ def label_system(n, kappa, shape, seed):
     (which the selector always pays) plus that choice's own solve, which is
     timed by running the solve directly (gesv_mixed / gesv_mixed_gmres with its
     factor precision, or the plain FP64 gesv) instead of repeating the probe
-    inside gesv_auto."""
+    inside gesv_auto.
+
+    choices: the choices to time (default all); the others get cost NaN and
+    cannot be the label."""
     import torch
     import slate_amd as S
     A0 = S.empty_colmajor(n, n, torch.float64)
@@ -78,15 +81,18 @@ def label_system(n, kappa, shape, seed):
     S.generate(B, S.DIST_RANDS, seed + 7)
     X = S.empty_colmajor(n, 1, torch.float64)
     K = len(S.CHOICES)
-    cost = np.zeros(K)
+    cost = np.full(K, np.nan)
     iters = np.zeros(K, dtype=np.int32)
     infos = np.zeros(K, dtype=np.int32)
+    todo = set(range(K) if choices is None else choices) | {0}      # choice 0 reports the features
     feat = None
     A = torch.empty_like(A0)
     A.copy_(A0)
     S.selector_features(A)
     t_probe = S.last_time_ms()
     for c in range(K):
+        if c not in todo:
+            continue
         A.copy_(A0)
         if c in (0, 1):
             info, it, ch, _, f = S.gesv_auto(A, B, X, force=c)
@@ -104,17 +110,30 @@ def label_system(n, kappa, shape, seed):
             cost[c] = t_probe + S.last_time_ms()
         iters[c], infos[c] = it, info
     return {"n": n, "kappa": kappa, "shape": shape, "seed": seed, "features": feat, "cost": cost,
-            "iters": iters, "info": infos, "t_probe": t_probe, "label": int(np.argmin(cost))}
+            "iters": iters, "info": infos, "t_probe": t_probe, "label": int(np.nanargmin(cost))}
 
 
-def label_many(count, seed, nmin=NMIN, nmax=NMAX, log=None):
+def label_many(count, seed, nmin=NMIN, nmax=NMAX, log=None, max_seconds=None, out=None, every=500,
+               choices=None, full_every=0):
+    """Label up to count systems drawn with this seed; stops early after
+    max_seconds, and (with out) rewrites the .npz every `every` systems so a
+    run cut short keeps what it labelled.  choices: time only these (every
+    full_every-th system times all)."""
     rng = np.random.default_rng(seed)
     rows = []
-    t0 = time.time()
+    t0 = t_log = time.time()
     for i in range(count):
-        rows.append(label_system(*sample_system(rng, nmin, nmax)))
-        if log and (i + 1) % 100 == 0:
-            log("selector: %d/%d systems labelled (%.0f s)" % (i + 1, count, time.time() - t0))
+        ch = None if (choices is None or (full_every and i % full_every == 0)) else choices
+        rows.append(label_system(*sample_system(rng, nmin, nmax), choices=ch))
+        if log and ((i + 1) % 100 == 0 or time.time() - t_log > 30):
+            t_log = time.time()
+            log("selector: %d/%d systems labelled (%.0f s, last n = %d)" % (i + 1, count, t_log - t0, rows[-1]["n"]))
+        if out and (i + 1) % every == 0:
+            save_rows(out, rows)
+        if max_seconds is not None and time.time() - t0 > max_seconds:
+            if log:
+                log("selector: time budget reached after %d systems" % len(rows))
+            break
     return rows
 
 
@@ -158,9 +177,8 @@ def train(rows, steps=3000, batch=512, lr=3e-3, seed=0, log=None):
     return P.cpu().numpy(), loss
 
 
-def predict_numpy(params, features):
-    """Forward pass on the host (the layout of slate_amd/csrc/mlp.hip): the
-    choice per row.  Used for evaluating committed parameters without a GPU."""
+def _logits(params, features):
+    """Network outputs on the host (the layout of slate_amd/csrc/mlp.hip)."""
     import slate_amd as S
     F, H, K = S.SELECTOR_FEATURES, 64, len(S.CHOICES)
     P = np.asarray(params, dtype=np.float64)
@@ -174,7 +192,13 @@ def predict_numpy(params, features):
     X = np.asarray(features, dtype=np.float64)
     h1 = np.maximum(X @ W1 + b1, 0)
     h2 = np.maximum(h1 @ W2 + b2, 0)
-    return np.argmax(h2 @ W3 + b3, axis=1)
+    return h2 @ W3 + b3
+
+
+def predict_numpy(params, features):
+    """Forward pass on the host: the choice per row.  Used for evaluating
+    committed parameters without a GPU."""
+    return np.argmax(_logits(params, features), axis=1)
 
 
 def evaluate(params, rows, pred=None):
@@ -182,7 +206,10 @@ def evaluate(params, rows, pred=None):
     choice per system and against fixed policies: SLATE's default (float
     factor + classic IR without the probe), and always the same choice."""
     import slate_amd as S
-    cost = np.array([r["cost"] for r in rows])
+    cost = np.array([r["cost"] for r in rows], dtype=np.float64)
+    full = ~np.isnan(cost).any(axis=1)
+    # a choice that was not timed for a system costs as much as its slowest timed one
+    cost = np.where(np.isnan(cost), np.nanmax(cost, axis=1, keepdims=True), cost)
     label = np.array([r["label"] for r in rows])
     t_probe = np.array([r["t_probe"] for r in rows])
     if pred is None:
@@ -194,7 +221,7 @@ def evaluate(params, rows, pred=None):
     default = float(np.maximum(cost[:, S.DEFAULT_CHOICE] - t_probe, 0).sum())
     near = np.mean(cost[idx, pred] <= 1.1 * cost[idx, label])
     return {
-        "systems": len(rows),
+        "systems": len(rows), "systems_all_choices_timed": int(full.sum()),
         "misselection_rate": float(np.mean(pred != label)),
         "within_10pct_of_best_rate": float(near),
         "ms_mlp": float(mlp), "ms_best": float(best), "mlp_over_best": float(mlp / best),
@@ -226,13 +253,18 @@ def main(argv=None):
     ap.add_argument("--holdout-out", default=None, help="write the held-out systems here (.npz)")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--max-seconds", type=float, default=None, help="label: stop after this long")
+    ap.add_argument("--choices", default=None, help="label: comma-separated choices to time (default all)")
+    ap.add_argument("--full-every", type=int, default=0, help="label: every k-th system times all choices")
     args = ap.parse_args(argv)
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
     if args.cmd == "label":
-        rows = label_many(args.count, args.seed, args.nmin, args.nmax, log=log)
+        rows = label_many(args.count, args.seed, args.nmin, args.nmax, log=log, max_seconds=args.max_seconds,
+                          out=args.out, choices=None if not args.choices else [int(c) for c in args.choices.split(",")],
+                          full_every=args.full_every)
         save_rows(args.out, rows)
         log("selector: wrote %d systems to %s" % (len(rows), args.out))
         return

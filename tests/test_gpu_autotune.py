@@ -35,3 +35,16 @@ def test_committed_table_is_installed_and_used(gpu):
     _, ip1 = S.getrf(A1)
     _, ip2 = S.getrf(A2, S.Options(block_size=nb))
     assert (ip1 == ip2).all() and torch.equal(A1, A2)
+
+
+def test_committed_batched_configuration_is_installed(gpu):
+    with open(autotune.DEFAULT_PATH) as f:
+        data = json.load(f)
+    got = S.tune_get_batched()
+    say("batched configuration in the context: %s, committed %s" % (got, data.get("batched")))
+    if "batched" in data:
+        cfg = data["batched"]
+        assert got[:3] == (cfg["nbf"], cfg["bt"], cfg["wgs"])
+    else:
+        assert got[:3] == (32, 256, 1)
+    assert 1 <= got[2] <= got[3]

@@ -1,7 +1,10 @@
 """Batched uneven solves (one persistent launch): every instance against the
 oracle's FP64 solve of the same matgen system, SLATE's scaled-residual bound
 per instance, the refinement codes (0..30, -31 fallback, -3 singular float
-factor) and the per-system FP64 mode."""
+factor), the per-system FP64 mode, the BF16 factor, the learned choice
+(forced and by a network, against the host restatement of its decision),
+graph replay, every tile configuration, and the 4096-system workload
+against the plain FP64 batched path."""
 import numpy as np
 import pytest
 import torch
@@ -34,7 +37,7 @@ def unpack(X, ns):
     return out
 
 
-@pytest.mark.parametrize("prec", [slate_amd.PREC_FP32, slate_amd.PREC_FP64])
+@pytest.mark.parametrize("prec", [slate_amd.PREC_FP32, slate_amd.PREC_FP64, slate_amd.PREC_BF16])
 def test_batched_uneven_matches_oracle(gpu, prec):
     rng = np.random.default_rng(0)
     ns = [1, 2, 31, 32, 33, 63, 64, 65, 100, 257, 511, 1024] + list(rng.integers(3, 400, 36))
@@ -54,6 +57,8 @@ def test_batched_uneven_matches_oracle(gpu, prec):
         assert berr[i] <= 4 * n * np.finfo(np.float64).eps
         if prec == slate_amd.PREC_FP32:
             assert 0 <= iters[i] <= 30
+        elif prec == slate_amd.PREC_BF16:
+            assert 0 <= iters[i] <= 30 or iters[i] == -31, iters[i]
         else:
             assert iters[i] == 0
 
@@ -81,3 +86,140 @@ def test_batched_modes_fallback_and_singular(gpu):
     assert iters[1] == 0 and info[1] == 0
     assert 0 <= iters[2] <= 30 and info[2] == 0
     assert iters[3] == -3 and info[3] == 1
+
+
+def _check_solutions(ns, mats, rhs, dX, berr, tag):
+    X = unpack(dX.cpu().numpy(), ns)
+    for i, n in enumerate(ns):
+        r = scaled_residual(mats[i], X[i][:, None], rhs[i][:, None])
+        assert r <= 3.0, (tag, i, n, r)
+        assert berr[i] <= 4 * n * np.finfo(np.float64).eps, (tag, i, n, berr[i])
+
+
+@pytest.mark.parametrize("force", [0, 1, 2])
+def test_batched_auto_forced_choice(gpu, force):
+    rng = np.random.default_rng(5)
+    ns = [1, 7, 64, 129, 300, 1024] + list(rng.integers(3, 500, 18))
+    mats, rhs, dA, dB = make_batch(ns, 300)
+    dX = torch.zeros_like(dB)
+    iters, info, berr, ch, feat, ms = slate_amd.gesv_batched_auto(ns, dA, dB, dX, force=force,
+                                                                 want_features=True, want_ms=True)
+    say("forced %d: choices %s iters %s" % (force, np.bincount(ch, minlength=3), np.bincount(iters[iters >= 0])))
+    _check_solutions(ns, mats, rhs, dX, berr, "force %d" % force)
+    assert np.all(info == 0)
+    assert np.all(ch == force)
+    assert np.all(ms > 0)
+    assert np.all(np.isfinite(feat))
+    np.testing.assert_allclose(feat[:, 0], 1.0)
+    np.testing.assert_allclose(feat[:, 1], np.log2(ns) / 10, rtol=1e-6)
+    assert np.all(feat[:, 7:] == 0)
+    if force == 2:
+        assert np.all(iters == 0)
+    else:
+        assert np.all((iters >= 0) & (iters <= 30) | (iters == -31))
+
+
+def test_batched_auto_follows_network(gpu):
+    """A random network: the kernel's per-system choice equals the host
+    restatement of the forward pass on the features it reported."""
+    from slate_amd import batched, selector
+    rng = np.random.default_rng(9)
+    ns = list(rng.integers(16, 400, 40))
+    mats, rhs, dA, dB = make_batch(ns, 500)
+    for seed in (1, 2, 3):
+        P = selector.init_params(seed) * 3.0
+        dX = torch.zeros_like(dB)
+        iters, info, berr, ch, feat, _ = slate_amd.gesv_batched_auto(ns, dA, dB, dX, params=P, want_features=True)
+        say("network seed %d: choices %s" % (seed, np.bincount(ch, minlength=3)))
+        assert np.array_equal(ch, batched.predict(P, feat))
+        _check_solutions(ns, mats, rhs, dX, berr, "network %d" % seed)
+
+
+def test_batched_graph_replay_equals_eager(gpu):
+    rng = np.random.default_rng(11)
+    ns = list(rng.integers(8, 300, 30))
+    mats, rhs, dA, dB = make_batch(ns, 700)
+    X0 = torch.zeros_like(dB)
+    ref = slate_amd.gesv_batched(ns, dA, dB, X0, slate_amd.PREC_FP32)
+    b0, r0 = slate_amd.batched_graph_stats()
+    o = slate_amd.Options(use_graph=1)
+    X = torch.zeros_like(dB)                      # same buffers every call: the graph is replayed
+    for rep in range(3):
+        X.zero_()
+        out = slate_amd.gesv_batched(ns, dA, dB, X, slate_amd.PREC_FP32, opts=o)
+        torch.cuda.synchronize()
+        assert torch.equal(X, X0), rep
+        for a, b in zip(out, ref):
+            assert np.array_equal(a, b), rep
+        # eager work between replays
+        Y = torch.zeros_like(dB)
+        slate_amd.gesv_batched(ns, dA, dB, Y, slate_amd.PREC_FP32)
+        assert torch.equal(Y, X0)
+    b1, r1 = slate_amd.batched_graph_stats()
+    say("batched graph: builds %d -> %d, replays %d -> %d" % (b0, b1, r0, r1))
+    assert b1 - b0 == 1 and r1 - r0 == 2
+
+
+def test_batched_tile_configurations(gpu):
+    saved = slate_amd.tune_get_batched()
+    rng = np.random.default_rng(13)
+    ns = [1, 33, 1024] + list(rng.integers(2, 600, 13))
+    mats, rhs, dA, dB = make_batch(ns, 900)
+    try:
+        for nbf, bt in ((16, 256), (32, 256), (16, 512)):
+            for wgs in (1, 2):
+                slate_amd.tune_set_batched(nbf, bt, 1)
+                mx = slate_amd.tune_get_batched()[3]
+                if wgs > mx:
+                    continue
+                slate_amd.tune_set_batched(nbf, bt, wgs)
+                assert slate_amd.tune_get_batched()[:3] == (nbf, bt, wgs)
+                for prec in (slate_amd.PREC_FP32, slate_amd.PREC_FP64, slate_amd.PREC_BF16):
+                    dX = torch.zeros_like(dB)
+                    iters, info, berr = slate_amd.gesv_batched(ns, dA, dB, dX, prec)
+                    say("config nbf %d bt %d wgs %d (max %d) prec %d: %.1f ms" % (nbf, bt, wgs, mx, prec,
+                                                                                 slate_amd.last_time_ms()))
+                    _check_solutions(ns, mats, rhs, dX, berr, (nbf, bt, wgs, prec))
+        with pytest.raises(slate_amd.SlateError):
+            slate_amd.tune_set_batched(32, 512, 1)
+    finally:
+        slate_amd.tune_set_batched(*saved[:3])
+
+
+def test_batched_4096_systems_vs_fp64_batched(gpu):
+    """The batched workload at its full shape: 4096 systems, n uniform in
+    [32, 1024], half general and half SPD, kappa log-uniform in [1e1, 1e8]
+    (Householder generator); the learned choice per system (the committed
+    network) against the plain FP64 batched path on the same systems."""
+    from slate_amd import batched
+    b = batched.make_batch(4096, 2024, log=say)
+    ns, A, B = b["ns"], b["A"], b["B"]
+    X64 = torch.zeros_like(B)
+    it64, info64, berr64 = slate_amd.gesv_batched(ns, A, B, X64, slate_amd.PREC_FP64)
+    t64 = slate_amd.last_time_ms()
+    params = slate_amd.batched_params()
+    X = torch.zeros_like(B)
+    if params is not None:
+        iters, info, berr, ch, _, _ = slate_amd.gesv_batched_auto(ns, A, B, X, params=params)
+    else:
+        iters, info, berr, ch, _, _ = slate_amd.gesv_batched_auto(ns, A, B, X, force=1)
+    t = slate_amd.last_time_ms()
+    say("4096 systems: FP64 %.1f ms, learned %.1f ms (choices %s, fallbacks %d)"
+        % (t64, t, np.bincount(ch, minlength=3), int(np.sum(iters == -31))))
+    eps = np.finfo(np.float64).eps
+    assert np.all(info64 == 0) and np.all(info == 0)
+    assert np.all(berr64 <= 4 * ns * eps)
+    assert np.all(berr <= 4 * ns * eps), np.max(berr / (ns * eps))
+    # per instance: scaled residual on the GPU and forward error against FP64
+    xs, x64s = X.cpu().numpy(), X64.cpu().numpy()
+    for i in range(len(ns)):
+        n = int(ns[i])
+        Ai = A[b["offA"][i]:b["offA"][i + 1]].view(n, n).t()
+        bi = B[b["offB"][i]:b["offB"][i + 1]]
+        xi = X[b["offB"][i]:b["offB"][i + 1]]
+        r = bi - Ai @ xi
+        sr = float(r.abs().sum() / (n * Ai.abs().sum(dim=0).max() * xi.abs().sum()) / eps)
+        assert sr <= 3.0, (i, n, sr)
+        x, x64 = xs[b["offB"][i]:b["offB"][i + 1]], x64s[b["offB"][i]:b["offB"][i + 1]]
+        fe = np.abs(x - x64).max() / np.abs(x64).max()
+        assert fe <= 16 * b["kappa"][i] * n * eps, (i, n, b["kappa"][i], fe)
