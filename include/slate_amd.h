@@ -261,17 +261,18 @@ int64_t slate_amd_dgesv_batched(slate_amd_context ctx, int count, const int* n, 
                                 const double* dB, double* dX, const int* mode, int factor_precision,
                                 int* iters, int* info, double* berr, const slate_amd_options* opts);
 /* The same with the factorization precision chosen per system by the
- * selector network (SLATE_AMD_BATCHED_* choices): each system is factored in
- * BF16 first (the probe), solved once, and the probe's normwise quantities
- * (n, ||A|| ||x|| / ||b||, ||b - A x|| / (||A|| ||x||), ||b - A x|| / ||b||)
- * go through the network, which keeps the probe factor and refines (0),
- * factors again in FP32 and refines (1), or solves in FP64 (2); a
- * refinement that misses the tolerance falls back to FP64.  params: host
- * array of slate_amd_mlp_num_params() floats (the layout of the selector
- * network); force in 0..2 runs that choice for every system instead (params
- * may then be NULL).  choice (may be NULL): what ran per system;
- * features (may be NULL): count x 24 floats; ms (may be NULL): each
- * system's time on its workgroup. */
+ * selector network (SLATE_AMD_BATCHED_* choices): each system is factored
+ * first in the probe precision (options.factor_precision: FP32, the default,
+ * or BF16), solved once, and the probe's normwise quantities (n,
+ * ||A|| ||x|| / ||b||, ||b - A x|| / (||A|| ||x||), ||b - A x|| / ||b||) go
+ * through the network, which picks BF16 + IR (0), FP32 + IR (1) or FP64 (2);
+ * the probe factor is kept when it has the chosen precision (with an FP32
+ * probe, 0 and 1 both keep it), and a refinement that misses the tolerance
+ * falls back to FP64.  params: host array of slate_amd_mlp_num_params()
+ * floats (the layout of the selector network); force in 0..2 runs that
+ * choice for every system instead (params may then be NULL).  choice (may
+ * be NULL): what ran per system; features (may be NULL): count x 24 floats;
+ * ms (may be NULL): each system's time on its workgroup. */
 enum { SLATE_AMD_BATCHED_BF16 = 0, SLATE_AMD_BATCHED_FP32 = 1, SLATE_AMD_BATCHED_FP64 = 2 };
 int64_t slate_amd_dgesv_batched_auto(slate_amd_context ctx, int count, const int* n, const double* dA,
                                      const double* dB, double* dX, const float* params, int force, int* iters,

@@ -7,12 +7,14 @@ in [1e1, 1e8], A = U diag(sigma) V^T (SPD: V diag(sigma) V^T) with U, V
 products of n Householder reflectors and sigma geometric
 (matgen/generate_sigma.hh:61-111), one right-hand side each.
 
-The batched selector picks per system between keeping the BF16 probe factor
-(choice 0), factoring again in FP32 (1) and FP64 (2) from the probe's
-normwise quantities (features computed inside the kernel).  Labels come from
-the solver: the same batch is solved with each choice forced, every system's
-own time on its workgroup is measured, and the label is the cheapest choice
-(a refinement that misses pays for its FP64 fallback).
+The batched selector picks per system between BF16 + IR (choice 0), FP32 +
+IR (1) and FP64 (2) from the normwise quantities of a probe factor (FP32 by
+default, or BF16; features computed inside the kernel), keeping the probe
+factor when it has the chosen precision.  Labels come from the solver: the
+same batch is solved with each choice forced, every system's own time on its
+workgroup is measured, and the label is the cheapest choice (a refinement
+that misses pays for its FP64 fallback).  With an FP32 probe, choices 0 and 1
+run the same path, so only 1 and 2 are timed.
 
 Usage (on a GPU):
   python -m slate_amd.batched label --count 4096 --seed 1 --out b1.npz
@@ -58,18 +60,20 @@ def make_batch(count, seed, nmin=NMIN, nmax=NMAX, kmin=KMIN, kmax=KMAX, shapes=N
             "offA": offs, "offB": np.concatenate([[0], np.cumsum(ns.astype(np.int64))])}
 
 
-def label(batch, opts=None, log=None):
+def label(batch, probe="fp32", log=None):
     """Solve the batch with each choice forced; per-system features, costs
-    (ms on its workgroup), iteration codes, and the label."""
+    (ms on its workgroup; NaN for a choice not timed), iteration codes, and
+    the label."""
     import torch
     import slate_amd as S
     K = len(S.BATCHED_CHOICES)
     cnt = len(batch["ns"])
     X = torch.zeros_like(batch["B"])
-    cost = np.zeros((cnt, K))
+    cost = np.full((cnt, K), np.nan)
     iters = np.zeros((cnt, K), dtype=np.int32)
     feat = None
-    for c in range(K):
+    opts = S.Options(factor_precision=S.PREC_BF16 if probe == "bf16" else S.PREC_FP32)
+    for c in ((0, 1, 2) if probe == "bf16" else (1, 2)):
         X.zero_()
         it, info, berr, ch, f, ms = S.gesv_batched_auto(batch["ns"], batch["A"], batch["B"], X, force=c, opts=opts,
                                                        want_features=True, want_ms=True)
@@ -82,7 +86,8 @@ def label(batch, opts=None, log=None):
     rows = []
     for i in range(cnt):
         rows.append({"n": int(batch["ns"][i]), "kappa": float(batch["kappa"][i]), "sym": int(batch["sym"][i]),
-                     "features": feat[i], "cost": cost[i], "iters": iters[i], "label": int(np.argmin(cost[i]))})
+                     "features": feat[i], "cost": cost[i], "iters": iters[i],
+                     "label": int(np.nanargmin(cost[i]))})
     return rows
 
 
@@ -105,13 +110,19 @@ def load_rows(*paths):
 
 def predict(params, features):
     """Host restatement of the kernel's choice: argmax over the first three
-    network outputs, and a probe flagged unusable (feature 4) cannot be kept."""
+    network outputs; a probe flagged unusable (feature 4) cannot be kept
+    (BF16 probe, feature 7 = 1: FP32 instead; FP32 probe: FP64), and with an
+    FP32 probe choices 0 and 1 both keep it (FP32 + IR)."""
     import slate_amd as S
     from .selector import _logits
-    z = _logits(params, features)[:, :len(S.BATCHED_CHOICES)]
+    f = np.asarray(features)
+    z = _logits(params, f)[:, :len(S.BATCHED_CHOICES)]
     c = np.argmax(z, axis=1)
-    bad = np.asarray(features)[:, 4] != 0
-    c[(c == 0) & bad] = 1
+    bad = f[:, 4] != 0
+    bf = f[:, 7] != 0
+    c[bf & (c == 0) & bad] = 1
+    low = ~bf & (c <= 1)
+    c[low] = np.where(bad[low], 2, 1)
     return c
 
 
@@ -119,7 +130,8 @@ def evaluate(params, rows):
     """Misselection rate and summed system time of the network's choices
     against the best choice per system and against each fixed choice."""
     import slate_amd as S
-    cost = np.array([r["cost"] for r in rows])
+    cost = np.array([r["cost"] for r in rows], dtype=np.float64)
+    cost = np.where(np.isnan(cost), np.nanmax(cost, axis=1, keepdims=True), cost)
     lab = np.array([r["label"] for r in rows])
     pred = predict(params, np.array([r["features"] for r in rows]))
     idx = np.arange(len(rows))
@@ -148,6 +160,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=None)
     ap.add_argument("--holdout-out", default=None)
+    ap.add_argument("--probe", default="fp32", choices=["fp32", "bf16"])
     args = ap.parse_args(argv)
 
     def log(m):
@@ -155,7 +168,7 @@ def main(argv=None):
 
     if args.cmd == "label":
         batch = make_batch(args.count, args.seed, log=log)
-        rows = label(batch, log=log)
+        rows = label(batch, probe=args.probe, log=log)
         save_rows(args.out, rows)
         log("batched: wrote %d labelled systems to %s (labels %s)"
             % (len(rows), args.out, np.bincount([r["label"] for r in rows], minlength=3).tolist()))

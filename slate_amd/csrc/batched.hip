@@ -16,11 +16,11 @@
 //   bf16-operand MFMA trailing updates with FP32 accumulation) + FP64 IR,
 //   3 learned choice, 4/5/6 the learned path with its choice forced to
 //   0/1/2 (labelling).
-// The learned path factors in BF16 first (the probe), solves once and forms
-// the FP64 residual (iteration 0 of the refinement), and feeds the probe's
-// normwise quantities to the selector network (the MLP of mlp.hip, run by one
-// wave): choice 0 keeps the probe factor and refines, 1 factors again in
-// FP32 and refines, 2 factors in FP64.
+// The learned path factors in BF16 or FP32 first (the probe), solves once and
+// forms the FP64 residual (iteration 0 of the refinement), and feeds the
+// probe's normwise quantities to the selector network (the MLP of mlp.hip,
+// run by one wave), which picks what runs: BF16 + IR (0), FP32 + IR (1) or
+// FP64 (2) — keeping the probe factor when it has the chosen precision.
 //
 // Factorization: blocked right-looking LU with NB-column panels held in LDS
 // (up to 1024 rows, odd row stride so column reads are conflict free).  A
@@ -163,8 +163,13 @@ __device__ void wg_panel(T* __restrict__ s_p, int mr, int kb, int k0, int* s_piv
             T cur[NB];
             #pragma unroll
             for (int c = 0; c < NB; ++c) {
-                const T rv = row[c];
-                cur[c] = isp ? rj[c] : rv;
+                if (c >= j) {
+                    const T rv = row[c];
+                    cur[c] = isp ? rj[c] : rv;
+                }
+                else {
+                    cur[c] = rj[c];                 // read only for the pivot row's position
+                }
             }
             T l = T(0);
             #pragma unroll
@@ -418,40 +423,60 @@ __device__ __forceinline__ void wg_getrf(T* __restrict__ W, int n, int ldw, int*
 }
 
 // x := op(LU)^{-1} x for the factors in W (row-major), x of length n in
-// global memory (FP64).  Lower (unit) or upper, 64-row blocks: one wave solves
-// the diagonal block, the workgroup updates the rows below (above).
+// global memory (FP64).  Lower (unit) or upper, 64-row blocks: the diagonal
+// block is staged in LDS (s_blk, 64 x 65 T) and solved by wave 0 with
+// shuffles, the solved piece is kept in LDS (s_x) and the rows below (above)
+// are updated by the whole workgroup, one row per thread with eight
+// independent partial sums.
 template <class T, int BT>
-__device__ __forceinline__ void wg_trsv(const T* __restrict__ W, int n, int ldw, double* __restrict__ x, bool lower)
+__device__ __forceinline__ void wg_trsv(const T* __restrict__ W, int n, int ldw, double* __restrict__ x, bool lower,
+                                        T* s_blk, double* s_x)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = (n + 63) / 64;
     for (int bb = 0; bb < nblk; ++bb) {
         const int blk = lower ? bb : nblk - 1 - bb;
         const int b0 = blk * 64, bn = min(64, n - b0);
+        for (int e = tid; e < 64 * 64; e += BT) {
+            const int r = e >> 6, c = e & 63;
+            s_blk[r * 65 + c] = (r < bn && c < bn) ? W[int64_t(b0 + r) * ldw + b0 + c] : T(0);
+        }
+        __syncthreads();
         if (wave == 0) {
             double y = lane < bn ? x[b0 + lane] : 0.0;
             if (lower) {
                 for (int j = 0; j < bn; ++j) {
                     const double yj = __shfl(y, j, 64);
-                    if (lane > j && lane < bn) y -= double(W[int64_t(b0 + lane) * ldw + b0 + j]) * yj;
+                    if (lane > j) y -= double(s_blk[lane * 65 + j]) * yj;
                 }
             }
             else {
                 for (int j = bn - 1; j >= 0; --j) {
-                    if (lane == j) y /= double(W[int64_t(b0 + j) * ldw + b0 + j]);
+                    if (lane == j) y /= double(s_blk[j * 65 + j]);
                     const double yj = __shfl(y, j, 64);
-                    if (lane < j) y -= double(W[int64_t(b0 + lane) * ldw + b0 + j]) * yj;
+                    if (lane < j) y -= double(s_blk[lane * 65 + j]) * yj;
                 }
             }
             if (lane < bn) x[b0 + lane] = y;
+            s_x[lane] = lane < bn ? y : 0.0;
         }
         __syncthreads();
         const int r0 = lower ? b0 + bn : 0, r1 = lower ? n : b0;
         for (int i = r0 + tid; i < r1; i += BT) {
-            double s = 0.0;
             const T* wr = W + int64_t(i) * ldw + b0;
-            for (int j = 0; j < bn; ++j) s += double(wr[j]) * x[b0 + j];
-            x[i] -= s;
+            double ps[8];
+            #pragma unroll
+            for (int u = 0; u < 8; ++u) ps[u] = 0.0;
+            if (bn == 64) {
+                #pragma unroll
+                for (int j = 0; j < 64; j += 8)
+                    #pragma unroll
+                    for (int u = 0; u < 8; ++u) ps[u] += double(wr[j + u]) * s_x[j + u];
+            }
+            else {
+                for (int j = 0; j < bn; ++j) ps[j & 7] += double(wr[j]) * s_x[j];
+            }
+            x[i] -= ((ps[0] + ps[1]) + (ps[2] + ps[3])) + ((ps[4] + ps[5]) + (ps[6] + ps[7]));
         }
         __syncthreads();
     }
@@ -479,23 +504,48 @@ struct BatchArgs {
     int64_t work_stride;            // bytes per workgroup
     int itermax;
     double tol;                     // <= 0: eps * sqrt(n)
+    int probe_bf16;                 // learned path: probe factor in BF16 (1) or FP32 (0)
 };
 
+// sum_j |A(i, j)| (ABS) or sum_j A(i, j) v_j over the n columns of the
+// column-major A, with v read from LDS; eight independent partial sums keep
+// eight loads in flight per thread.
+template <bool ABS>
+__device__ __forceinline__ double row_dot(const double* __restrict__ A, int n, int i, const double* s_v)
+{
+    double ps[8];
+    #pragma unroll
+    for (int u = 0; u < 8; ++u) ps[u] = 0.0;
+    int j = 0;
+    for (; j + 8 <= n; j += 8)
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double a = A[i + int64_t(j + u) * n];
+            ps[u] += ABS ? fabs(a) : a * s_v[j + u];
+        }
+    for (; j < n; ++j) {
+        const double a = A[i + int64_t(j) * n];
+        ps[j & 7] += ABS ? fabs(a) : a * s_v[j];
+    }
+    return ((ps[0] + ps[1]) + (ps[2] + ps[3])) + ((ps[4] + ps[5]) + (ps[6] + ps[7]));
+}
+
 // r = b - A x (FP64, A column-major) and infinity norms: ||r|| in s_dred[0],
-// ||x|| in s_dred[1].
+// ||x|| in s_dred[1].  s_v: n doubles of LDS for x.
 template <int BT>
 __device__ void wg_residual(const double* __restrict__ A, int n, const double* __restrict__ b,
-                            const double* __restrict__ x, double* __restrict__ r, double* s_dred)
+                            const double* __restrict__ x, double* __restrict__ r, double* s_dred, double* s_v)
 {
     constexpr int NW = BT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < n; i += BT) s_v[i] = x[i];
+    __syncthreads();
     double rmax = 0, xmax = 0;
     for (int i = tid; i < n; i += BT) {
-        double s = b[i];
-        for (int j = 0; j < n; ++j) s -= A[i + int64_t(j) * n] * x[j];
+        const double s = b[i] - row_dot<false>(A, n, i, s_v);
         r[i] = s;
         rmax = fmax(rmax, fabs(s));
-        xmax = fmax(xmax, fabs(x[i]));
+        xmax = fmax(xmax, fabs(s_v[i]));
     }
     #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -523,9 +573,7 @@ __device__ void wg_norms(const double* __restrict__ A, int n, const double* __re
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double m = 0, vm = 0;
     for (int i = tid; i < n; i += BT) {
-        double s = 0;
-        for (int j = 0; j < n; ++j) s += fabs(A[i + int64_t(j) * n]);
-        m = fmax(m, s);
+        m = fmax(m, row_dot<true>(A, n, i, nullptr));
         vm = fmax(vm, fabs(v[i]));
     }
     #pragma unroll
@@ -598,6 +646,11 @@ void batched_solve_kernel(BatchArgs a)
     double* y = r + BMAXN;
     float* s_pf = reinterpret_cast<float*>(s_pan);
     double* s_pd = reinterpret_cast<double*>(s_pan);
+    // outside the factorizations the panel LDS holds the solves' diagonal
+    // block (64 x 65, up to FP64) and vectors: x for residuals, a solved piece
+    static_assert(pan_lds_bytes<NBF>() >= 64 * 65 * 8 + BMAXN * 8 + 64 * 8, "panel LDS too small for the solves");
+    double* s_v = reinterpret_cast<double*>(s_pan + 64 * 65 * 8);
+    double* s_x = s_v + BMAXN;
     const double eps = std::numeric_limits<double>::epsilon();
     while (true) {
         if (tid == 0) {
@@ -626,12 +679,12 @@ void batched_solve_kernel(BatchArgs a)
             for (int i = tid; i < n; i += BT) out[i] = rhs[perm[i]];
             __syncthreads();
             if (f64) {
-                wg_trsv<double, BT>(W64, n, ldw, out, true);
-                wg_trsv<double, BT>(W64, n, ldw, out, false);
+                wg_trsv<double, BT>(W64, n, ldw, out, true, s_pd, s_x);
+                wg_trsv<double, BT>(W64, n, ldw, out, false, s_pd, s_x);
             }
             else {
-                wg_trsv<float, BT>(W, n, ldw, out, true);
-                wg_trsv<float, BT>(W, n, ldw, out, false);
+                wg_trsv<float, BT>(W, n, ldw, out, true, s_pf, s_x);
+                wg_trsv<float, BT>(W, n, ldw, out, false, s_pf, s_x);
             }
         };
         auto factor32 = [&](bool bf) {
@@ -651,7 +704,7 @@ void batched_solve_kernel(BatchArgs a)
         // x is already in r / s_dred.  Returns the iterations, or -1 on a miss.
         auto refine = [&](bool have_r) {
             for (int it = 0; it <= a.itermax; ++it) {
-                if (!(have_r && it == 0)) wg_residual<BT>(A, n, b, x, r, s_dred);
+                if (!(have_r && it == 0)) wg_residual<BT>(A, n, b, x, r, s_dred, s_v);
                 const bool conv = s_dred[0] <= s_dred[1] * anorm * tol;
                 __syncthreads();
                 if (conv) return it;
@@ -680,12 +733,12 @@ void batched_solve_kernel(BatchArgs a)
             }
         }
         else {
-            // learned path: BF16 probe factor, one solve, iteration-0 residual
-            const int pinfo = factor32(true);
+            // learned path: probe factor (BF16 or FP32), one solve, iteration-0 residual
+            const int pinfo = factor32(a.probe_bf16 != 0);
             double xnorm = 0, rnorm = 0;
             if (pinfo == 0) {
                 lu_solve(false, b, x);
-                wg_residual<BT>(A, n, b, x, r, s_dred);
+                wg_residual<BT>(A, n, b, x, r, s_dred, s_v);
                 rnorm = s_dred[0];
                 xnorm = s_dred[1];
             }
@@ -705,6 +758,7 @@ void batched_solve_kernel(BatchArgs a)
                 else {
                     s_feat[4] = 1.f;
                 }
+                s_feat[7] = a.probe_bf16 ? 1.f : 0.f;
             }
             __syncthreads();
             if (a.feat && tid < SEL_F) a.feat[int64_t(sys) * SEL_F + tid] = s_feat[tid];
@@ -717,9 +771,21 @@ void batched_solve_kernel(BatchArgs a)
                 __syncthreads();
                 c = s_choice;
             }
-            if (c == 0 && s_feat[4] != 0.f) c = 1;          // singular or non-finite probe: cannot keep it
+            // what runs: 0 BF16 + IR, 1 FP32 + IR, 2 FP64.  A singular or
+            // non-finite probe cannot be kept (BF16 probe: FP32 instead; FP32
+            // probe: FP64); with an FP32 probe choices 0 and 1 both keep it.
+            const bool bad = s_feat[4] != 0.f;
+            bool keep;
+            if (a.probe_bf16) {
+                if (c == 0 && bad) c = 1;
+                keep = c == 0;
+            }
+            else {
+                if (c <= 1) c = bad ? 2 : 1;
+                keep = c == 1;
+            }
             ch = c;
-            if (c == 0) {
+            if (keep) {
                 const int it = refine(true);
                 if (it < 0) { need64 = true; iters = -a.itermax - 1; }
                 else iters = it;
@@ -748,7 +814,7 @@ void batched_solve_kernel(BatchArgs a)
             lu_solve(true, b, x);
         }
         // backward error
-        wg_residual<BT>(A, n, b, x, r, s_dred);
+        wg_residual<BT>(A, n, b, x, r, s_dred, s_v);
         if (tid == 0) {
             a.iters[sys] = iters;
             a.info[sys] = info;
@@ -798,10 +864,10 @@ int batched_max_wgs_per_cu(int nbf, int bt)
 void batched_fill_args(void* out, int count, const int* order, const int* n, const int* mode, const int64_t* offA,
                        const int64_t* offB, const double* A, const double* B, double* X, int* iters, int* info,
                        double* berr, int* choice, float* feat, unsigned long long* ticks, const float* sel,
-                       int* queue, char* work, int itermax, double tol)
+                       int* queue, char* work, int itermax, double tol, int probe_bf16)
 {
     BatchArgs a{count, order, n, mode, offA, offB, A, B, X, iters, info, berr, choice, feat, ticks, sel, queue,
-                work, batched_work_stride(4), itermax, tol};
+                work, batched_work_stride(4), itermax, tol, probe_bf16};
     static_assert(sizeof(BatchArgs) <= BATCH_ARGS_BYTES, "BatchArgs size");
     std::memcpy(out, &a, sizeof(a));
 }

@@ -2809,7 +2809,8 @@ static int64_t batched_run(sa::Context& c, int count, const int* n, const double
     alignas(16) char args[sa::BATCH_ARGS_BYTES] = {};
     sa::batched_fill_args(args, count, d_order, d_n, d_mode, d_offA, d_offB, dA, dB, dX, d_iters, d_info, d_berr,
                           d_choice, features ? d_feat : nullptr, ms ? d_ticks : nullptr, nsel ? d_sel : nullptr,
-                          c.bqueue.as<int>(), static_cast<char*>(c.bwork.p), int(o.itermax), o.tol);
+                          c.bqueue.as<int>(), static_cast<char*>(c.bwork.p), int(o.itermax), o.tol,
+                          oc.factor_precision == SLATE_AMD_PREC_BF16 ? 1 : 0);
     if (use_graph) {
         const int cfg[3] = {c.b_nbf, c.b_bt, grid};
         if (!c.bgraph_exec || memcmp(args, c.bgraph_args, sizeof(args)) != 0
@@ -2880,6 +2881,8 @@ int64_t slate_amd_dgesv_batched_auto(slate_amd_context ctx, int count, const int
         SA_REQUIRE(n && dA && dB && dX && iters && info && berr, -3, "null argument");
         SA_REQUIRE(force >= -1 && force <= 2, -7, "force must be -1 (network) or a choice 0..2");
         SA_REQUIRE(force >= 0 || params, -7, "selector parameters required unless a choice is forced");
+        SA_REQUIRE(!opts || opts->factor_precision == SLATE_AMD_PREC_FP32 || opts->factor_precision == SLATE_AMD_PREC_BF16,
+                   -8, "batched learned path: the probe factor_precision must be FP32 or BF16");
         std::vector<int> md(count, force < 0 ? 3 : 4 + force);
         return batched_run(c, count, n, dA, dB, dX, md.data(), force < 0 ? params : nullptr, iters, info, berr,
                            choice, features, ms, opts);

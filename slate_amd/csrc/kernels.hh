@@ -76,7 +76,7 @@ int batched_max_wgs_per_cu(int nbf, int bt);
 void batched_fill_args(void* out, int count, const int* order, const int* n, const int* mode, const int64_t* offA,
                        const int64_t* offB, const double* A, const double* B, double* X, int* iters, int* info,
                        double* berr, int* choice, float* feat, unsigned long long* ticks, const float* sel,
-                       int* queue, char* work, int itermax, double tol);
+                       int* queue, char* work, int itermax, double tol, int probe_bf16);
 void batched_launch(const void* args, int nbf, int bt, int grid, hipStream_t s);
 void batched_graph(void* args, int nbf, int bt, int grid, hipGraph_t* g, hipGraphExec_t* ge);
 

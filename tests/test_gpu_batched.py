@@ -96,23 +96,31 @@ def _check_solutions(ns, mats, rhs, dX, berr, tag):
         assert berr[i] <= 4 * n * np.finfo(np.float64).eps, (tag, i, n, berr[i])
 
 
+@pytest.mark.parametrize("probe", [slate_amd.PREC_FP32, slate_amd.PREC_BF16])
 @pytest.mark.parametrize("force", [0, 1, 2])
-def test_batched_auto_forced_choice(gpu, force):
+def test_batched_auto_forced_choice(gpu, force, probe):
+    """Forced choices after an FP32 or BF16 probe factor: what ran (an FP32
+    probe is kept for choices 0 and 1), the features, per-system times, and
+    every instance against the oracle's system."""
     rng = np.random.default_rng(5)
     ns = [1, 7, 64, 129, 300, 1024] + list(rng.integers(3, 500, 18))
     mats, rhs, dA, dB = make_batch(ns, 300)
     dX = torch.zeros_like(dB)
-    iters, info, berr, ch, feat, ms = slate_amd.gesv_batched_auto(ns, dA, dB, dX, force=force,
-                                                                 want_features=True, want_ms=True)
-    say("forced %d: choices %s iters %s" % (force, np.bincount(ch, minlength=3), np.bincount(iters[iters >= 0])))
+    iters, info, berr, ch, feat, ms = slate_amd.gesv_batched_auto(
+        ns, dA, dB, dX, force=force, opts=slate_amd.Options(factor_precision=probe), want_features=True,
+        want_ms=True)
+    say("probe %d forced %d: choices %s iters %s" % (probe, force, np.bincount(ch, minlength=3),
+                                                    np.bincount(iters[iters >= 0])))
     _check_solutions(ns, mats, rhs, dX, berr, "force %d" % force)
     assert np.all(info == 0)
-    assert np.all(ch == force)
+    expect = force if probe == slate_amd.PREC_BF16 else max(force, 1)
+    assert np.all(ch == expect)
     assert np.all(ms > 0)
     assert np.all(np.isfinite(feat))
     np.testing.assert_allclose(feat[:, 0], 1.0)
     np.testing.assert_allclose(feat[:, 1], np.log2(ns) / 10, rtol=1e-6)
-    assert np.all(feat[:, 7:] == 0)
+    assert np.all(feat[:, 7] == (1.0 if probe == slate_amd.PREC_BF16 else 0.0))
+    assert np.all(feat[:, 8:] == 0)
     if force == 2:
         assert np.all(iters == 0)
     else:
@@ -126,10 +134,11 @@ def test_batched_auto_follows_network(gpu):
     rng = np.random.default_rng(9)
     ns = list(rng.integers(16, 400, 40))
     mats, rhs, dA, dB = make_batch(ns, 500)
-    for seed in (1, 2, 3):
+    for seed, probe in ((1, slate_amd.PREC_FP32), (2, slate_amd.PREC_BF16), (3, slate_amd.PREC_BF16)):
         P = selector.init_params(seed) * 3.0
         dX = torch.zeros_like(dB)
-        iters, info, berr, ch, feat, _ = slate_amd.gesv_batched_auto(ns, dA, dB, dX, params=P, want_features=True)
+        iters, info, berr, ch, feat, _ = slate_amd.gesv_batched_auto(
+            ns, dA, dB, dX, params=P, opts=slate_amd.Options(factor_precision=probe), want_features=True)
         say("network seed %d: choices %s" % (seed, np.bincount(ch, minlength=3)))
         assert np.array_equal(ch, batched.predict(P, feat))
         _check_solutions(ns, mats, rhs, dX, berr, "network %d" % seed)
