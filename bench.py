@@ -1,18 +1,19 @@
 """Benchmark: FP64-accurate gesv TFLOP/s, mixed precision + iterative refinement
 vs plain FP64 LU, n = 65536, one MI355X.
 
-One step = one slate_amd.gesv_mixed_gmres call (float LU factorization +
-FP64 GMRES-IR to SLATE's stopping criterion, src/gesv_mixed_gmres.cc) on a
-matrix already resident in HBM.  By default the float factorization's
-trailing updates run as two-term split-BF16 products (three bf16 products per
-update, factor_precision BF16X2) on v_mfma_f32_16x16x32_bf16: measured on the
-same MI355X, that pairing is the fastest that reaches FP64 accuracy on this
-matrix (1332 ms, 22 GMRES iterations, against 1435 ms for three-term
-BF16X3 factors with classic IR, 7 iterations; BF16X2 with classic IR misses
-the tolerance).  --classic selects classic IR (src/gesv_mixed.cc),
---factor-precision bf16x3 / fp32 the FP32-accurate forms.  TFLOP/s counts FP64-equivalent LU flops
+One step = one FP64-accurate solve of A x = b (A = matgen rands seed 0,
+b = seed 1, both resident in HBM) in the MLP-selected form:
+slate_amd.gesv_auto with the committed selector network
+(slate_amd/selector_params.npy).  Inside the timed step it factors A in float
+with two-term split-BF16 trailing updates (the probe, BF16X2), computes the
+features (one pass over A, a sampled exponent histogram, the probe's first
+solve x0 and its FP64 residual), runs the network, and continues with the
+chosen factor precision and refinement scheme — on this matrix it keeps the
+probe factor and runs FP64 GMRES-IR (src/gesv_mixed_gmres.cc), reusing x0 and
+its residual as iteration 0.  --form fixed runs a fixed form instead
+(--factor-precision, --classic).  TFLOP/s counts FP64-equivalent LU flops
 (2 n^3 / 3), whatever precision executes them.  The same input is also solved
-once with the plain FP64 gesv for the "vs plain FP64 LU" comparison, and the
+with the plain FP64 gesv for the "vs plain FP64 LU" comparison, and the
 oracle's CPU restatement of the HostTask path is timed on a bounded sample.
 
 Prints ONE JSON line (see the keys below).  --gpus > 1 is not supported: this

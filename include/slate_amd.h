@@ -304,9 +304,13 @@ int slate_amd_tune_get_batched(slate_amd_context ctx, int* nbf, int* bt, int* wg
 /* ---- learned solver selection (no counterpart in the reference) ---------
  * For callers that want the factorization precision and refinement scheme
  * picked per system.  A probe factor in BF16X2 (the float LU with two-term
- * split-BF16 updates) gives the features: 24 floats (log2 n, log10 of the
- * infinity-norm condition estimate from the probe factor, per-tile max |a|
- * spread, a 16-bin exponent histogram, a constant).  The MLP (3 layers, 64
+ * split-BF16 updates) and its first solve give the features: 24 floats
+ * (log2 n; log10 ||A||_inf ||x0||_inf / ||b||_inf, a condition estimate from
+ * x0 = (LU)^{-1} b; per-tile max |a| spread; a 16-bin exponent histogram; a
+ * constant; log10 of the probe solve's backward error and relative residual;
+ * whether x0 already meets the stopping rule).  x0 and its residual are
+ * iteration 0 of the refinement that follows when the probe factor is kept.
+ * The MLP (3 layers, 64
  * hidden, FP32; slate_amd_mlp_num_params() floats) maps them to one of
  * SLATE_AMD_SELECTOR_CHOICES choices:
  *   0 BF16X2 + classic IR   1 BF16X2 + GMRES-IR   (keep the probe factor)
@@ -317,6 +321,7 @@ int slate_amd_tune_get_batched(slate_amd_context ctx, int* nbf, int* bt, int* wg
 enum { SLATE_AMD_SELECTOR_CHOICES = 8 };
 int slate_amd_mlp_num_params(void);
 int64_t slate_amd_selector_features(slate_amd_context ctx, int64_t n, const double* dA, int64_t lda,
+                                    const double* dB, double* dX /* n: receives x0 */,
                                     const slate_amd_options* opts, float* features /* host, 24 */);
 int slate_amd_mlp_forward(slate_amd_context ctx, const float* dparams, const float* dX, int batch,
                           float* dprobs);

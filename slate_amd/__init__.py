@@ -140,7 +140,7 @@ def lib():
             "slate_amd_batched_graph_stats": (i32, [vp, pi64, pi64]),
             "slate_amd_tune_set_batched": (i32, [vp, i32, i32, i32]),
             "slate_amd_tune_get_batched": (i32, [vp, pint, pint, pint, pint]),
-            "slate_amd_selector_features": (i64, [vp, i64, vp, i64, po, vp]),
+            "slate_amd_selector_features": (i64, [vp, i64, vp, i64, vp, vp, po, vp]),
             "slate_amd_mlp_forward": (i32, [vp, vp, vp, i32, vp]),
             "slate_amd_mlp_train_step": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_float, i32,
                                                ctypes.POINTER(ctypes.c_float)]),
@@ -509,12 +509,17 @@ def mlp_num_params():
     return int(lib().slate_amd_mlp_num_params())
 
 
-def selector_features(A, opts=None):
-    """The selector's 24 features of a column-major float64 CUDA matrix
-    (factors A in float internally; A itself is not modified)."""
+def selector_features(A, B, opts=None):
+    """The selector's 24 features of the system A x = B (column-major float64
+    CUDA A, one right-hand side; factors A in float internally, A and B are
+    not modified).  Returns (features, info)."""
+    import torch
     pa, lda, dev = _mat(A, ("float64",))
+    pb, _, _ = _mat(B, ("float64",))
+    _rows(A, B)
+    X = torch.empty(A.shape[0], dtype=torch.float64, device=A.device)
     f = np.zeros(SELECTOR_FEATURES, dtype=np.float32)
-    info = _check(lib().slate_amd_selector_features(context(dev), A.shape[0], pa, lda, _opts(opts),
+    info = _check(lib().slate_amd_selector_features(context(dev), A.shape[0], pa, lda, pb, X.data_ptr(), _opts(opts),
                                                     f.ctypes.data_as(ctypes.c_void_p)))
     return f, info
 

@@ -1316,6 +1316,14 @@ struct MixedState {
     double* norms;        // [0]: ||A||, [1..]: col norms
 };
 
+// Iteration 0 of a refinement already done by the selector's probe (one
+// right-hand side): X holds x0 = (LU)^{-1} b from the probe factor, the
+// state's R holds r0 = b - A x0, with ||A||_inf, ||x0||_inf and ||r0||_inf.
+struct PreStart {
+    bool valid = false;
+    double anorm = 0, nx = 0, nr = 0;
+};
+
 // internal_util.hh:121-138 (converged unless ||r_i|| > ||x_i|| * cte), with
 // one difference: a non-finite norm is "not converged".  The reference's
 // comparison is false for NaN and for inf > inf, so a refinement that diverges
@@ -1461,7 +1469,8 @@ template <class T> static T* as_mut(const T* p) { return const_cast<T*>(p); }
 // or null to set up and factor here.
 static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, double* A, int64_t lda,
                         int64_t* ipiv_host, const double* B, int64_t ldb, double* X, int64_t ldx,
-                        int* iter, const Opts& o, const MixedState* pre = nullptr, int64_t pre_info = 0)
+                        int* iter, const Opts& o, const MixedState* pre = nullptr, int64_t pre_info = 0,
+                        const PreStart* ps = nullptr)
 {
     const double eps = std::numeric_limits<double>::epsilon();
     const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
@@ -1469,13 +1478,18 @@ static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, 
     *iter = 0;
     SA_REQUIRE(!(sym && uplo == SLATE_AMD_UPPER), -2, "posv_mixed: Upper storage not supported");
     MixedState st = pre ? *pre : mixed_setup(c, n, nrhs);
-    const double Anorm = norm_host(c, A, n, lda, sym, st.work, st.work_elems, st.norms);
+    const bool started = ps && ps->valid && pre && nrhs == 1;
+    const double Anorm = started ? ps->anorm : norm_host(c, A, n, lda, sym, st.work, st.work_elems, st.norms);
     const double cte = Anorm * tol;
     std::vector<double> nx, nr;
     bool converged = false;
     int64_t info = pre ? pre_info : factor_lo(c, st, A, lda, n, sym, o);
     if (info != 0) {
         *iter = -3;
+    }
+    else if (started) {
+        nx.assign(1, ps->nx);                  // x0, r0 from the probe
+        nr.assign(1, ps->nr);
     }
     else {
         if (nrhs == 1) {
@@ -1489,6 +1503,8 @@ static int64_t mixed_ir(Context& c, int sym, int uplo, int64_t n, int64_t nrhs, 
         residual(A, n, lda, sym, X, ldx, B, ldb, st.R, n, nrhs, st.work, st.work_elems, c.s0);
         colnorms_host(c, X, n, ldx, nrhs, st.norms + 1, nx);
         colnorms_host(c, st.R, n, n, nrhs, st.norms + 1, nr);
+    }
+    if (info == 0) {
         if (converged_host(nr, nx, cte)) { *iter = 0; converged = true; }
         for (int64_t it = 0; it < itermax && !converged; ++it) {
             if (nrhs == 1) {
@@ -1654,7 +1670,8 @@ static inline void rot(double* x, double* y, double cc, double ss)
 // matrix and Givens rotations are on the host like the reference (:168-177).
 static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, int64_t lda,
                            int64_t* ipiv_host, const double* B, double* X, int* iter, const Opts& o,
-                           int64_t nb_hint, const MixedState* pre = nullptr, int64_t pre_info = 0)
+                           int64_t nb_hint, const MixedState* pre = nullptr, int64_t pre_info = 0,
+                           const PreStart* ps = nullptr)
 {
     const double eps = std::numeric_limits<double>::epsilon();
     const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
@@ -1670,7 +1687,8 @@ static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, 
     double* V = c.basis.as<double>();
     double* W = V + n * R1;
     double* dsc = W + n * R1;          // device scalars: [0..R1) h, [64..] z, [128] norm
-    const double Anorm = norm_host(c, A, n, lda, sym, st.work, st.work_elems, st.norms);
+    bool started = ps && ps->valid && pre;
+    const double Anorm = started ? ps->anorm : norm_host(c, A, n, lda, sym, st.work, st.work_elems, st.norms);
     const double cte = Anorm * tol;
     std::vector<double> H(R1 * R1, 0.0), S(R1, 0.0), ga(restart), gb(restart), hz(R1), zz(R1);
     std::vector<double> nx(1), nr(1);
@@ -1682,12 +1700,19 @@ static int64_t mixed_gmres(Context& c, int sym, int uplo, int64_t n, double* A, 
     else {
         const double negone = -1.0;
         SA_HIP(hipMemcpyAsync(dsc + 129, &negone, sizeof(double), hipMemcpyHostToDevice, c.s0));
-        lo_solve_vec(c, st, sym, n, B, X, false);
+        if (!started) lo_solve_vec(c, st, sym, n, B, X, false);
         int64_t iiter = 0;
         while (iiter < itermax) {
-            residual(A, n, lda, sym, X, n, B, n, st.R, n, 1, st.work, st.work_elems, c.s0);
-            colnorms_host(c, X, n, n, 1, st.norms + 1, nx);
-            colnorms_host(c, st.R, n, n, 1, st.norms + 1, nr);
+            if (started) {                     // x0, r0 and their norms from the probe
+                nx.assign(1, ps->nx);
+                nr.assign(1, ps->nr);
+                started = false;
+            }
+            else {
+                residual(A, n, lda, sym, X, n, B, n, st.R, n, 1, st.work, st.work_elems, c.s0);
+                colnorms_host(c, X, n, n, 1, st.norms + 1, nx);
+                colnorms_host(c, st.R, n, n, 1, st.norms + 1, nr);
+            }
             if (converged_host(nr, nx, cte)) { *iter = int(iiter); converged = true; break; }
             // v0 = r / ||r||
             copy_convert<double, double>(st.R, n, V, n, n, 1, c.s0);
@@ -1824,22 +1849,42 @@ static double norm1est_inv(int64_t n, double* xd, Solve solve)
 
 //------------------------------------------------------------------------------
 // Learned selection (no counterpart in the reference).  Features of a general
-// FP64 matrix, computed on the device: log2 n, the 1-norm condition estimate
-// from the float factor, the spread of the per-tile max |a_ij| and a 16-bin
-// histogram of binary exponents relative to the largest.  The float factor
-// is left in st for the solve that follows.
+// FP64 system A x = b, computed on the device:
+//   f0  log2 n / 16
+//   f1  log10 of ||A||_inf ||x0||_inf / ||b||_inf / 16, with x0 = (LU)^{-1} b
+//       from the probe factor: a lower bound on kappa_inf(A) that a random
+//       right-hand side makes tight within about sqrt(n) (it reads ~20 when
+//       the probe factor is singular)
+//   f2  log2(max / min) of the per-256x256-tile max |a_ij| / 64
+//   f3  standard deviation of log2 of those tile maxima / 16
+//   f4..f19  fractions of entries whose binary exponent lies 0..14 (and
+//       >= 15 or zero) below the largest one, over every column when
+//       n <= 4096 and about 4096 evenly spaced columns above
+//   f20 1
+//   f21 log10 of the probe solve's backward error ||r0|| / (||A|| ||x0||) / 16,
+//       r0 = b - A x0 in FP64
+//   f22 log10 ||r0|| / ||b|| / 16
+//   f23 1 if x0 already meets the refinement's stopping rule
+// One full pass over A gives the tile maxima and the row sums (so ||A||_inf),
+// one pass over the sampled columns the histogram; x0, r0 and the norms are
+// iteration 0 of the refinement, handed to the chosen driver in *ps when the
+// probe factor is kept.  The float factor is left in st.
 static constexpr int SEL_F = 24;
 
 static void selector_features(Context& c, MixedState& st, int64_t n, const double* A, int64_t lda,
-                              const Opts& o, int64_t* info, float* f)
+                              const double* B, double* X, const Opts& o, int64_t* info, float* f, PreStart* ps)
 {
     const int64_t t = ceildiv(n, 256);
-    c.flags.ensure(sizeof(double) * size_t(t * t) + sizeof(unsigned long long) * 16 + 256);
+    c.flags.ensure(sizeof(double) * size_t(t * t + n + t * n) + sizeof(unsigned long long) * 16 + 512);
     double* tmax = c.flags.as<double>();
-    unsigned long long* hist = reinterpret_cast<unsigned long long*>(tmax + t * t);
-    feature_tiles(A, lda, n, tmax, c.s0);
-    std::vector<double> tm(size_t(t * t));
+    double* rowsum = tmax + t * t;
+    unsigned long long* hist = reinterpret_cast<unsigned long long*>(rowsum + n);
+    double* part = reinterpret_cast<double*>(hist + 16);
+    feature_tiles_rows(A, lda, n, tmax, part, rowsum, c.s0);
+    std::vector<double> tm(static_cast<size_t>(t * t));
+    std::vector<double> rs(static_cast<size_t>(n));
     SA_HIP(hipMemcpyAsync(tm.data(), tmax, sizeof(double) * tm.size(), hipMemcpyDeviceToHost, c.s0));
+    SA_HIP(hipMemcpyAsync(rs.data(), rowsum, sizeof(double) * rs.size(), hipMemcpyDeviceToHost, c.s0));
     SA_HIP(hipStreamSynchronize(c.s0));
     double mx = 0, mn = std::numeric_limits<double>::infinity(), s1 = 0, s2 = 0;
     int64_t cnt = 0;
@@ -1847,45 +1892,57 @@ static void selector_features(Context& c, MixedState& st, int64_t n, const doubl
         mx = std::max(mx, v);
         if (v > 0) { mn = std::min(mn, v); double l = std::log2(v); s1 += l; s2 += l * l; ++cnt; }
     }
+    double anorm = 0;
+    for (double v : rs) anorm = std::max(anorm, v);
     int emax = 0;
     std::frexp(mx > 0 ? mx : 1.0, &emax);
+    const int64_t cstep = std::max<int64_t>(1, n / 4096);
     zero_u32(reinterpret_cast<unsigned*>(hist), 32, c.s0);
-    feature_hist(A, lda, n, emax, hist, c.s0);
+    feature_hist(A, lda, n, cstep, emax, hist, c.s0);
     unsigned long long h[16];
     SA_HIP(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, c.s0));
     *info = factor_lo(c, st, A, lda, n, 0, o);
-    double rcond = 0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tol = o.tol > 0 ? o.tol : eps * std::sqrt(double(n));
+    double nx = 0, nr = 0, nb = 0;
     if (*info == 0) {
-        const double anorm1 = [&] {
-            // ||A||_1 from the device (column sums): norm_inf of A^T is the
-            // row-sum norm of the transpose; use the residual workspace
-            return norm_host(c, A, n, lda, 0, st.work, st.work_elems, st.norms);
-        }();
-        // the infinity norm pairs with ||A^{-1}||_inf: solves with A^T and A swapped
-        const double ainv = norm1est_inv(n, st.R, [&](double* x, int tr) {
-            if (tr == 0) {
-                // x := A^{-T} x with the float factor (infinity-norm estimate)
-                copy_convert<double, float>(x, n, st.xlo, n, n, 1, c.s0);
-                getrs_vec<float>(c, st.Alo, n, 0, n, st.perm, st.xlo, st.xtmp, 1, c.s0);
-                copy_convert<float, double>(st.xlo, n, x, n, n, 1, c.s0);
-            }
-            else {
-                lo_solve_vec(c, st, 0, n, x, x, false);
-            }
-            SA_HIP(hipStreamSynchronize(c.s0));
-        });
-        rcond = (ainv > 0 && anorm1 > 0) ? 1.0 / (ainv * anorm1) : 0.0;
+        lo_solve_vec(c, st, 0, n, B, X, false);
+        residual(A, n, lda, 0, X, n, B, n, st.R, n, 1, st.work, st.work_elems, c.s0);
+        std::vector<double> v;
+        colnorms_host(c, X, n, n, 1, st.norms + 1, v);
+        nx = v[0];
+        colnorms_host(c, st.R, n, n, 1, st.norms + 1, v);
+        nr = v[0];
+    }
+    {
+        std::vector<double> v;
+        colnorms_host(c, B, n, n, 1, st.norms + 1, v);
+        nb = v[0];
     }
     SA_HIP(hipStreamSynchronize(c.s0));
-    const double total = double(n) * double(n);
+    const double ncols = double(ceildiv(n, cstep));
+    const double total = double(n) * ncols;
+    const double tiny = 1e-300;
+    const bool ok = *info == 0 && std::isfinite(nx) && std::isfinite(nr) && nx > 0 && nb > 0 && anorm > 0;
     for (int k = 0; k < SEL_F; ++k) f[k] = 0.f;
     f[0] = float(std::log2(double(n)) / 16.0);
-    f[1] = float((rcond > 0 ? std::log10(1.0 / rcond) : 20.0) / 16.0);
+    f[1] = float((ok ? std::log10(std::max(anorm * nx / nb, 1.0)) : 20.0) / 16.0);
     f[2] = float((cnt > 0 && mn < std::numeric_limits<double>::infinity()) ? std::log2(mx / mn) / 64.0 : 1.0);
     const double mean = cnt ? s1 / double(cnt) : 0.0;
     f[3] = float(cnt ? std::sqrt(std::max(0.0, s2 / double(cnt) - mean * mean)) / 16.0 : 0.0);
-    for (int b = 0; b < 16; ++b) f[4 + b] = float(double(h[b]) / total);
+    for (int k = 0; k < 16; ++k) f[4 + k] = float(double(h[k]) / total);
     f[20] = 1.0f;                                   // bias-like constant input
+    if (ok) {
+        f[21] = float(std::log10(std::max(nr / (anorm * nx), tiny)) / 16.0);
+        f[22] = float(std::log10(std::max(nr / nb, tiny)) / 16.0);
+        f[23] = (nr <= nx * anorm * tol) ? 1.f : 0.f;
+    }
+    if (ps) {
+        ps->valid = ok;
+        ps->anorm = anorm;
+        ps->nx = nx;
+        ps->nr = nr;
+    }
 }
 
 // The selector's choices: (factor precision, scheme).  The features come from
@@ -1924,7 +1981,8 @@ static int64_t mixed_select(Context& c, int64_t n, double* A, int64_t lda, int64
     MixedState st = mixed_setup(c, n, 1);
     int64_t finfo = 0;
     float f[SEL_F];
-    selector_features(c, st, n, A, lda, op, &finfo, f);
+    PreStart ps;
+    selector_features(c, st, n, A, lda, B, X, op, &finfo, f, &ps);
     if (feat_out) std::copy(f, f + SEL_F, feat_out);
     int ch = force;
     if (ch < 0) {
@@ -1965,9 +2023,9 @@ static int64_t mixed_select(Context& c, int64_t n, double* A, int64_t lda, int64
     const bool keep = sc.prec == SEL_PROBE;       // the probe factor is this choice's factor
     if (sc.gmres)
         return mixed_gmres(c, 0, SLATE_AMD_LOWER, n, A, lda, ipiv_host, B, X, iter, oc, o.nb,
-                           keep ? &st : nullptr, keep ? finfo : 0);
+                           keep ? &st : nullptr, keep ? finfo : 0, keep ? &ps : nullptr);
     return mixed_ir(c, 0, SLATE_AMD_LOWER, n, 1, A, lda, ipiv_host, B, n, X, n, iter, oc,
-                    keep ? &st : nullptr, keep ? finfo : 0);
+                    keep ? &st : nullptr, keep ? finfo : 0, keep ? &ps : nullptr);
 }
 
 } // namespace sa
@@ -2679,11 +2737,12 @@ int64_t slate_amd_tune_get_nb(slate_amd_context ctx, int kind, int64_t n)
 int slate_amd_mlp_num_params(void) { return sa::mlp_num_params(); }
 
 int64_t slate_amd_selector_features(slate_amd_context ctx, int64_t n, const double* dA, int64_t lda,
-                                    const slate_amd_options* opts, float* features)
+                                    const double* dB, double* dX, const slate_amd_options* opts, float* features)
 {
     return guarded(ctx, [&](sa::Context& c) -> int64_t {
         check_dims(n, lda);
-        SA_REQUIRE(features != nullptr, -6, "null features");
+        SA_REQUIRE(features != nullptr, -8, "null features");
+        SA_REQUIRE(dB != nullptr && dX != nullptr, -5, "null right-hand side or solution vector");
         SA_REQUIRE(n >= 2, -3, "selector features need n >= 2");
         SA_ENTRY_SYNC();
         sa::Opts o = sa::read_opts(opts);
@@ -2691,7 +2750,7 @@ int64_t slate_amd_selector_features(slate_amd_context ctx, int64_t n, const doub
         sa::Timer t(c);
         sa::MixedState st = sa::mixed_setup(c, n, 1);
         int64_t info = 0;
-        sa::selector_features(c, st, n, dA, lda, o, &info, features);
+        sa::selector_features(c, st, n, dA, lda, dB, dX, o, &info, features, nullptr);
         t.stop();
         sa::check_kernel_error(c);
         return info;

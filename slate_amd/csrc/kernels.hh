@@ -55,7 +55,10 @@ void larft_from_gram(const double* G, int64_t b, double* T, hipStream_t s);
 int mlp_num_params();
 int mlp_scratch_per_sample();
 void feature_tiles(const double* A, int64_t lda, int64_t n, double* tmax, hipStream_t s);
-void feature_hist(const double* A, int64_t lda, int64_t n, int emax, unsigned long long* hist, hipStream_t s);
+void feature_tiles_rows(const double* A, int64_t lda, int64_t n, double* tmax, double* part, double* rowsum,
+                        hipStream_t s);
+void feature_hist(const double* A, int64_t lda, int64_t n, int64_t cstep, int emax, unsigned long long* hist,
+                  hipStream_t s);
 void mlp_forward(const float* P, const float* X, int B, float* probs, hipStream_t s);
 void mlp_train(float* P, float* grad, float* m, float* v, const float* X, const int* y, int B, float* scratch,
                float lr, int t, float* loss, hipStream_t s);

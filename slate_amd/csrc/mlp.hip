@@ -52,7 +52,7 @@ void tile_maxabs_kernel(const double* __restrict__ A, int64_t lda, int64_t n, in
 // hist[b] += number of entries with emax - exponent(|a|) in bin b (b = 15
 // collects d >= 15 and exact zeros); exponent as frexp's.
 __global__ __launch_bounds__(256)
-void exp_hist_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int emax,
+void exp_hist_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int64_t cstep, int emax,
                      unsigned long long* __restrict__ hist)
 {
     // per-thread counters in registers (compile-time indices), one LDS
@@ -60,9 +60,9 @@ void exp_hist_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int e
     unsigned int c[16];
     #pragma unroll
     for (int k = 0; k < 16; ++k) c[k] = 0;
-    const int64_t total = n * n;
+    const int64_t total = n * ceildiv(n, cstep);            // columns 0, cstep, 2 cstep, ...
     for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total; t += int64_t(gridDim.x) * 256) {
-        const int64_t i = t % n, j = t / n;
+        const int64_t i = t % n, j = (t / n) * cstep;
         const double a = fabs(A[i + j * lda]);
         int b = 15;
         if (a != 0.0) {
@@ -84,6 +84,57 @@ void exp_hist_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int e
     }
 }
 
+// The same tile maxima, and each tile's row sums of |a_ij| (part[tj * n + i]
+// for tile column tj) for ||A||_inf, in the same single pass over A; the
+// partial sums are added in tile order by rowsum_reduce_kernel, so the norm
+// does not depend on scheduling.
+__global__ __launch_bounds__(256)
+void tile_maxabs_rows_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int64_t tiles,
+                             double* __restrict__ tmax, double* __restrict__ part)
+{
+    const int64_t ti = blockIdx.x, tj = blockIdx.y;
+    const int64_t i0 = ti * 256, j0 = tj * 256;
+    const int64_t i = i0 + threadIdx.x;
+    double m = 0.0, s = 0.0;
+    if (i < n) {
+        const int64_t j1 = std::min<int64_t>(j0 + 256, n);
+        for (int64_t j = j0; j < j1; ++j) {
+            const double a = fabs(A[i + j * lda]);
+            m = fmax(m, a);
+            s += a;
+        }
+        part[tj * n + i] = s;
+    }
+    __shared__ double red[4];
+    m = fmax(m, __shfl_xor(m, 32, 64));
+    for (int o = 16; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) tmax[ti * tiles + tj] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256)
+void rowsum_reduce_kernel(const double* __restrict__ part, int64_t n, int64_t tiles, double* __restrict__ rowsum)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t tj = 0; tj < tiles; ++tj) s += part[tj * n + i];
+    rowsum[i] = s;
+}
+
+void feature_tiles_rows(const double* A, int64_t lda, int64_t n, double* tmax, double* part, double* rowsum,
+                        hipStream_t s)
+{
+    const int64_t t = ceildiv(n, 256);
+    SA_REQUIRE(t < 65536, -2, "feature_tiles: matrix too large");
+    hipLaunchKernelGGL(tile_maxabs_rows_kernel, dim3(unsigned(t), unsigned(t)), dim3(256), 0, s, A, lda, n, t,
+                       tmax, part);
+    SA_CHECK_LAUNCH();
+    hipLaunchKernelGGL(rowsum_reduce_kernel, dim3(unsigned(ceildiv(n, 256))), dim3(256), 0, s, part, n, t, rowsum);
+    SA_CHECK_LAUNCH();
+}
+
 void feature_tiles(const double* A, int64_t lda, int64_t n, double* tmax, hipStream_t s)
 {
     const int64_t t = ceildiv(n, 256);
@@ -92,10 +143,12 @@ void feature_tiles(const double* A, int64_t lda, int64_t n, double* tmax, hipStr
     SA_CHECK_LAUNCH();
 }
 
-void feature_hist(const double* A, int64_t lda, int64_t n, int emax, unsigned long long* hist, hipStream_t s)
+void feature_hist(const double* A, int64_t lda, int64_t n, int64_t cstep, int emax, unsigned long long* hist,
+                  hipStream_t s)
 {
-    const unsigned g = unsigned(std::min<int64_t>(ceildiv(n * n, 256), 4096));
-    hipLaunchKernelGGL(exp_hist_kernel, dim3(g), dim3(256), 0, s, A, lda, n, emax, hist);
+    SA_REQUIRE(cstep >= 1, -2, "feature_hist: column step < 1");
+    const unsigned g = unsigned(std::min<int64_t>(ceildiv(n * ceildiv(n, cstep), 256), 4096));
+    hipLaunchKernelGGL(exp_hist_kernel, dim3(g), dim3(256), 0, s, A, lda, n, cstep, emax, hist);
     SA_CHECK_LAUNCH();
 }
 

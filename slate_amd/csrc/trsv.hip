@@ -22,9 +22,6 @@
 #include "common.hh"
 #include "kernels.hh"
 
-#include <algorithm>
-#include <cstdlib>
-
 namespace sa {
 
 template <class T> struct TvBS { static constexpr int v = 128; };
@@ -129,7 +126,7 @@ template <class T, int BS, int LOWER, int CM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const T* __restrict__ Dinv,
                           T* __restrict__ X, uint64_t* __restrict__ xw, uint32_t tag,
-                          uint64_t* __restrict__ progress, int* error, int xrun)
+                          uint64_t* __restrict__ progress, int* error)
 {
     constexpr int NWV = int(sizeof(T)) / 4;
     constexpr int RPW = BS / 4;                 // rows per wave (row-major product)
@@ -137,21 +134,7 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
     constexpr int NPART = 256 / BS;             // threads per row in the column-major product
     constexpr int KP = BS / NPART;              // columns per thread
     const int nblk = int(ceildiv(n, BS));
-    // Chain position q of this workgroup.  Workgroup g runs on XCD g % 8;
-    // with xrun = R > 0 the chain visits R consecutive positions on one XCD
-    // before moving to the next (position ((s / R) 8 + x) R + s % R for
-    // g = 8 s + x), so one step in R crosses XCDs instead of every step,
-    // while each XCD still holds positions spread over the whole chain.  The
-    // host enables it only when every workgroup is resident at once (a wait
-    // may then name a position whose workgroup was dispatched later).
-    int q = blockIdx.x;
-    if (xrun > 0) {
-        const int full = (nblk / (8 * xrun)) * (8 * xrun);
-        if (q < full) {
-            const int x = q & 7, sg = q >> 3;
-            q = ((sg / xrun) * 8 + x) * xrun + (sg % xrun);
-        }
-    }
+    const int q = blockIdx.x;
     const int b = LOWER ? q : nblk - 1 - q;
     const int64_t r0 = int64_t(b) * BS;
     const int rows = int((n - r0 < BS) ? n - r0 : BS);
@@ -325,24 +308,10 @@ void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T
     SA_REQUIRE(sync.tag != 0, -2, "trsv_lookback: tag 0 is reserved");
     unsigned nblk = unsigned(ceildiv(n, BS));
     int key = (lower ? 2 : 0) | (cm ? 1 : 0);
-    // XCD-local runs of the chain (SLATE_AMD_TRSV_XCD_RUN, default 8; 0 off),
-    // only when the whole grid is resident at once
-    static int cap = 0;
-    const char* e = std::getenv("SLATE_AMD_TRSV_XCD_RUN");
-    const int run_env = e ? std::max(0, std::atoi(e)) : 8;
-    if (cap == 0) {
-        int dev = 0, cus = 0, per = 0;
-        SA_HIP(hipGetDevice(&dev));
-        SA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        SA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)trsv_lookback_kernel<T, BS, 1, 0>,
-                                                            256, 0));
-        cap = std::max(1, cus * per);
-    }
-    const int xrun = (run_env > 0 && int(nblk) <= cap && int(nblk) >= 8 * run_env) ? run_env : 0;
 #define SA_TV_CASE(K, LO, CMV)                                                                      \
     case K:                                                                                         \
         hipLaunchKernelGGL((trsv_lookback_kernel<T, BS, LO, CMV>), dim3(nblk), dim3(256), 0, s,    \
-                           M, ld, n, Dinv, X, sync.words, sync.tag, sync.progress, sync.error, xrun); \
+                           M, ld, n, Dinv, X, sync.words, sync.tag, sync.progress, sync.error);    \
         break;
     switch (key) {
         SA_TV_CASE(0, 0, 0)

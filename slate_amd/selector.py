@@ -88,7 +88,7 @@ def label_system(n, kappa, shape, seed, choices=None):
     feat = None
     A = torch.empty_like(A0)
     A.copy_(A0)
-    S.selector_features(A)
+    S.selector_features(A, B)
     t_probe = S.last_time_ms()
     for c in range(K):
         if c not in todo:
@@ -138,6 +138,38 @@ def label_many(count, seed, nmin=NMIN, nmax=NMAX, log=None, max_seconds=None, ou
 
 
 FIELDS = ("n", "kappa", "shape", "seed", "features", "cost", "iters", "info", "t_probe", "label")
+
+
+def refeature(rows, log=None, out=None, every=500):
+    """Recompute the features of labelled systems (regenerated from their
+    n, kappa, spectrum shape and seed) with the current feature definition.
+    Every choice's cost includes the probe, so each cost moves by the change
+    in the probe's time; the label is recomputed (for the choices that keep
+    the probe factor this keeps the first solve and residual the new probe
+    computes in their cost, so they are, if anything, overcharged)."""
+    import torch
+    import slate_amd as S
+    out_rows = []
+    t0 = t_log = time.time()
+    for i, r in enumerate(rows):
+        n, kappa, shape, seed = int(r["n"]), float(r["kappa"]), int(r["shape"]), int(r["seed"])
+        A = S.empty_colmajor(n, n, torch.float64)
+        S.generate_svd(A, shape, kappa, seed)
+        B = S.empty_colmajor(n, 1, torch.float64)
+        S.generate(B, S.DIST_RANDS, seed + 7)
+        feat, _ = S.selector_features(A, B)
+        t_probe = S.last_time_ms()
+        cost = np.asarray(r["cost"], dtype=np.float64) - float(r["t_probe"]) + t_probe
+        nr = dict(r)
+        nr.update(features=feat, cost=cost, t_probe=t_probe, label=int(np.nanargmin(cost)))
+        out_rows.append(nr)
+        del A, B
+        if log and ((i + 1) % 100 == 0 or time.time() - t_log > 30):
+            t_log = time.time()
+            log("selector: %d/%d systems re-featured (%.0f s, last n = %d)" % (i + 1, len(rows), t_log - t0, n))
+        if out and (i + 1) % every == 0:
+            save_rows(out, out_rows)
+    return out_rows
 
 
 def save_rows(path, rows):
@@ -243,7 +275,7 @@ def split(rows, holdout=0.2, seed=0):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("cmd", choices=["label", "train"])
+    ap.add_argument("cmd", choices=["label", "train", "refeature"])
     ap.add_argument("--count", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--nmin", type=int, default=NMIN)
@@ -269,6 +301,11 @@ def main(argv=None):
         log("selector: wrote %d systems to %s" % (len(rows), args.out))
         return
     rows = load_rows(*args.data)
+    if args.cmd == "refeature":
+        rows = refeature(rows, log=log, out=args.out)
+        save_rows(args.out, rows)
+        log("selector: wrote %d re-featured systems to %s" % (len(rows), args.out))
+        return
     tr, ho = split(rows, args.holdout)
     params, loss = train(tr, steps=args.steps, log=log)
     np.save(args.out or DEFAULT_PARAMS, params)

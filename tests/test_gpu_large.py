@@ -193,22 +193,3 @@ def test_getrf_block_above_one_pass_swap_limit(gpu):
     say("  info %d scaled residual %.3f" % (info, r))
     assert info == 0 and r <= 3.0
 
-
-@pytest.mark.parametrize("dtype,n", [(torch.float32, 65536), (torch.float64, 32768)])
-def test_trsv_xcd_runs_bitwise_equal(gpu, dtype, n, monkeypatch):
-    """The look-back solve with XCD-local runs of the chain (the default)
-    gives the same bits as the plain dispatch order; both times reported."""
-    A = dev_rands(3, n, n, dtype)
-    info, ipiv = slate_amd.getrf(A)
-    assert info == 0
-    B = dev_rands(4, n, 1, dtype)
-    out = {}
-    for run in ("0", "8", "0", "8"):
-        monkeypatch.setenv("SLATE_AMD_TRSV_XCD_RUN", run)
-        X = B.clone()
-        slate_amd.getrs(A, ipiv, X)
-        out.setdefault(run, []).append((X, slate_amd.last_time_ms()))
-    say("getrs n=%d %s: plain order %s ms, XCD runs %s ms" % (n, dtype, [round(t, 2) for _, t in out["0"]],
-                                                            [round(t, 2) for _, t in out["8"]]))
-    assert torch.equal(out["0"][0][0], out["8"][0][0])
-    assert torch.equal(out["8"][0][0], out["8"][1][0])

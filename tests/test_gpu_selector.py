@@ -57,24 +57,32 @@ def test_mlp_adam_step_and_forward(gpu):
 
 
 def test_selector_features_track_kappa(gpu):
-    """The condition estimate from the BF16X2 probe factor: within a decade of
-    the true kappa_inf while kappa * u_probe stays below 1 (u ~ 2^-16 for the
-    two-term split); beyond, the probe's own error dominates and the estimate
-    only has to say "large"."""
+    """f1 = log10(||A|| ||x0|| / ||b||) / 16 from the BF16X2 probe's first
+    solve x0: at most kappa_inf (up to the probe's own error) and, for a
+    random right-hand side, within about sqrt(n) of it while kappa * u_probe
+    stays below 1 (u ~ 2^-16 for the two-term split); beyond, it only has to
+    say "large".  The probe solve is backward stable: its normwise backward
+    error (f21) stays near the probe's precision whatever kappa is."""
     n = 512
+    B = dev_rands(5, n, 1)
+    berr = []
     for kappa in (1e2, 1e4, 1e6, 1e10):
         A = slate_amd.empty_colmajor(n, n, torch.float64)
         slate_amd.generate_svd(A, slate_amd.SIGMA_GEO, kappa, 3)
-        f, info = slate_amd.selector_features(A)
+        f, info = slate_amd.selector_features(A, B)
         k_inf = np.linalg.cond(to_host(A), np.inf)
-        say("kappa %.0e: estimate 10^%.2f, true kappa_inf 10^%.2f" % (kappa, 16 * f[1], np.log10(k_inf)))
+        say("kappa %.0e: estimate 10^%.2f, true kappa_inf 10^%.2f, probe backward error 10^%.2f"
+            % (kappa, 16 * f[1], np.log10(k_inf), 16 * f[21]))
         assert info == 0
         if kappa <= 1e4:
-            assert abs(16 * f[1] - np.log10(k_inf)) <= 1.0
+            assert np.log10(k_inf) - np.log10(n) <= 16 * f[1] <= np.log10(k_inf) + 0.3
         else:
-            assert 16 * f[1] >= 4.0
+            assert 16 * f[1] >= 3.0
         assert abs(f[0] - np.log2(n) / 16) < 1e-6
         assert abs(f[4:20].sum() - 1.0) < 1e-5
+        assert f[20] == 1.0 and f[23] in (0.0, 1.0)
+        berr.append(f[21])
+    assert all(-10.0 <= 16 * b <= -4.0 for b in berr), berr
 
 
 @pytest.mark.parametrize("choice", list(range(len(slate_amd.CHOICES))))

@@ -30,3 +30,18 @@ def test_batched_selector_holdout():
     assert rep["mlp_over_best"] <= 1.10
     # no worse than the best single fixed choice (FP32 + IR, SLATE's gesv_mixed form)
     assert rep["mlp_over_best_fixed"] <= 1.0
+
+
+def test_main_selector_holdout():
+    """The committed precision/scheme network on the 809 systems held out of
+    its training (n in [512, 8192] plus a few up to 65536, kappa up to 1e14)."""
+    P = np.load(selector.DEFAULT_PARAMS, allow_pickle=False)
+    assert P.size == S.SELECTOR_FEATURES * 64 + 64 + 64 * 64 + 64 + 64 * len(S.CHOICES) + len(S.CHOICES)
+    rows = selector.load_rows(os.path.join(DATA, "selector_holdout.npz"))
+    rep = selector.evaluate(P, rows)
+    print(rep)
+    assert rep["systems"] >= 800
+    assert rep["misselection_rate"] <= 0.2
+    assert rep["mlp_over_best"] <= 1.10
+    assert rep["mlp_over_best_fixed"] < 1.0          # gains over every single fixed choice
+    assert rep["mlp_over_default"] < 1.0             # and over SLATE's default (FP32 + classic IR)
