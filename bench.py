@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--factor-precision", default="bf16x2", choices=sorted(PRECS),
                     help="factor precision of the mixed solve (bf16x3 = float-accurate split-BF16)")
     ap.add_argument("--classic", action="store_true", help="classic IR instead of GMRES-IR")
+    ap.add_argument("--method", default="partialpiv", choices=["partialpiv", "calu"],
+                    help="MethodLU of the low-precision factor (CALU: tournament pivoting, src/getrf_tntpiv.cc)")
     sel_params = os.path.join(ROOT, "slate_amd", "selector_params.npy")
     ap.add_argument("--form", default="auto" if os.path.exists(sel_params) else "fixed", choices=["auto", "fixed"],
                     help="auto: the learned selector picks the factor precision and the refinement scheme "
@@ -265,7 +267,8 @@ def main():
 
     n = args.n
     prec_code, products = PRECS[args.factor_precision]
-    opts = S.Options(block_size=args.nb, factor_precision=prec_code)
+    opts = S.Options(block_size=args.nb, factor_precision=prec_code,
+                     method_lu=S.LU_CALU if args.method == "calu" else S.LU_PARTIALPIV)
     gmres = not args.classic
     choices = []
     if args.form == "auto":
@@ -451,6 +454,7 @@ def main():
                            "gesv_mixed_gmres" if gmres else "gesv_mixed", args.factor_precision,
                            "GMRES-IR" if gmres else "classic IR", n, mat_desc),
             "n": n, "nrhs": 1, "nb": args.nb or "auto", "factor_precision": args.factor_precision,
+            "method_lu": args.method,
             "refinement": "GMRES-IR" if gmres else "classic IR", "iters": iters,
             "form": ("selector (slate_amd_dsgesv_auto, probe + features in the timed step): choice %d %s"
                      % (choices[-1], S.CHOICES[choices[-1]])) if args.form == "auto" else "fixed",
