@@ -106,7 +106,7 @@ int64_t slate_amd_sgetrs(slate_amd_context ctx, int trans, int64_t n, int64_t nr
  * allocate device memory (code under graph capture, serving loops, harnesses
  * that forbid hipMalloc).  Its workspace (device pivots, interchange plans,
  * inverted diagonal blocks, look-back exchange words) is a static device
- * array of 18 MiB that every GPU the library runs on holds until the process
+ * array of 36 MiB that every GPU the library runs on holds until the process
  * ends; n <= 16384 (larger n: -3).  Calls on one device share that array:
  * calls that overlap on one device race.  ipiv: host, 1-based (getrf's).
  * Same result, bit for bit, as slate_amd_dgetrs with nrhs = 1. */

@@ -1028,15 +1028,12 @@ static TvSync tv_next(Context& c, int64_t n)
     return t;
 }
 
-// Single right-hand side: x := op(LU)^{-1} x with x contiguous, using the
-// one-launch look-back triangular solves.  perm is the getrf permutation.
-// Workspace for the two inverted-diagonal-block sets of a triangular pair.
+// Workspace for the look-back solve data (trsv_diag_inverse) of a
+// triangular pair.
 template <class T>
 static T* dinv_pair(Context& c, int64_t n)
 {
-    const int64_t bs = trsv_block(int(sizeof(T)));
-    const int64_t per = ceildiv(n, bs) * bs * bs;
-    c.dinv.ensure(sizeof(T) * size_t(2 * per));
+    c.dinv.ensure(sizeof(T) * size_t(2 * trsv_aux_elems(n, int(sizeof(T)))));
     return c.dinv.as<T>();
 }
 
@@ -1046,9 +1043,8 @@ template <class T>
 static void getrs_vec(Context& c, const T* LU, int64_t ld, int cm, int64_t n, const int64_t* perm,
                       T* x, T* tmp, int trans, hipStream_t s)
 {
-    const int64_t bs = trsv_block(int(sizeof(T)));
     T* d0 = dinv_pair<T>(c, n);
-    T* d1 = d0 + ceildiv(n, bs) * bs * bs;
+    T* d1 = d0 + trsv_aux_elems(n, int(sizeof(T)));
     if (!trans) {
         trsv_diag_inverse<T>(LU, ld, cm, 1, 1, n, d0, s);
         trsv_diag_inverse<T>(LU, ld, cm, 0, 0, n, d1, s);
@@ -1275,9 +1271,8 @@ static void potrs_cm(Context& c, int uplo, T* A, int64_t lda, int64_t n, T* B, i
         // Lower column-major L (A = L L^T): L y = b, L^T x = y.
         // Upper column-major U (A = U^T U): U^T y = b, U x = y.
         const int cmL = (uplo == SLATE_AMD_UPPER) ? 0 : 1;
-        const int64_t bs = trsv_block(int(sizeof(T)));
         T* d0 = dinv_pair<T>(c, n);
-        T* d1 = d0 + ceildiv(n, bs) * bs * bs;
+        T* d1 = d0 + trsv_aux_elems(n, int(sizeof(T)));
         trsv_diag_inverse<T>(A, lda, cmL, 1, 0, n, d0, c.s0);
         trsv_diag_inverse<T>(A, lda, 1 - cmL, 0, 0, n, d1, c.s0);
         trsv_lookback<T>(A, lda, cmL, 1, n, d0, B, tv_next(c, n), c.s0);
@@ -1308,7 +1303,7 @@ struct MixedState {
     int64_t* perm;        // device, getrf permutation of the float factor
     float* xlo;           // row-major n x nrhs
     float* xtmp;          // scratch vector
-    float* dinvL;         // inverted diagonal blocks of the first solve
+    float* dinvL;         // look-back solve data (trsv_diag_inverse) of the first solve
     float* dinvU;         // ... and of the second
     double* R;            // column-major n x nrhs
     double* work;
@@ -1399,7 +1394,7 @@ static void from_lo(Context& c, const float* xlo, double* V, int64_t ldv, int64_
 static MixedState mixed_setup(Context& c, int64_t n, int64_t nrhs)
 {
     MixedState st{};
-    const int64_t dper = ceildiv(n, trsv_block(4)) * trsv_block(4) * trsv_block(4);
+    const int64_t dper = trsv_aux_elems(n, 4);
     c.lo.ensure(sizeof(float) * (size_t(n) * size_t(n) + size_t(2 * dper)) + 1024);
     st.Alo = c.lo.as<float>();
     st.dinvL = st.Alo + size_t(n) * size_t(n);

@@ -182,8 +182,9 @@ void trsm_lower_fused(const T* L, int64_t ldl, int64_t m, T* B, int64_t ldb, int
 
 // One-launch triangular solve of tri * x = b (one right-hand side, x overwrites
 // b, contiguous) with a look-back chain between blocks of trsv_block() rows.
-// Dinv holds the inverted diagonal blocks made by trsv_diag_inverse
-// (ceil(n/bs) * bs^2 elements; the unit/non-unit choice is folded into it).
+// Aux (trsv_aux_elems(n) elements) holds what trsv_diag_inverse prepares per
+// factor: the inverted diagonal blocks (the unit/non-unit choice is folded
+// into them) and each block's inverse times its last off-diagonal tile.
 // TvSync: words = 2n tagged exchange words, progress = one tagged word, both
 // zeroed when allocated; tag = nonzero and unused by earlier launches on those
 // words; error set to 2 if a wait times out.
@@ -194,11 +195,12 @@ struct TvSync {
     uint32_t tag;
 };
 int64_t trsv_block(int elem_bytes);
+int64_t trsv_aux_elems(int64_t n, int elem_bytes);
 template <class T>
-void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* Dinv,
+void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* Aux,
                        hipStream_t s);
 template <class T>
-void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T* Dinv, T* X,
+void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T* Aux, T* X,
                    const TvSync& sync, hipStream_t s);
 
 // Cholesky leaf on an nb x nb row-major block (nb <= 64), upper factor U with
@@ -280,7 +282,7 @@ void zero_f64(double* p, int64_t n, hipStream_t s);
 // No-allocation scratch (noalloc.hip): one device array per GPU, NOALLOC_BYTES,
 // holding the workspace of slate_amd_dgetrs_noalloc for n <= NOALLOC_NMAX.
 constexpr int64_t NOALLOC_NMAX = 16384;
-constexpr size_t NOALLOC_BYTES = size_t(18) << 20;
+constexpr size_t NOALLOC_BYTES = size_t(36) << 20;
 constexpr size_t noalloc_align(size_t b) { return (b + 255) & ~size_t(255); }
 struct NoallocLayout {
     static constexpr size_t ipiv1 = 0;
@@ -291,7 +293,7 @@ struct NoallocLayout {
     static constexpr size_t tvw = err + 256;
     static constexpr size_t tv_words = 2 * size_t(NOALLOC_NMAX) + 32;
     static constexpr size_t dinv = tvw + noalloc_align(8 * tv_words);
-    static constexpr size_t dinv_per = size_t(NOALLOC_NMAX) * 64;     // doubles per triangle (64-row blocks)
+    static constexpr size_t dinv_per = 2 * size_t(NOALLOC_NMAX) * 64; // doubles per triangle (64-row blocks)
     static constexpr size_t end = dinv + 2 * 8 * dinv_per;
 };
 static_assert(NoallocLayout::end <= NOALLOC_BYTES, "no-allocation scratch too small");

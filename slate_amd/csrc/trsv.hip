@@ -7,14 +7,19 @@
 // reached from src/getrs.cc:94-97).  Here the whole sweep is one launch:
 // block q (BS rows) is handled by workgroup q, which accumulates the products
 // with every already-solved block as soon as that block's solved piece has
-// been published (a "look-back" chain: blocks finish in order), then applies
-// the precomputed inverse of its diagonal block and publishes its own piece.
+// been published (a "look-back" chain: blocks finish in order) and then
+// publishes its own piece.
 //
-// The diagonal blocks are inverted once per factor by trsv_diag_inverse (the
-// inverted-diagonal-block form of trsm that GPU BLAS libraries use), so the
-// chain's critical path per block is: look-back wait, one BS x BS tile
-// product, an LDS row reduction and a BS x BS matrix-vector product whose
-// matrix was prefetched into registers before the wait.
+// Per factor, trsv_diag_inverse prepares two BS x BS matrices per block b:
+// the inverse D_b^{-1} of its diagonal block (the inverted-diagonal-block
+// form of trsm that GPU BLAS libraries use) and P_b = D_b^{-1} M_{b,l}, l the
+// block solved just before b.  With r' = x_b - sum over the other solved
+// blocks j of M_{b,j} x_j,
+//     x_b = D_b^{-1} r' - P_b x_l,
+// so everything but the last term is done while block l is still being
+// solved, and the chain's critical path per block is: look-back wait for
+// x_l, one BS x BS matrix-vector product with P_b held in registers, a
+// lane-shuffle reduction, the publication.
 //
 // Workgroups are dispatched in index order and only wait on lower indices,
 // so the chain always progresses; every wait is bounded and reports a timeout
@@ -29,24 +34,34 @@ template <> struct TvBS<double> { static constexpr int v = 64; };
 
 int64_t trsv_block(int elem_bytes) { return elem_bytes == 8 ? TvBS<double>::v : TvBS<float>::v; }
 
+int64_t trsv_aux_elems(int64_t n, int elem_bytes)
+{
+    const int64_t bs = trsv_block(elem_bytes);
+    return 2 * ceildiv(std::max<int64_t>(n, 1), bs) * bs * bs;
+}
+
 //------------------------------------------------------------------------------
-// Dinv[b] = inverse of diagonal block b (padded with identity past n), stored
-// so that element (t, k) of block b is at Dinv[b*BS*BS + k*BS + t].
+// Aux[b*BS*BS + t*BS + k] = D_b^{-1}(t, k) (padded with identity past n) and
+// Aux[(nblk + b)*BS*BS + t*BS + k] = P_b(t, k) = (D_b^{-1} M_{b,l})(t, k),
+// l = b - 1 (LOWER) or b + 1 (upper), zero where l does not exist and in the
+// rows and columns past n.
 // tri(i, k) = M[i*ld + k] (CM == 0) or M[i + k*ld] (CM == 1).
 template <class T, int BS, int LOWER, int UNIT, int CM>
 __global__ __launch_bounds__(256)
-void diag_inverse_kernel(const T* __restrict__ M, int64_t ld, int64_t n, T* __restrict__ Dinv)
+void diag_inverse_kernel(const T* __restrict__ M, int64_t ld, int64_t n, T* __restrict__ Aux)
 {
     __shared__ T Tl[BS][BS + 1];
     __shared__ T Xs[BS][BS + 1];          // Xs[i][t]: row i of inverse column t
+    const int nblk = int(ceildiv(n, BS));
     const int b = blockIdx.x;
     const int64_t r0 = int64_t(b) * BS;
     const int rows = int((n - r0 < BS) ? n - r0 : BS);
     const int tid = threadIdx.x;
+    auto elem = [&](int64_t i, int64_t k) { return CM ? M[i + k * ld] : M[i * ld + k]; };
     for (int e = tid; e < BS * BS; e += 256) {
         int i = e / BS, k = e % BS;
         T v;
-        if (i < rows && k < rows) v = CM ? M[(r0 + i) + (r0 + k) * ld] : M[(r0 + i) * ld + r0 + k];
+        if (i < rows && k < rows) v = elem(r0 + i, r0 + k);
         else v = (i == k) ? T(1) : T(0);
         Tl[i][k] = v;
     }
@@ -72,15 +87,29 @@ void diag_inverse_kernel(const T* __restrict__ M, int64_t ld, int64_t n, T* __re
         }
     }
     __syncthreads();
-    T* out = Dinv + int64_t(b) * BS * BS;
+    T* out = Aux + int64_t(b) * BS * BS;
+    for (int e = tid; e < BS * BS; e += 256) out[e] = Xs[e / BS][e % BS];
+    // the off-diagonal tile of the block solved just before this one
+    const int l = LOWER ? b - 1 : b + 1;
+    const bool has = l >= 0 && l < nblk;
+    const int64_t c0 = int64_t(has ? l : 0) * BS;
+    const int cols = has ? int((n - c0 < BS) ? n - c0 : BS) : 0;
     for (int e = tid; e < BS * BS; e += 256) {
-        int k = e / BS, t = e % BS;          // out[k*BS + t] = inv(t, k)
-        out[e] = Xs[t][k];
+        int i = e / BS, k = e % BS;
+        Tl[i][k] = (i < rows && k < cols) ? elem(r0 + i, c0 + k) : T(0);
+    }
+    __syncthreads();
+    T* pout = Aux + int64_t(nblk + b) * BS * BS;
+    for (int e = tid; e < BS * BS; e += 256) {
+        const int t = e / BS, k = e % BS;
+        T s = T(0);
+        for (int m = 0; m < BS; ++m) s += Xs[t][m] * Tl[m][k];
+        pout[e] = s;
     }
 }
 
 template <class T>
-void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* Dinv,
+void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int64_t n, T* Aux,
                        hipStream_t s)
 {
     if (n <= 0) return;
@@ -90,7 +119,7 @@ void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int6
 #define SA_DI_CASE(K, LO, UN, CMV)                                                                  \
     case K:                                                                                         \
         hipLaunchKernelGGL((diag_inverse_kernel<T, BS, LO, UN, CMV>), dim3(nblk), dim3(256), 0, s, \
-                           M, ld, n, Dinv);                                                         \
+                           M, ld, n, Aux);                                                          \
         break;
     switch (key) {
         SA_DI_CASE(0, 0, 0, 0)
@@ -108,6 +137,39 @@ void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int6
 }
 
 //------------------------------------------------------------------------------
+// Sum of v over the lanes whose index differs only in the bits of MASK
+// (a butterfly: every lane of the group ends with the same sum).
+template <int MASK, class T>
+__device__ inline T lanes_sum(T v)
+{
+    #pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        if (MASK & o) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Row sums of N rows of per-lane partials spread over the 32 lanes of a
+// half-wave: a register butterfly in which each exchange halves the rows a
+// lane keeps (the lanes whose index bit o is set keep the upper half and add
+// their partner's partials of it), o = 16, 8, ...  Afterwards a[0] holds the
+// sum over the lanes that agree in the exchanged bits, for row
+// (lane >> (5 - log2 N)) & (N - 1).  H is a template argument so every
+// register index is a constant (a run-time index into a register array
+// becomes a compare-and-select chain over the whole array).
+template <int H, int N, class T>
+__device__ __forceinline__ void row_butterfly(T (&a)[N], int lane)
+{
+    constexpr int o = 32 * H / N;
+    const bool up = (lane & o) != 0;
+    #pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const T send = up ? a[i] : a[i + H];
+        const T keep = up ? a[i + H] : a[i];
+        a[i] = keep + __shfl_xor(send, o, 64);
+    }
+    if constexpr (H > 1) row_butterfly<H / 2, N>(a, lane);
+}
+
 // Look-back sweep.  Block q publishes its solved piece twice: plainly into X
 // (the result) and as tagged words (32-bit payload + the launch's tag) into
 // xw, which is what later blocks read: a reader that sees the tag has the
@@ -116,23 +178,32 @@ void trsv_diag_inverse(const T* M, int64_t ld, int cm, int lower, int unit, int6
 // the waiting: blocks far from their turn sleep in proportion to the
 // distance instead of polling.
 //
-// Each dependency's BS x BS tile of the factor is loaded into registers
+// Each earlier block's BS x BS tile of the factor is loaded into registers
 // before its x arrives (it does not depend on x), so after the wait only the
-// x words, one LDS pass and the FMAs remain.
+// x words, one LDS pass and the FMAs remain.  Row-major tiles are read with
+// 16-byte loads: 32 lanes cover one row of the tile (VEC elements each), a
+// wave two rows per load, so every cache line is fetched by one instruction;
+// lane l keeps NI rows x VEC columns and the per-lane partial row sums are
+// folded by a register butterfly.  Column-major tiles are spread as one row
+// per lane and a wave-uniform column range per wave (scalar column
+// addresses), folded through LDS.
 template <class T>
 __device__ inline void tv_put(uint64_t* p, T v, uint32_t tag) { tw_store_val<T>(p, v, tag); }
 
 template <class T, int BS, int LOWER, int CM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const T* __restrict__ Dinv,
+void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const T* __restrict__ Aux,
                           T* __restrict__ X, uint64_t* __restrict__ xw, uint32_t tag,
                           uint64_t* __restrict__ progress, int* error)
 {
     constexpr int NWV = int(sizeof(T)) / 4;
-    constexpr int RPW = BS / 4;                 // rows per wave (row-major product)
-    constexpr int CPL = BS / 64;                // columns per lane (row-major product)
-    constexpr int NPART = 256 / BS;             // threads per row in the column-major product
-    constexpr int KP = BS / NPART;              // columns per thread
+    constexpr int VEC = 16 / int(sizeof(T));    // elements per 16-byte load (row-major tile)
+    constexpr int NI = BS * BS / (256 * VEC);   // row-major tile rows per lane
+    constexpr int LG = (NI == 16) ? 4 : (NI == 8) ? 3 : 2;         // log2 NI
+    constexpr int NPART = 256 / BS;             // lanes per row (column-major and diagonal products)
+    constexpr int KP = BS / NPART;              // columns per lane
+    constexpr int DLD = BS + 16 / int(sizeof(T));                  // padded row of D^{-1} in LDS
+    static_assert(BS == 32 * VEC && NI == (1 << LG) && NPART * KP == BS, "trsv tiling");
     const int nblk = int(ceildiv(n, BS));
     const int q = blockIdx.x;
     const int b = LOWER ? q : nblk - 1 - q;
@@ -140,57 +211,89 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
     const int rows = int((n - r0 < BS) ? n - r0 : BS);
     const int tid = threadIdx.x, lane = tid & 63;
     // Wave-uniform, so a tile's row (row-major) or column (column-major)
-    // addresses are scalar; that keeps the kernel at two waves per SIMD, so
-    // every block of an n = 65536 solve is resident from the start.
+    // bases are scalar; that keeps the kernel at two waves per SIMD, so every
+    // block of an n = 65536 solve is resident from the start.
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int vc = VEC * (lane & 31), vr = lane >> 5;   // row-major tile: column group, row parity
+    const int it = tid / NPART, ih = tid % NPART;       // row and column range of the diagonal products
+    const int ct = tid % BS, cw = (wave * 64) / BS;     // row and column range of a column-major tile
 
-    __shared__ uint32_t sxw[2][BS * NWV];       // x words of the dependency being applied
-    __shared__ T sr[BS];
-    __shared__ T spart[NPART][BS];
-    __shared__ T sred[4][RPW][65];
+    __shared__ __attribute__((aligned(16))) uint32_t sxw[2][BS * NWV];   // x words of a dependency
+    __shared__ __attribute__((aligned(16))) T sr[BS];                    // r'
+    __shared__ __attribute__((aligned(16))) T sdinv[BS * DLD];           // D_b^{-1}
+    __shared__ T spart[CM ? NPART : 1][BS];                              // column-major partial sums
 
-    // Prefetch this block's inverse: thread (it, ih) holds inv(it, ih*KP .. ih*KP+KP-1).
-    const int it = tid % BS, ih = (wave * 64) / BS;   // BS is a multiple of 64
-    T dinv[KP];
+    // P_b in registers (lane (it, ih) holds P_b(it, ih*KP .. ih*KP+KP-1)),
+    // D_b^{-1} in LDS, and this block's right-hand side entries.
+    T pm[KP];
     {
-        const T* dp = Dinv + int64_t(b) * BS * BS;
+        const T* pp = Aux + int64_t(nblk + b) * BS * BS + it * BS + ih * KP;
         #pragma unroll
-        for (int k = 0; k < KP; ++k) dinv[k] = dp[int64_t(ih * KP + k) * BS + it];
+        for (int k = 0; k < KP; ++k) pm[k] = pp[k];
+        const T* dp = Aux + int64_t(b) * BS * BS;
+        for (int e = tid; e < BS * BS; e += 256) sdinv[(e / BS) * DLD + e % BS] = dp[e];
     }
+    // the row whose r' this lane finishes (row-major: lane l keeps rows
+    // wave*2*NI + 2i + (l >> 5), and the butterfly leaves i = (l >> (5 - LG)) & (NI - 1))
+    const int xrow = CM ? ct : wave * 2 * NI + 2 * ((lane >> (5 - LG)) & (NI - 1)) + vr;
+    const T xrhs = X[r0 + (xrow < rows ? xrow : rows - 1)];
 
     const int ndep = LOWER ? b : nblk - 1 - b;
-    T acc_rm[RPW];
+    T acc_rm[NI];
     T acc_cm = T(0);
     #pragma unroll
-    for (int r = 0; r < RPW; ++r) acc_rm[r] = T(0);
+    for (int r = 0; r < NI; ++r) acc_rm[r] = T(0);
 
     // Tile registers.  Loads use clamped addresses (edge values are zeroed
-    // when applied), so a tile's loads issue back to back.
-    T mrm[CM ? 1 : RPW][CM ? 1 : CPL];
+    // when applied), so a tile's loads issue back to back.  A tile narrower
+    // than BS (the last block column) is only ever dependency 0 of an upper
+    // solve, so it takes element loads before the loop; inside the loop every
+    // row-major tile is BS wide and is read with 16-byte loads, the rows
+    // clamped through a 32-bit lane offset from the block's base.
+    T mrm[CM ? 1 : NI][CM ? 1 : VEC];
     T mcm[CM ? KP : 1];
+    auto tile_cols = [&](int t) {
+        const int64_t c0 = int64_t(LOWER ? t : nblk - 1 - t) * BS;
+        return int((n - c0 < BS) ? n - c0 : BS);
+    };
     auto load_tile = [&](int t) {
         const int j = LOWER ? t : nblk - 1 - t;
         const int64_t c0 = int64_t(j) * BS;
-        const int cols = int((n - c0 < BS) ? n - c0 : BS);
         if constexpr (CM) {
-            const int64_t row = r0 + (it < rows ? it : rows - 1);
-            const T* mp = M + row + c0 * ld;
+            const int cols = tile_cols(t);
+            const T* mp = M + r0 + (ct < rows ? ct : rows - 1) + c0 * ld;
             #pragma unroll
             for (int c = 0; c < KP; ++c) {
-                const int cc = ih * KP + c;
+                const int cc = cw * KP + c;
                 mcm[c] = mp[int64_t(cc < cols ? cc : cols - 1) * ld];
             }
         }
         else {
+            // row 2r + vr of the wave's rows: a wave-uniform (clamped) base
+            // row and one of two lane offsets (the odd row, or the base row
+            // again where the odd row lies past the block's last row)
+            const T* sb = M + r0 * ld + c0;
+            const uint32_t off0 = uint32_t(vc * int(sizeof(T)));
+            const uint32_t off1 = uint32_t((int64_t(vr) * ld + vc) * int64_t(sizeof(T)));
             #pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                const int i = wave * RPW + r;
-                const T* mp = M + (r0 + (i < rows ? i : rows - 1)) * ld + c0;
-                #pragma unroll
-                for (int u = 0; u < CPL; ++u) {
-                    const int c = CPL * lane + u;
-                    mrm[r][u] = mp[c < cols ? c : cols - 1];
-                }
+            for (int r = 0; r < NI; ++r) {
+                const int rb = wave * 2 * NI + 2 * r;
+                const char* base = reinterpret_cast<const char*>(sb + int64_t(rb < rows ? rb : rows - 1) * ld);
+                __builtin_memcpy(&mrm[r][0], base + (rb + 1 < rows ? off1 : off0), 16);
+            }
+        }
+    };
+    auto load_tile_edge = [&](int t) {            // row-major, narrower than BS
+        const int64_t c0 = int64_t(LOWER ? t : nblk - 1 - t) * BS;
+        const int cols = tile_cols(t);
+        #pragma unroll
+        for (int r = 0; r < NI; ++r) {
+            const int i = wave * 2 * NI + 2 * r + vr;
+            const T* mp = M + (r0 + (i < rows ? i : rows - 1)) * ld + c0;
+            #pragma unroll
+            for (int u = 0; u < VEC; ++u) {
+                const int c = vc + u;
+                mrm[r][u] = mp[c < cols ? c : cols - 1];
             }
         }
     };
@@ -198,13 +301,11 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
     const uint64_t t_start = wall_clock64();
     bool failed = false;
     unsigned known = 0;                         // thread 0: chain progress last seen
-    if (ndep > 0) load_tile(0);
-    #pragma unroll 1
-    for (int t = 0; t < ndep; ++t) {
+    // Wait until dependency t's x words are in sxw[t & 1].
+    auto wait_x = [&](int t) {
         const int j = LOWER ? t : nblk - 1 - t;
         const int64_t c0 = int64_t(j) * BS;
         const int cols = int((n - c0 < BS) ? n - c0 : BS);
-        const int buf = t & 1;
         // pace: sleep while the chain is more than two blocks away from t;
         // progress is re-read only while the last value seen is that far
         if (tid == 0 && !failed) {
@@ -219,6 +320,7 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
                 if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0
                     || wall_clock64() - t_start > 400000000ull) {
                     __hip_atomic_store(error, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    failed = true;
                     break;
                 }
             }
@@ -228,71 +330,93 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
         if (tid < BS * NWV) {
             uint32_t wv = 0;
             if (tid < cols * NWV) wv = tw_wait(xw + c0 * NWV + tid, tag, error, 2);
-            sxw[buf][tid] = wv;
+            sxw[t & 1][tid] = wv;
         }
         __syncthreads();
+    };
+
+    // ---- every dependency but the last: r' accumulation
+    if (ndep > 1) {
+        if (!CM && tile_cols(0) < BS) load_tile_edge(0);
+        else load_tile(0);
+    }
+    #pragma unroll 1
+    for (int t = 0; t + 1 < ndep; ++t) {
+        const int j = LOWER ? t : nblk - 1 - t;
+        const int64_t c0 = int64_t(j) * BS;
+        const int cols = int((n - c0 < BS) ? n - c0 : BS);
+        const int buf = t & 1;
+        wait_x(t);
         if (CM) {
             #pragma unroll
             for (int c = 0; c < KP; ++c) {
-                const int cc = ih * KP + c;
+                const int cc = cw * KP + c;
                 const T xv = lds_word_val<T>(&sxw[buf][cc * NWV]);
                 acc_cm += (cc < cols) ? mcm[c] * xv : T(0);
             }
         }
         else {
             #pragma unroll
-            for (int u = 0; u < CPL; ++u) {
-                const int c = CPL * lane + u;
+            for (int u = 0; u < VEC; ++u) {
+                const int c = vc + u;
                 const T xv = lds_word_val<T>(&sxw[buf][c * NWV]);
                 const T y = (c < cols) ? xv : T(0);
                 #pragma unroll
-                for (int r = 0; r < RPW; ++r) acc_rm[r] += mrm[r][u] * y;
+                for (int r = 0; r < NI; ++r) acc_rm[r] += mrm[r][u] * y;
             }
         }
-        if (t + 1 < ndep) load_tile(t + 1);
-        failed = failed || (tid == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0);
+        // No error-flag read here: a load issued after the next tile's loads
+        // completes after them, and thread 0 would wait for the whole tile
+        // before its next poll (two memory round trips per dependency
+        // instead of one).  The waits read the flag while they spin.
+        if (t + 2 < ndep) load_tile(t + 1);
     }
-    // ---- residual of this block's rows
+    // ---- r' of this block's rows into LDS
     if (CM) {
-        spart[ih][it] = acc_cm;
+        spart[cw][ct] = acc_cm;
         __syncthreads();
         if (tid < BS) {
             T sum = T(0);
             #pragma unroll
             for (int p = 0; p < NPART; ++p) sum += spart[p][tid];
-            T xv = X[r0 + (tid < rows ? tid : rows - 1)];
-            sr[tid] = (tid < rows) ? xv - sum : T(0);
+            sr[tid] = (tid < rows) ? xrhs - sum : T(0);
         }
     }
     else {
-        // transpose-reduce the per-lane partials through LDS
-        #pragma unroll
-        for (int r = 0; r < RPW; ++r) sred[wave][r][lane] = acc_rm[r];
-        __syncthreads();
-        if (tid < BS) {
-            const int w = tid / RPW, r = tid % RPW;
-            T sum = T(0);
-            #pragma unroll 16
-            for (int l = 0; l < 64; ++l) sum += sred[w][r][l];
-            T xv = X[r0 + (tid < rows ? tid : rows - 1)];
-            sr[tid] = (tid < rows) ? xv - sum : T(0);
-        }
+        row_butterfly<NI / 2, NI>(acc_rm, lane);
+        const T sum = lanes_sum<(1 << (5 - LG)) - 1>(acc_rm[0]);
+        if ((lane & ((1 << (5 - LG)) - 1)) == 0) sr[xrow] = (xrow < rows) ? xrhs - sum : T(0);
     }
     __syncthreads();
-    // ---- y = Dinv_b * r (partials over this thread's k range, folded in LDS)
+    // ---- y' = D_b^{-1} r' (row it, folded over the NPART lanes of the row)
+    T yv;
     {
-        T acc = T(0);
+        T a0 = T(0), a1 = T(0);
+        const T* dr = sdinv + it * DLD + ih * KP;
+        const T* rr = sr + ih * KP;
         #pragma unroll
-        for (int k = 0; k < KP; ++k) acc += dinv[k] * sr[ih * KP + k];
-        spart[ih][it] = acc;
+        for (int k = 0; k < KP; k += 2) {
+            a0 += dr[k] * rr[k];
+            a1 += dr[k + 1] * rr[k + 1];
+        }
+        yv = lanes_sum<NPART - 1>(a0 + a1);
     }
-    __syncthreads();
-    if (tid < rows) {
-        T yv = T(0);
+    // ---- the last dependency: x_b = y' - P_b x_l
+    if (ndep > 0) {
+        const int t = ndep - 1;
+        wait_x(t);
+        const uint32_t* xs = &sxw[t & 1][ih * KP * NWV];
+        T z0 = T(0), z1 = T(0);
         #pragma unroll
-        for (int p = 0; p < NPART; ++p) yv += spart[p][tid];
-        tv_put<T>(xw + (r0 + tid) * NWV, yv, tag);
-        X[r0 + tid] = yv;
+        for (int k = 0; k < KP; k += 2) {
+            z0 += pm[k] * lds_word_val<T>(xs + k * NWV);
+            z1 += pm[k + 1] * lds_word_val<T>(xs + (k + 1) * NWV);
+        }
+        yv -= lanes_sum<NPART - 1>(z0 + z1);
+    }
+    if (ih == 0 && it < rows) {
+        tv_put<T>(xw + (r0 + it) * NWV, yv, tag);
+        X[r0 + it] = yv;
     }
     if (tid == 0)
         __hip_atomic_fetch_max(progress, (uint64_t(tag) << 32) | uint32_t(q + 1), __ATOMIC_RELAXED,
@@ -300,7 +424,7 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
 }
 
 template <class T>
-void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T* Dinv, T* X,
+void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T* Aux, T* X,
                    const TvSync& sync, hipStream_t s)
 {
     if (n <= 0) return;
@@ -311,7 +435,7 @@ void trsv_lookback(const T* M, int64_t ld, int cm, int lower, int64_t n, const T
 #define SA_TV_CASE(K, LO, CMV)                                                                      \
     case K:                                                                                         \
         hipLaunchKernelGGL((trsv_lookback_kernel<T, BS, LO, CMV>), dim3(nblk), dim3(256), 0, s,    \
-                           M, ld, n, Dinv, X, sync.words, sync.tag, sync.progress, sync.error);    \
+                           M, ld, n, Aux, X, sync.words, sync.tag, sync.progress, sync.error);     \
         break;
     switch (key) {
         SA_TV_CASE(0, 0, 0)
