@@ -105,6 +105,47 @@ def test_getrf_threshold_zero_means_no_pivoting(gpu):
     assert np.abs(H - L @ np.triu(LU)).max() < 1e-10
 
 
+def _perm_from_ipiv(ipiv, n):
+    """perm with (P^T b)[i] = b[perm[i]] for LAPACK's 1-based interchange sequence."""
+    perm = np.arange(n)
+    for i, p in enumerate(np.asarray(ipiv, dtype=np.int64) - 1):
+        perm[i], perm[p] = perm[p], perm[i]
+    return perm
+
+
+@pytest.mark.parametrize("n", [129, 300, 1031])
+@pytest.mark.parametrize("trans", [False, True])
+def test_sgetrs_single_rhs_backward_error(gpu, n, trans):
+    """FP32 getrs with one right-hand side runs the look-back triangular
+    solves: a ragged last block, a narrow first tile in the upper sweep, the
+    column-major forms when transposed.  Its normwise backward error against
+    the float factors it used stays at the FP32 rounding level (8 n eps); a
+    wrong tile or index leaves an O(1) error."""
+    say("sgetrs n=%d nrhs=1 trans=%s" % (n, trans))
+    A = dev_rands(11, n, n, torch.float32)
+    info, ipiv = slate_amd.getrf(A)
+    assert info == 0
+    B = dev_rands(12, n, 1, torch.float32)
+    b = to_host(B)[:, 0]
+    slate_amd.getrs(A, ipiv, B, trans=trans)
+    x = to_host(B)[:, 0]
+    LU = to_host(A)
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    perm = _perm_from_ipiv(ipiv, n)
+    if trans:
+        # A^T x = U^T L^T P^T x
+        r = b - U.T @ (L.T @ x[perm])
+        scale = np.abs(U.T).sum(1).max() * np.abs(L.T).sum(1).max() * np.abs(x).max()
+    else:
+        # P^T A = L U
+        r = b[perm] - L @ (U @ x)
+        scale = np.abs(L).sum(1).max() * np.abs(U).sum(1).max() * np.abs(x).max()
+    berr = np.abs(r).max() / scale
+    say("  backward error %.3e (8 n eps = %.3e)" % (berr, 8 * n * np.finfo(np.float32).eps))
+    assert berr <= 8 * n * np.finfo(np.float32).eps
+
+
 @pytest.mark.parametrize("nrhs", [1, 7])
 @pytest.mark.parametrize("trans", [False, True])
 def test_getrs_matches_oracle(gpu, nrhs, trans):
