@@ -119,6 +119,8 @@ def test_gesv_mixed_split_reaches_fp64(gpu, prec, gmres):
     assert it >= 0 or it == -31                  # converged, or the FP64 fallback ran
     if prec == slate_amd.PREC_BF16X3:
         assert it >= 0                           # float-accurate factor: IR converges
+    if prec == slate_amd.PREC_BF16X2 and gmres:
+        assert 0 <= it <= 30                     # the benchmark's form converges without the fallback
     assert scaled_residual(A_in, to_host(X), to_host(B)) <= 3.0
 
 
