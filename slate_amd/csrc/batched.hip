@@ -201,7 +201,11 @@ __device__ void wg_panel(T* __restrict__ s_p, int mr, int kb, int k0, int* s_piv
 }
 
 // C (nr x nr, row stride ldw) -= L (nr x NB, LDS, row stride LD) * U (NB x nr,
-// row stride ldw), 32 x 32 wave tiles.
+// row stride ldw), 32 x 32 wave tiles.  A tile inside the matrix reads its 16
+// C values per lane before the products, unguarded, so they are in flight
+// together with the U operand (one memory round trip per tile; guarded
+// element-by-element updates after the products wait once per element).
+// Row of accumulator register r: 4 r + l/16 (FP64), 4 (l/16) + r (FP32).
 template <class T, int NB, int LD, int BT, bool BFU>
 __device__ void wg_update(T* __restrict__ C, const T* __restrict__ U, int ldw, const T* __restrict__ L, int nr)
 {
@@ -209,8 +213,22 @@ __device__ void wg_update(T* __restrict__ C, const T* __restrict__ U, int ldw, c
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int fr = lane & 15, fg = lane >> 4;
     const int tt = (nr + 31) >> 5;
+    auto crow = [&](int i0, int a, int r) {
+        return sizeof(T) == 8 ? i0 + 16 * a + 4 * r + fg : i0 + 16 * a + 4 * fg + r;
+    };
     for (int t = wave; t < tt * tt; t += NW) {
         const int i0 = (t / tt) << 5, j0 = (t % tt) << 5;
+        const bool full = i0 + 32 <= nr && j0 + 32 <= nr;
+        T cv[2][2][4];
+        if (full) {
+            #pragma unroll
+            for (int a = 0; a < 2; ++a)
+                #pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    #pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        cv[a][b][r] = C[int64_t(crow(i0, a, r)) * ldw + j0 + 16 * b + fr];
+        }
         if constexpr (sizeof(T) == 8) {
             v4d acc[2][2];
             #pragma unroll
@@ -244,8 +262,9 @@ __device__ void wg_update(T* __restrict__ C, const T* __restrict__ U, int ldw, c
                 for (int b = 0; b < 2; ++b)
                     #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int i = i0 + 16 * a + 4 * r + fg, jj = j0 + 16 * b + fr;
-                        if (i < nr && jj < nr) C[int64_t(i) * ldw + jj] -= acc[a][b][r];
+                        const int i = crow(i0, a, r), jj = j0 + 16 * b + fr;
+                        if (full) C[int64_t(i) * ldw + jj] = cv[a][b][r] - acc[a][b][r];
+                        else if (i < nr && jj < nr) C[int64_t(i) * ldw + jj] -= acc[a][b][r];
                     }
         }
         else {
@@ -310,8 +329,9 @@ __device__ void wg_update(T* __restrict__ C, const T* __restrict__ U, int ldw, c
                 for (int b = 0; b < 2; ++b)
                     #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int i = i0 + 16 * a + 4 * fg + r, jj = j0 + 16 * b + fr;
-                        if (i < nr && jj < nr) C[int64_t(i) * ldw + jj] -= acc[a][b][r];
+                        const int i = crow(i0, a, r), jj = j0 + 16 * b + fr;
+                        if (full) C[int64_t(i) * ldw + jj] = cv[a][b][r] - acc[a][b][r];
+                        else if (i < nr && jj < nr) C[int64_t(i) * ldw + jj] -= acc[a][b][r];
                     }
         }
     }
@@ -517,6 +537,15 @@ __device__ __forceinline__ double row_dot(const double* __restrict__ A, int n, i
     #pragma unroll
     for (int u = 0; u < 8; ++u) ps[u] = 0.0;
     int j = 0;
+    // 32 loads in flight per thread; column j + u still goes to partial sum
+    // u mod 8 in column order, as in the 8-column loop below
+    for (; j + 32 <= n; j += 32) {
+        double av[32];
+        #pragma unroll
+        for (int u = 0; u < 32; ++u) av[u] = A[i + int64_t(j + u) * n];
+        #pragma unroll
+        for (int u = 0; u < 32; ++u) ps[u & 7] += ABS ? fabs(av[u]) : av[u] * s_v[j + u];
+    }
     for (; j + 8 <= n; j += 8)
         #pragma unroll
         for (int u = 0; u < 8; ++u) {
