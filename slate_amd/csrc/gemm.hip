@@ -204,6 +204,30 @@ void gemm_kernel(GemmArgs p)
     }
 
     const T alpha = T(p.alpha), beta = T(p.beta);
+    // Interior tiles: C read per 16-row fragment row with no per-element
+    // branches, so the loads are in flight together (the guarded form below
+    // waits one memory round trip per element).
+    if (m0 + BM <= M && n0 + BN <= N && (p.uplo == 0 || n0 >= m0 + BM)) {
+        #pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            T cv[FN][4];
+            #pragma unroll
+            for (int j = 0; j < FN; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    cv[j][r] = beta != T(0)
+                        ? C[(m0 + wm * TM + i * 16 + Mf<T>::row(lane, r)) * ldc + n0 + wn * TN + j * 16 + fr] : T(0);
+            #pragma unroll
+            for (int j = 0; j < FN; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * cv[j][r];
+                    C[(m0 + wm * TM + i * 16 + Mf<T>::row(lane, r)) * ldc + n0 + wn * TN + j * 16 + fr] = v;
+                }
+        }
+        return;
+    }
     #pragma unroll
     for (int i = 0; i < FM; ++i)
         #pragma unroll
@@ -354,6 +378,30 @@ void gemm_f64_kernel(GemmArgs p)
     }
 
     const double alpha = p.alpha, beta = p.beta;
+    // Interior tiles: each 16-row block reads its 16 C values per lane with
+    // no per-element branches, so they are in flight together (the guarded
+    // element-by-element form below waits one memory round trip per element).
+    const bool full = m0 + BM <= M && n0 + BN <= N && (p.uplo == 0 || n0 >= m0 + BM);
+    if (full) {
+        double* cb = C + (m0 + wm * 64 + fk) * ldc + n0 + wn * 64 + fr;
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double cv[4][4];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) cv[j][r] = beta != 0.0 ? cb[(i * 16 + 4 * r) * ldc + j * 16] : 0.0;
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v += beta * cv[j][r];
+                    cb[(i * 16 + 4 * r) * ldc + j * 16] = v;
+                }
+        }
+        return;
+    }
     #pragma unroll
     for (int i = 0; i < 4; ++i)
         #pragma unroll
@@ -496,6 +544,29 @@ void gemm_f32_kernel(GemmArgs p)
     }
 
     const float alpha = float(p.alpha), beta = float(p.beta);
+    // Interior tiles: C read per 16-row block with no per-element branches
+    // (see gemm_f64_kernel).
+    const bool full = m0 + BM <= M && n0 + BN <= N && (p.uplo == 0 || n0 >= m0 + BM);
+    if (full) {
+        float* cb = C + (m0 + wm * 64 + 4 * fk) * ldc + n0 + wn * 64 + fr;
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float cv[4][4];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) cv[j][r] = beta != 0.0f ? cb[(i * 16 + r) * ldc + j * 16] : 0.0f;
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * cv[j][r];
+                    cb[(i * 16 + r) * ldc + j * 16] = v;
+                }
+        }
+        return;
+    }
     #pragma unroll
     for (int i = 0; i < 4; ++i)
         #pragma unroll

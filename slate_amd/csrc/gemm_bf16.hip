@@ -215,6 +215,25 @@ void gemm_bf16s_kernel(int64_t M, int64_t N, int64_t Kp, const __bf16* __restric
         }
     }
 
+    // Interior tiles: C read per 16-row block with no per-element branches,
+    // so the loads are in flight together (the guarded element-by-element
+    // update below waits one memory round trip per element).
+    if (m0 + BM <= M && n0 + BN <= N && (uplo == 0 || n0 >= m0 + BM)) {
+        float* cb = C + (m0 + wm * 64 + 4 * (lane >> 4)) * ldc + n0 + wn * 64 + fr;
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float cv[4][4];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) cv[j][r] = cb[(i * 16 + r) * ldc + j * 16];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) cb[(i * 16 + r) * ldc + j * 16] = cv[j][r] + alpha * acc[i][j][r];
+        }
+        return;
+    }
     #pragma unroll
     for (int i = 0; i < 4; ++i)
         #pragma unroll
@@ -414,6 +433,25 @@ void gemm_sp_kernel(int64_t M, int64_t N, int64_t K, const float* __restrict__ L
         }
     }
 
+    // Interior tiles: C read per 16-row block with no per-element branches,
+    // so the loads are in flight together (the guarded element-by-element
+    // update below waits one memory round trip per element).
+    if (m0 + BM <= M && n0 + BN <= N && (uplo == 0 || n0 >= m0 + BM)) {
+        float* cb = C + (m0 + wm * 64 + 4 * (lane >> 4)) * ldc + n0 + wn * 64 + fr;
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float cv[4][4];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) cv[j][r] = cb[(i * 16 + r) * ldc + j * 16];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j)
+                #pragma unroll
+                for (int r = 0; r < 4; ++r) cb[(i * 16 + r) * ldc + j * 16] = cv[j][r] + alpha * acc[i][j][r];
+        }
+        return;
+    }
     #pragma unroll
     for (int i = 0; i < 4; ++i)
         #pragma unroll
