@@ -117,6 +117,8 @@ void col_norms_max(const double* X, int64_t m, int64_t n, int64_t ldx, double* o
 // A[i,j] X[j,r].  Pass 1', sym only: colpart[b][r][j] = sum_{i > j, i in row
 // chunk b} A[i,j] X[i,r] (the mirrored upper triangle, column dots, one wave
 // per (column, row chunk)).  Pass 2 folds the partials in a fixed order.
+// A is streamed once per call, so its loads are nontemporal (measured at
+// n = 65536: 6.15 -> 6.45 TB/s for one row per thread).
 template <int NR>
 __global__ __launch_bounds__(256)
 void gemv_rows_kernel(const double* __restrict__ A, int64_t n, int64_t lda, int sym,
@@ -131,12 +133,53 @@ void gemv_rows_kernel(const double* __restrict__ A, int64_t n, int64_t lda, int 
     for (int r = 0; r < NR; ++r) acc[r] = 0;
     int64_t je = sym ? ((i + 1 < j1) ? i + 1 : j1) : j1;
     for (int64_t j = j0; j < je; ++j) {
-        double a = A[i + j * lda];
+        double a = __builtin_nontemporal_load(&A[i + j * lda]);
         #pragma unroll
         for (int r = 0; r < NR; ++r) acc[r] += a * X[j + r * ldx];
     }
     #pragma unroll
     for (int r = 0; r < NR; ++r) part[(c * NR + r) * n + i] = acc[r];
+}
+
+// The same with two rows (i, i + 1) per thread and one 16-byte nontemporal
+// load per column (A 16-byte aligned, lda and n even): half the load
+// instructions, each row summed in the same order as above (bitwise equal);
+// n = 65536: 5.59 -> 5.11 ms per pass over the 34 GB matrix (6.72 TB/s).
+typedef double gv_d2 __attribute__((ext_vector_type(2)));
+template <int NR>
+__global__ __launch_bounds__(256)
+void gemv_rows2_kernel(const double* __restrict__ A, int64_t n, int64_t lda, int sym,
+                       const double* __restrict__ X, int64_t ldx, double* __restrict__ part)
+{
+    const int64_t i = (int64_t(blockIdx.x) * 256 + threadIdx.x) * 2;
+    const int64_t c = blockIdx.y;
+    const int64_t j0 = c * RCH, j1 = (j0 + RCH < n) ? j0 + RCH : n;
+    if (i >= n) return;
+    double a0[NR], a1[NR];
+    #pragma unroll
+    for (int r = 0; r < NR; ++r) a0[r] = a1[r] = 0;
+    // sym: row i runs over j <= i, row i + 1 over j <= i + 1
+    const int64_t je0 = sym ? ((i + 1 < j1) ? i + 1 : j1) : j1;
+    const int64_t je1 = sym ? ((i + 2 < j1) ? i + 2 : j1) : j1;
+    for (int64_t j = j0; j < je0; ++j) {
+        const gv_d2 a = __builtin_nontemporal_load(reinterpret_cast<const gv_d2*>(&A[i + j * lda]));
+        #pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double x = X[j + r * ldx];
+            a0[r] += a[0] * x;
+            a1[r] += a[1] * x;
+        }
+    }
+    for (int64_t j = (je0 > j0 ? je0 : j0); j < je1; ++j) {
+        const double a = A[i + 1 + j * lda];
+        #pragma unroll
+        for (int r = 0; r < NR; ++r) a1[r] += a * X[j + r * ldx];
+    }
+    #pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        part[(c * NR + r) * n + i] = a0[r];
+        part[(c * NR + r) * n + i + 1] = a1[r];
+    }
 }
 
 static constexpr int CROWS = 4096;   // rows per column-dot chunk (sym residual)
@@ -199,8 +242,12 @@ static void residual_nr(const double* A, int64_t n, int64_t lda, int sym, const 
     int64_t nb = ceildiv(n, CROWS);
     double* part = work;
     double* cpart = sym ? work + nchunk * NR * n : nullptr;
-    hipLaunchKernelGGL(gemv_rows_kernel<NR>, dim3(unsigned(ceildiv(n, 256)), unsigned(nchunk)),
-                       dim3(256), 0, s, A, n, lda, sym, X, ldx, part);
+    if (n % 2 == 0 && lda % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0)
+        hipLaunchKernelGGL(gemv_rows2_kernel<NR>, dim3(unsigned(ceildiv(n, 512)), unsigned(nchunk)),
+                           dim3(256), 0, s, A, n, lda, sym, X, ldx, part);
+    else
+        hipLaunchKernelGGL(gemv_rows_kernel<NR>, dim3(unsigned(ceildiv(n, 256)), unsigned(nchunk)),
+                           dim3(256), 0, s, A, n, lda, sym, X, ldx, part);
     SA_CHECK_LAUNCH();
     if (sym) {
         hipLaunchKernelGGL(gemv_cols_below_kernel<NR>, dim3(unsigned(ceildiv(n, 4)), unsigned(nb)),

@@ -275,11 +275,16 @@ void trsv_lookback_kernel(const T* __restrict__ M, int64_t ld, int64_t n, const 
             const T* sb = M + r0 * ld + c0;
             const uint32_t off0 = uint32_t(vc * int(sizeof(T)));
             const uint32_t off1 = uint32_t((int64_t(vr) * ld + vc) * int64_t(sizeof(T)));
+            // the factor is streamed once per solve: nontemporal loads
+            typedef T tv_vec __attribute__((ext_vector_type(VEC), aligned(sizeof(T))));
             #pragma unroll
             for (int r = 0; r < NI; ++r) {
                 const int rb = wave * 2 * NI + 2 * r;
                 const char* base = reinterpret_cast<const char*>(sb + int64_t(rb < rows ? rb : rows - 1) * ld);
-                __builtin_memcpy(&mrm[r][0], base + (rb + 1 < rows ? off1 : off0), 16);
+                const tv_vec v = __builtin_nontemporal_load(
+                    reinterpret_cast<const tv_vec*>(base + (rb + 1 < rows ? off1 : off0)));
+                #pragma unroll
+                for (int u = 0; u < VEC; ++u) mrm[r][u] = v[u];
             }
         }
     };
