@@ -99,7 +99,7 @@ void tile_maxabs_rows_kernel(const double* __restrict__ A, int64_t lda, int64_t 
     if (i < n) {
         const int64_t j1 = std::min<int64_t>(j0 + 256, n);
         for (int64_t j = j0; j < j1; ++j) {
-            const double a = fabs(A[i + j * lda]);
+            const double a = fabs(__builtin_nontemporal_load(&A[i + j * lda]));
             m = fmax(m, a);
             s += a;
         }

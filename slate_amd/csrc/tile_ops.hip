@@ -438,9 +438,10 @@ void transpose_convert_kernel(const S* __restrict__ src, int64_t lds, D* __restr
     __shared__ D X[32][33];
     const int64_t I0 = int64_t(blockIdx.x) * 32, J0 = int64_t(blockIdx.y) * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    // the source is read once: nontemporal loads
     for (int bb = ty; bb < 32; bb += 8) {
         int64_t i = I0 + tx, j = J0 + bb;
-        X[tx][bb] = (i < m && j < n) ? cvt<D>(src[i + j * lds]) : D(0);
+        X[tx][bb] = (i < m && j < n) ? cvt<D>(__builtin_nontemporal_load(&src[i + j * lds])) : D(0);
     }
     __syncthreads();
     for (int aa = ty; aa < 32; aa += 8) {
