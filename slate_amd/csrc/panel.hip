@@ -742,8 +742,9 @@ void laswp_pass_kernel(T* __restrict__ A, int64_t ld, int64_t c0, int64_t ncol, 
         T* a = A + row * ld + c0 + col;
         T* t = tmp + int64_t(e) * ncol + col;
         if (VEC && col + VE <= ncol) {
-            if (PASS == 0) *reinterpret_cast<vec_t*>(t) = *reinterpret_cast<const vec_t*>(a);
-            else *reinterpret_cast<vec_t*>(a) = *reinterpret_cast<const vec_t*>(t);
+            // each source element is read once: nontemporal loads
+            if (PASS == 0) *reinterpret_cast<vec_t*>(t) = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(a));
+            else *reinterpret_cast<vec_t*>(a) = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(t));
         }
         else {
             for (int q = 0; q < VE && col + q < ncol; ++q) {
