@@ -160,11 +160,15 @@ void gemm_lt_kernel(int64_t M, int64_t N, int64_t Kp, const __bf16* __restrict__
             #pragma unroll
             for (int j = 0; j < 4; ++j)
                 #pragma unroll
-                for (int r = 0; r < 4; ++r) cv[j][r] = cb[(i * 16 + r) * ldc + j * 16];
+                for (int r = 0; r < 4; ++r) cv[j][r] = __builtin_nontemporal_load(&cb[(i * 16 + r) * ldc + j * 16]);
+            // C is read and written once per launch: nontemporal, so it does
+            // not displace the operand planes from L2 (replay of the n = 65536
+            // updates: 8.98 -> 8.93 ms per launch)
             #pragma unroll
             for (int j = 0; j < 4; ++j)
                 #pragma unroll
-                for (int r = 0; r < 4; ++r) cb[(i * 16 + r) * ldc + j * 16] = cv[j][r] + alpha * acc[i][j][r];
+                for (int r = 0; r < 4; ++r)
+                    __builtin_nontemporal_store(cv[j][r] + alpha * acc[i][j][r], &cb[(i * 16 + r) * ldc + j * 16]);
         }
         return;
     }

@@ -189,7 +189,7 @@ void trsm_lunit_kernel(TrsmProb<T> P)
             #pragma unroll
             for (int e = 0; e < VN; ++e) acc[e] = T(0);
             if (P.vec_ok) {
-                #pragma unroll 4
+                #pragma unroll 8
                 for (int k = 0; k < p0; k += VN) {
                     const V l = *reinterpret_cast<const V*>(Li + k);
                     const V x = *reinterpret_cast<const V*>(Xc + k);
