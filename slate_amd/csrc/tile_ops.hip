@@ -114,12 +114,17 @@ template void trsm_leaf<float>(const float*, int64_t, int, int, int, int64_t, fl
 // m <= trsm_fused_max.
 // Workgroup x owns the columns [8x, 8x + 8), kept in LDS k-contiguous.  The
 // sweep is left-looking over 32-row blocks: every thread owns one (row,
-// column) of the block and forms y = b - L(i, 0:p0) X(0:p0, c) with 16-byte
-// loads of the L row (L1/L2 resident: every workgroup reads the same
-// triangle) and of the X column; then the block's forward substitution runs
-// in registers, each wave holding two columns (one per 32-lane half) and
-// broadcasting each solved entry with v_readlane.  Two barriers per block,
-// no launches inside the sweep: the recursive form needs 2 m/32 - 1 launches.
+// column) of the block and forms y = b - L(i, 0:p0) X(0:p0, c) from LDS; then
+// the block's forward substitution runs in registers, each wave holding two
+// columns (one per 32-lane half) and broadcasting each solved entry with
+// v_readlane.  The block row L(p0:p0+32, 0:p0+32) reaches LDS in chunks of
+// 32 rows x 512 bytes, each loaded once per workgroup with coalesced 16-byte
+// loads (consecutive threads read consecutive pieces of a row) and the next
+// chunk's loads in flight while the current one is multiplied.  Reading the
+// row straight from memory instead, one row per lane, makes every load touch
+// 32 rows and leaves the kernel bound by address processing (1.7-2x slower,
+// the same sums).  Two barriers per chunk, no launches inside the sweep: the
+// recursive form needs 2 m/32 - 1 launches.
 template <class T> __device__ __forceinline__ T bcast_half(T v, int k, bool upper);
 template <> __device__ __forceinline__ float bcast_half<float>(float v, int k, bool upper)
 {
@@ -149,12 +154,19 @@ struct TrsmProb {
     const T* L; int64_t ldl; T* B; int64_t ldb; int64_t N; int m; int ldx; int vec_ok;
 };
 
+// 16-byte vectors per row of a staged chunk of the block row
+static constexpr int TRSM_NV = 32;
+
 template <class T, int UNIT>
 __global__ __launch_bounds__(256)
 void trsm_lunit_kernel(TrsmProb<T> P)
 {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::n;
+    constexpr int NV = TRSM_NV;
+    constexpr int KC = NV * VN;                         // columns per chunk
+    constexpr int LP = KC + VN;                         // padded chunk row pitch
+    constexpr int NU = 32 * NV / 256;                   // chunk vectors per thread
     const int64_t j0 = int64_t(blockIdx.x) * 8;
     if (j0 >= P.N) return;                              // uniform per workgroup, before any barrier
     const T* __restrict__ L = P.L;
@@ -163,7 +175,7 @@ void trsm_lunit_kernel(TrsmProb<T> P)
     const int m = P.m, ldx = P.ldx;
     extern __shared__ __align__(16) unsigned char trsm_lds[];
     T* Xs = reinterpret_cast<T*>(trsm_lds);            // column c at Xs + c*ldx
-    T* Ls = Xs + 8 * ldx;                               // 32 x 33 strictly lower diagonal block
+    T* Lc = Xs + 8 * ldx;                               // 32 x KC chunk of the block row, pitch LP
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = lane & 31;
     const bool upper = lane >= 32;
@@ -175,39 +187,69 @@ void trsm_lunit_kernel(TrsmProb<T> P)
             Xs[c * ldx + i] = cv ? B[int64_t(i) * ldb + j0 + c] : T(0);
     }
     T* Xc = Xs + cl * ldx;
-    for (int p0 = 0; p0 < m; p0 += 32) {
-        for (int e = tid; e < 32 * 32; e += 256) {
-            const int a = e >> 5, b = e & 31;
-            Ls[a * 33 + b] = (b < a && p0 + a < m) ? L[int64_t(p0 + a) * ldl + p0 + b] : T(0);
-        }
-        __syncthreads();
-        const int i = p0 + r;
-        T y = T(0);
-        if (i < m) {
-            const T* Li = L + int64_t(i) * ldl;
-            T acc[VN];
+    const V* Lr = reinterpret_cast<const V*>(Lc + r * LP);
+    // chunk (q0, kc): rows [q0, q0 + 32), columns [kc, kc + KC), strictly-lower
+    // entries only (zeros elsewhere), into registers
+    V pv[NU];
+    auto fetch = [&](int q0, int kc) {
+        #pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int e = tid + 256 * u;
+            const int col = kc + (e % NV) * VN, row = q0 + e / NV;
             #pragma unroll
-            for (int e = 0; e < VN; ++e) acc[e] = T(0);
-            if (P.vec_ok) {
-                #pragma unroll 8
-                for (int k = 0; k < p0; k += VN) {
-                    const V l = *reinterpret_cast<const V*>(Li + k);
-                    const V x = *reinterpret_cast<const V*>(Xc + k);
+            for (int t = 0; t < VN; ++t) pv[u][t] = T(0);
+            if (row < m && col < row) {
+                const T* src = L + int64_t(row) * ldl + col;
+                if (P.vec_ok && col + VN <= row) pv[u] = *reinterpret_cast<const V*>(src);
+                else {
                     #pragma unroll
-                    for (int e = 0; e < VN; ++e) acc[e] += l[e] * x[e];
+                    for (int t = 0; t < VN; ++t) if (col + t < row) pv[u][t] = src[t];
                 }
             }
-            else {
-                for (int k = 0; k < p0; ++k) acc[0] += Li[k] * Xc[k];
+        }
+    };
+    fetch(0, 0);
+    for (int p0 = 0; p0 < m; p0 += 32) {
+        const int i = p0 + r;
+        T acc[VN];
+        #pragma unroll
+        for (int e = 0; e < VN; ++e) acc[e] = T(0);
+        T lr[32];
+        // chunks cover columns [0, p0 + 32); the last one holds the diagonal block
+        for (int kc = 0; kc < p0 + 32; kc += KC) {
+            __syncthreads();                            // previous chunk consumed, X rows written
+            #pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = tid + 256 * u;
+                *reinterpret_cast<V*>(Lc + (e / NV) * LP + (e % NV) * VN) = pv[u];
             }
+            __syncthreads();
+            {
+                int np0 = p0, nkc = kc + KC;
+                if (nkc >= p0 + 32) { np0 = p0 + 32; nkc = 0; }
+                if (np0 < m) fetch(np0, nkc);           // in flight during the products below
+            }
+            const int kdot = (p0 - kc < KC ? p0 - kc : KC) / VN;   // vectors left of the diagonal block
+            #pragma unroll 8
+            for (int v = 0; v < kdot; ++v) {
+                const V l = Lr[v];
+                const V x = *reinterpret_cast<const V*>(Xc + kc + v * VN);
+                #pragma unroll
+                for (int e = 0; e < VN; ++e) acc[e] += l[e] * x[e];
+            }
+            if (p0 + 32 <= kc + KC) {
+                const T* d = Lc + r * LP + (p0 - kc);
+                #pragma unroll
+                for (int k = 0; k < 32; ++k) lr[k] = d[k];
+            }
+        }
+        T y = T(0);
+        if (i < m) {
             T sum = acc[0];
             #pragma unroll
             for (int e = 1; e < VN; ++e) sum += acc[e];
             y = Xc[i] - sum;
         }
-        T lr[32];
-        #pragma unroll
-        for (int k = 0; k < 32; ++k) lr[k] = Ls[r * 33 + k];
         const T diag = (!UNIT && i < m) ? L[int64_t(i) * ldl + i] : T(1);
         #pragma unroll
         for (int k = 0; k < 31; ++k) {
@@ -217,8 +259,8 @@ void trsm_lunit_kernel(TrsmProb<T> P)
         }
         if (!UNIT && r == 31) y = y / diag;
         if (i < m) Xc[i] = y;
-        __syncthreads();
     }
+    __syncthreads();
     {
         const int c = tid & 7;
         if (j0 + c < N)
@@ -238,7 +280,7 @@ static TrsmProb<T> trsm_prob(const T* L, int64_t ldl, int64_t m, T* B, int64_t l
     q.L = L; q.ldl = ldl; q.B = B; q.ldb = ldb; q.N = N; q.m = int(m);
     q.ldx = int(ceildiv(m, 32) * 32) + VN;
     q.vec_ok = (reinterpret_cast<uintptr_t>(L) % 16 == 0 && ldl % VN == 0) ? 1 : 0;
-    lds = std::max(lds, sizeof(T) * (size_t(8) * size_t(q.ldx) + 32 * 33));
+    lds = std::max(lds, sizeof(T) * (size_t(8) * size_t(q.ldx) + 32 * size_t(TRSM_NV * VN + VN)));
     return q;
 }
 
