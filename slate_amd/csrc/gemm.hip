@@ -584,10 +584,105 @@ void gemm_f32_kernel(GemmArgs p)
             }
 }
 
+//------------------------------------------------------------------------------
+// Narrow products (A_KC / B_NC, N <= 64, K <= 64, all of C): the recursive
+// panel's lowest splits (a tall M x 32 or M x 64 block of L times the square
+// block of U beside it, ~1500 launches per factorization) and the small
+// off-diagonal blocks of the recursive triangular solve.  On the 128 x 128
+// MFMA tile each workgroup stages one slice, waits and writes, and such a
+// launch moves ~0.14 TB/s (M = 40000, N = K = 32: ~110 us on 32 CUs).  Here B
+// sits in LDS, each thread reads its own row of A (the lanes of one row share
+// its addresses) and owns 16 bytes of C, and each element of C sums its K
+// products in k order on the VALU.
+template <class T>
+__global__ __launch_bounds__(256)
+void gemm_narrow_kernel(GemmArgs p)
+{
+    constexpr int VN = 16 / int(sizeof(T));
+    constexpr int BP = 64 + VN;                        // padded row pitch of B in LDS
+    typedef T vec_t __attribute__((ext_vector_type(VN)));
+    __shared__ __align__(16) T Bs[64 * BP];
+    const int64_t M = p.M;
+    const int N = int(p.N), K = int(p.K);
+    const T* __restrict__ A = static_cast<const T*>(p.A);
+    const T* __restrict__ B = static_cast<const T*>(p.B);
+    T* __restrict__ C = static_cast<T*>(p.C);
+    const int64_t lda = p.lda, ldb = p.ldb, ldc = p.ldc;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < K * 64; e += 256) {
+        const int k = e >> 6, n = e & 63;
+        Bs[k * BP + n] = n < N ? B[int64_t(k) * ldb + n] : T(0);
+    }
+    __syncthreads();
+    const int tpr = (N + VN - 1) / VN;                 // threads per row of C
+    const int rw = 256 / tpr;                          // rows per workgroup
+    const int lr = tid / tpr, g = tid - lr * tpr;
+    const int64_t i = int64_t(blockIdx.x) * rw + lr;
+    if (lr >= rw || i >= M) return;                    // no barrier below
+    const T* Ai = A + i * lda;
+    const T* Bg = Bs + g * VN;
+    T acc[VN];
+    #pragma unroll
+    for (int e = 0; e < VN; ++e) acc[e] = T(0);
+    int k = 0;
+    if (reinterpret_cast<uintptr_t>(A) % 16 == 0 && lda % VN == 0) {
+        for (; k + 4 * VN <= K; k += 4 * VN) {
+            vec_t av[4];
+            #pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = *reinterpret_cast<const vec_t*>(Ai + k + u * VN);
+            #pragma unroll
+            for (int u = 0; u < 4; ++u)
+                #pragma unroll
+                for (int t = 0; t < VN; ++t) {
+                    const vec_t b = *reinterpret_cast<const vec_t*>(Bg + (k + u * VN + t) * BP);
+                    #pragma unroll
+                    for (int e = 0; e < VN; ++e) acc[e] += av[u][t] * b[e];
+                }
+        }
+    }
+    for (; k < K; ++k) {
+        const T a = Ai[k];
+        const vec_t b = *reinterpret_cast<const vec_t*>(Bg + k * BP);
+        #pragma unroll
+        for (int e = 0; e < VN; ++e) acc[e] += a * b[e];
+    }
+    const T alpha = T(p.alpha), beta = T(p.beta);
+    const int c0 = g * VN;
+    T* Ci = C + i * ldc + c0;
+    if (c0 + VN <= N && reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % VN == 0) {
+        vec_t v;
+        if (beta != T(0)) {
+            const vec_t cv = *reinterpret_cast<const vec_t*>(Ci);
+            #pragma unroll
+            for (int e = 0; e < VN; ++e) v[e] = alpha * acc[e] + beta * cv[e];
+        }
+        else {
+            #pragma unroll
+            for (int e = 0; e < VN; ++e) v[e] = alpha * acc[e];
+        }
+        *reinterpret_cast<vec_t*>(Ci) = v;
+    }
+    else {
+        #pragma unroll
+        for (int e = 0; e < VN; ++e)
+            if (c0 + e < N) Ci[e] = beta != T(0) ? alpha * acc[e] + beta * Ci[e] : alpha * acc[e];
+    }
+}
+
 template <class T, int ALAY, int BLAY, int ROLE = 0>
 static void launch(const GemmArgs& a, hipStream_t s)
 {
     if (a.M <= 0 || a.N <= 0) return;
+    if constexpr (ALAY == A_KC && BLAY == B_NC && ROLE == 0) {
+        if (a.uplo == 0 && a.N <= 64 && a.K <= 64) {
+            const int64_t rw = 256 / ((a.N + 16 / int64_t(sizeof(T)) - 1) / (16 / int64_t(sizeof(T))));
+            const int64_t nwg = ceildiv(a.M, rw);
+            SA_REQUIRE(nwg < (int64_t(1) << 31), -2, "gemm: grid too large");
+            hipLaunchKernelGGL((gemm_narrow_kernel<T>), dim3(unsigned(nwg)), dim3(256), 0, s, a);
+            SA_CHECK_LAUNCH();
+            return;
+        }
+    }
     // 128 x 128 tiles, 4 waves, two workgroups per CU (2 waves per SIMD) —
     // the fastest of the tilings measured for both FP64 and FP32.
     constexpr int BM = 128, BN = 128, BK = (sizeof(T) == 8) ? 16 : 32;
