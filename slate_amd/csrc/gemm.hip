@@ -621,17 +621,23 @@ void gemm_narrow_kernel(GemmArgs p)
     if (lr >= rw || i >= M) return;                    // no barrier below
     const T* Ai = A + i * lda;
     const T* Bg = Bs + g * VN;
+    const T alpha = T(p.alpha), beta = T(p.beta);
+    const int c0 = g * VN;
+    T* Ci = C + i * ldc + c0;
+    const bool cvec = c0 + VN <= N && reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % VN == 0;
+    vec_t cv;                                          // C read before the products: one round trip less
+    if (cvec && beta != T(0)) cv = *reinterpret_cast<const vec_t*>(Ci);
     T acc[VN];
     #pragma unroll
     for (int e = 0; e < VN; ++e) acc[e] = T(0);
     int k = 0;
     if (reinterpret_cast<uintptr_t>(A) % 16 == 0 && lda % VN == 0) {
-        for (; k + 4 * VN <= K; k += 4 * VN) {
-            vec_t av[4];
+        for (; k + 8 * VN <= K; k += 8 * VN) {
+            vec_t av[8];
             #pragma unroll
-            for (int u = 0; u < 4; ++u) av[u] = *reinterpret_cast<const vec_t*>(Ai + k + u * VN);
+            for (int u = 0; u < 8; ++u) av[u] = *reinterpret_cast<const vec_t*>(Ai + k + u * VN);
             #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 8; ++u)
                 #pragma unroll
                 for (int t = 0; t < VN; ++t) {
                     const vec_t b = *reinterpret_cast<const vec_t*>(Bg + (k + u * VN + t) * BP);
@@ -646,13 +652,9 @@ void gemm_narrow_kernel(GemmArgs p)
         #pragma unroll
         for (int e = 0; e < VN; ++e) acc[e] += a * b[e];
     }
-    const T alpha = T(p.alpha), beta = T(p.beta);
-    const int c0 = g * VN;
-    T* Ci = C + i * ldc + c0;
-    if (c0 + VN <= N && reinterpret_cast<uintptr_t>(C) % 16 == 0 && ldc % VN == 0) {
+    if (cvec) {
         vec_t v;
         if (beta != T(0)) {
-            const vec_t cv = *reinterpret_cast<const vec_t*>(Ci);
             #pragma unroll
             for (int e = 0; e < VN; ++e) v[e] = alpha * acc[e] + beta * cv[e];
         }
