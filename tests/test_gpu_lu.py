@@ -31,7 +31,11 @@ def test_generate_bitwise_equals_oracle(gpu):
     np.testing.assert_allclose(to_host(A), oracle.randn(3, 300, 200), rtol=1e-13, atol=1e-14)
 
 
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (128, 128, 16), (300, 257, 129), (1000, 513, 700)])
+# (32, 4097, 32) ... (48, 2000, 40): C has at most 64 rows and k <= 64, the
+# narrow VALU kernel of the panel's lowest splits (16-byte vectors, odd
+# leading dimensions on the element path, k tails after the 8-vector batches)
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (128, 128, 16), (300, 257, 129), (1000, 513, 700),
+                                   (32, 4097, 32), (64, 3001, 64), (33, 777, 17), (48, 2000, 40)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_gemm_matches_reference(gpu, m, n, k, dtype):
     say("gemm %dx%dx%d %s" % (m, n, k, dtype))
@@ -42,6 +46,21 @@ def test_gemm_matches_reference(gpu, m, n, k, dtype):
     ref = 0.5 * C - A @ B
     tol = 1e-13 if dtype == torch.float64 else 2e-5
     assert np.abs(to_host(dC) - ref).max() <= tol * max(1.0, k) ** 0.5 * 4
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gemm_beta_zero_does_not_read_c(gpu, dtype):
+    # beta = 0: C is written, never read (NaN in C must not reach the result),
+    # on the narrow kernel (m = 32, k = 32) and the MFMA tile (m = 200)
+    for m, n, k in ((32, 1500, 32), (200, 300, 48)):
+        say("gemm beta=0 %dx%dx%d %s" % (m, n, k, dtype))
+        rng = np.random.default_rng(m + n + k)
+        A, B = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+        dA, dB = to_dev(A, dtype), to_dev(B, dtype)
+        dC = to_dev(np.full((m, n), np.nan), dtype)
+        slate_amd.gemm(2.0, dA, dB, 0.0, dC)
+        tol = 1e-13 if dtype == torch.float64 else 2e-5
+        assert np.abs(to_host(dC) - 2.0 * (A @ B)).max() <= tol * k ** 0.5 * 8
 
 
 def test_getrf_n2000_pivots_equal_lapack_and_oracle(gpu):
